@@ -1,0 +1,294 @@
+/*
+ * sevc.h -- C ABI of the MI355X-native vertex-model time step.
+ *
+ * Drop-in boundary for the body of the reference's main loop
+ * (src/main.cu:120-394).  The reference keeps its state in three device
+ * arrays of pointer-linked structs (geometry.h:39-86) built by
+ * host_to_device (translation.h:140-155) from the loader output
+ * (io.h:66-264), and drives it from struct options (options.h:15-46).
+ * This ABI keeps the same options and the same vertex / boundary / grain
+ * arrays -- same slots, same ids (id == slot), same per-vertex boundary
+ * and grain order, same per-grain vertex order -- laid out as
+ * structure-of-arrays with int32 indices instead of pointers.
+ *
+ * Errors are returned as negative codes (never exit(), unlike
+ * HERR/CERR/CNULL in utils.h:17-44); sevc_last_error() gives the text.
+ * One state per host thread; distinct states may be used concurrently.
+ */
+#ifndef SEVC_H
+#define SEVC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- */
+/* Status codes                                                       */
+/* ---------------------------------------------------------------- */
+#define SEVC_OK              0
+#define SEVC_E_INVALID_ARG  -1   /* bad argument / option                      */
+#define SEVC_E_IO           -2   /* file missing or unreadable (io.h:78-81)    */
+#define SEVC_E_TOPOLOGY     -3   /* illegal grains / count mismatch (io.h:155-186) */
+#define SEVC_E_HIP          -4   /* HIP runtime error                          */
+#define SEVC_E_NOMEM        -5   /* allocation failed                          */
+#define SEVC_E_GRAIN_FULL   -6   /* grain list would exceed max_vrt_per_grn    */
+#define SEVC_E_STATE        -7   /* inconsistent topology met during an event  */
+#define SEVC_E_NO_DEVICE    -8   /* no HIP device                              */
+
+/* Reference fixed-size limits (geometry.h:9-11). */
+#define SEVC_REF_MAX_VRT_PER_GRN 100
+#define SEVC_MAX_T_EXT 100000.0
+#define SEVC_GRAD_EPS 1e-8          /* calculus.h:7 */
+
+/* Nucleation models. */
+#define SEVC_NUCL_MC   0   /* reference: one Metropolis site per gap (nucleation2.cu:239-443) */
+#define SEVC_NUCL_RATE 1   /* extension: per-grain probability rate*SE per step              */
+
+/* Velocity models. */
+#define SEVC_VEL_FD       0   /* reference forward difference (calculus.h:19-73,143-154) */
+#define SEVC_VEL_ANALYTIC 1   /* extension: exact -grad E (smooth, used by the adjoint)  */
+
+#define SEVC_FNAME 256
+
+/* ---------------------------------------------------------------- */
+/* Options: every key of options.h:15-46 plus extensions.            */
+/* ---------------------------------------------------------------- */
+typedef struct sevc_options {
+    /* --- reference keys (options.h:15-46, defaults options.h:71-85) --- */
+    double dt;                 /* fixed dt, or the upper bound when adaptive_dt */
+    double dtau;               /* nucleation period; nucleation_gap = (int)(dtau/dt) */
+    int32_t nucleation_gap;    /* derived (options.h:172)                      */
+    double tmax;               /* -1 = unset                                    */
+    int32_t nmax;              /* -1 = unset                                    */
+    int32_t snap_gap;
+    double leave_percent;      /* stop when n_grains <= floor(leave_percent*n_grains0) */
+    int32_t n_grains;          /* expected initial grain count (io.h:182)       */
+    char vertex_fname[SEVC_FNAME];
+    char boundary_fname[SEVC_FNAME];
+    char orientation_fname[SEVC_FNAME];
+    char SE_fname[SEVC_FNAME];
+    double GB_eps;
+    double GB_scaling;
+    double SE_eps;
+    double SE_min;
+    double SE_max;
+    double nucleation_scaling;
+    double domain_bound;
+    int32_t do_nucleation;
+    double MC_k;
+    int32_t n_blocks;          /* accepted and ignored: launch shapes are the library's */
+    int32_t n_threads;
+    int32_t debug_lvl;
+    char output_folder[SEVC_FNAME];
+
+    /* --- extensions (absent from the reference; defaults reproduce it) --- */
+    double mobility;           /* mu, multiplies -dE/eps (1.0)                  */
+    int32_t adaptive_dt;       /* 0: fixed dt (reference)                       */
+    double dt_safety;          /* dt_n = clamp(dt_safety * tau_min, dt_min, dt) */
+    double dt_min;
+    double flip_threshold;     /* edges shorter than this that shrink also flip (0 = off) */
+    int32_t nucleation_mode;   /* SEVC_NUCL_MC / SEVC_NUCL_RATE                 */
+    double nucleation_rate;    /* RATE: p_g = nucleation_rate * SE_g per step   */
+    uint64_t seed;             /* Philox key for nucleation                     */
+    int32_t max_vrt_per_grn;   /* grain list stride (capacity per grain)        */
+    double extra_capacity;     /* spare disabled slots appended, fraction of size */
+    int32_t velocity_model;    /* SEVC_VEL_FD / SEVC_VEL_ANALYTIC               */
+    int32_t events;            /* 0: topology-fixed (no T1/T2/nucleation)       */
+} sevc_options;
+
+/* ---------------------------------------------------------------- */
+/* Host view of one microstructure (structure of arrays).            */
+/* Slot i holds the entity with id i, as in the reference.           */
+/* ---------------------------------------------------------------- */
+typedef struct sevc_mesh {
+    int32_t n_vertices;        /* slots, enabled or not */
+    int32_t n_boundaries;
+    int32_t n_grains;
+    int32_t max_vrt_per_grn;   /* stride of g_vtx */
+    double domain_bound;
+
+    /* vertices (geometry.h:39-59) */
+    double*  pos;              /* 2*n_vertices, x y interleaved */
+    double*  vel;              /* 2*n_vertices                  */
+    int32_t* v_bnd;            /* 3*n_vertices, order as vertex.boundaries[3] */
+    int32_t* v_grn;            /* 3*n_vertices, order as vertex.grains[3]     */
+    uint8_t* v_enabled;
+    double*  v_energy;         /* optional (NULL): vertex energy diagnostic    */
+    double*  v_vel_parts;      /* optional (NULL): 4*n_vertices vel_bnd_x vel_grn_x vel_bnd_y vel_grn_y */
+
+    /* boundaries (geometry.h:61-72) */
+    int32_t* b_vtx;            /* 2*n_boundaries: ini, end */
+    double*  b_energy;
+    double*  b_arclength;
+    double*  b_t_ext;
+    uint8_t* b_enabled;
+    uint8_t* b_candidate;
+
+    /* grains (geometry.h:74-86) */
+    int32_t* g_vtx;            /* n_grains*max_vrt_per_grn, -1 padded */
+    int32_t* g_vlen;
+    double*  g_area;
+    double*  g_dAdt;           /* optional (NULL) */
+    double*  g_orientation;
+    double*  g_SE;
+    uint8_t* g_enabled;
+    uint8_t* g_type;           /* 0 COMMON, 1 NUCLEATED (geometry.h:19) */
+} sevc_mesh;
+
+/* Per-call statistics of sevc_step (the printf lines of main.cu:122,
+ * 231, 324-325 become fields). */
+typedef struct sevc_step_stats {
+    int32_t steps_done;        /* steps executed by this call               */
+    int32_t n;                 /* step counter after the call (main.cu:388) */
+    double  t;                 /* simulation time after the call            */
+    double  dt_last;           /* dt of the last executed step              */
+    int32_t n_grains;          /* enabled grains (main.cu:317)              */
+    int32_t n_vertices;        /* enabled vertices                          */
+    double  nucleated_area_pct;/* main.cu:321                               */
+    int32_t done;              /* stop rule reached                         */
+    int32_t status;            /* SEVC_OK or the first device-side error    */
+    int64_t n_flips;           /* T1 events during the call                 */
+    int64_t n_removals;        /* T2 events                                 */
+    int64_t n_two_grain;       /* two-triangle collapses                    */
+    int64_t n_nucleations;
+    int64_t n_vertex_steps;    /* sum over steps of enabled vertices        */
+} sevc_step_stats;
+
+typedef struct sevc_state sevc_state;
+
+/* ---------------------------------------------------------------- */
+/* Options                                                           */
+/* ---------------------------------------------------------------- */
+
+/* Defaults of load_config_file (options.h:71-85) plus extension defaults. */
+void sevc_options_default(sevc_options* o);
+
+/* Parse an .ini file with the semantics of load_config_file
+ * (options.h:69-206): '#'/'[' lines skipped, exact "key=value" match,
+ * do_nucleation true only for "true", nucleation_gap = (int)(dtau/dt),
+ * tmax/nmax defaulting (options.h:173-181).  Unknown keys are ignored;
+ * the extension keys are accepted too. */
+int sevc_options_from_ini(const char* path, sevc_options* o);
+
+/* ---------------------------------------------------------------- */
+/* Host meshes                                                       */
+/* ---------------------------------------------------------------- */
+
+/* Allocate every array of a host mesh (calloc, -1 in index arrays). */
+int  sevc_mesh_alloc(sevc_mesh* m, int32_t n_vertices, int32_t n_boundaries,
+                     int32_t n_grains, int32_t max_vrt_per_grn);
+void sevc_mesh_free(sevc_mesh* m);
+
+/* Build the initial mesh from the reference's input files with the ids
+ * and orderings of load_from_files (io.h:66-264): vertices in file order,
+ * boundaries in file order, grains by the face walk (io.h:137-180), then
+ * counter-clockwise boundary order (io.h:188-192), id-sorted vertex grain
+ * lists (io.h:196-200), orientations * 2pi (io.h:214-218) and the SE
+ * transform (io.h:223-248).  Positions are scaled by domain_bound. */
+int sevc_mesh_load_files(const sevc_options* o, sevc_mesh* out);
+
+/* Same, from arrays already in memory (positions in [0,1)^2, ridges as
+ * 2*n_ridges vertex pairs, one orientation and one SE value per grain). */
+int sevc_mesh_build(const sevc_options* o, int32_t n_vertices, const double* unit_pos,
+                    int32_t n_ridges, const int32_t* ridges, int32_t n_grains,
+                    const double* orientations01, const double* se_raw, sevc_mesh* out);
+
+/* Periodic Voronoi tessellation of n_seeds uniform seeds on the unit
+ * torus (the role of voronoi/voronoi.py:12-109), written as the arrays
+ * sevc_mesh_build takes.  Seeds are ordered along a space-filling curve so
+ * that neighbouring grains get neighbouring ids.  Arrays are malloc'd;
+ * free them with sevc_free. */
+int sevc_voronoi_generate(int32_t n_seeds, uint64_t seed,
+                          int32_t* n_vertices, double** unit_pos,
+                          int32_t* n_ridges, int32_t** ridges);
+void sevc_free(void* p);
+
+/* Host port of check_data (utils.h:85-336).  Returns the number of
+ * violations found (0 = valid) and writes the summed grain area. */
+int sevc_check_mesh(const sevc_mesh* m, double* total_area);
+
+/* Snapshot in the format of export_data (io.h:280-377) to
+ * <folder>/<index as %06d>.txt. */
+int sevc_export_snapshot(const sevc_mesh* m, const sevc_options* o, const char* folder,
+                         int32_t index, double time);
+
+/* ---------------------------------------------------------------- */
+/* Device state                                                      */
+/* ---------------------------------------------------------------- */
+
+/* Upload one mesh (host_to_device, translation.h:140-155; scratch of
+ * main.cu:68-107).  The library copies; the caller keeps ownership. */
+int sevc_create(const sevc_options* o, const sevc_mesh* m, sevc_state** out);
+
+/* Upload n_members independent meshes as one batched state: every kernel
+ * of a step covers all members in one launch. */
+int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t n_members,
+                      sevc_state** out);
+
+/* Run up to n_steps steps of main.cu:124-389 on every active member;
+ * a member stops at its own stop rule (main.cu:120,391).  stats may be
+ * NULL; for batches it receives member 0's statistics. */
+int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* stats);
+
+/* Same, capturing one step into a HIP graph on first use and replaying
+ * it (no host synchronisation inside the steps). */
+int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* stats);
+
+/* Per-member statistics after the last call. */
+int sevc_member_stats(sevc_state* s, int32_t member, sevc_step_stats* stats);
+
+/* Slot counts of a member (they include spare capacity). */
+int sevc_member_sizes(const sevc_state* s, int32_t member, int32_t* n_vertices,
+                      int32_t* n_boundaries, int32_t* n_grains, int32_t* max_vrt_per_grn);
+int32_t sevc_n_members(const sevc_state* s);
+
+/* Copy a member back to host arrays sized by sevc_member_sizes
+ * (device_to_host, translation.h:184-199, without freeing the device copy). */
+int sevc_get_mesh(sevc_state* s, int32_t member, sevc_mesh* out);
+
+/* Replace the positions / stored energies of a member (adjoint driver,
+ * parameter sweeps).  NULL arrays are left unchanged. */
+int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
+                           const double* g_SE, double gb_scaling, double mobility);
+
+/* Compute the diagnostics of the next step (vertex energies,
+ * dA/dt and velocity parts, calculus.h:123-133,243-253,62-69). */
+int sevc_request_diagnostics(sevc_state* s);
+
+/* Adjoint over a topology-fixed window (extension).  Runs `steps` steps of
+ * velocity+translation with the analytic velocity from the current state,
+ * checkpointing every ckpt_every steps, evaluates the area-histogram loss
+ * against `target_hist` (n_bins values, smooth linear binning of A/<A> on
+ * [0, hist_max)) and returns the loss and its gradients with respect to
+ * the global boundary-energy scale, the mobility and every grain's stored
+ * energy (g_SE_grad has n_grains slots of member 0). */
+int sevc_adjoint_window(sevc_state* s, int32_t steps, int32_t ckpt_every, double dt,
+                        int32_t n_bins, double hist_max, const double* target_hist,
+                        double* loss, double* d_gamma, double* d_mobility, double* g_SE_grad);
+
+/* Area histogram of member 0 with the binning used by the adjoint loss. */
+int sevc_area_histogram(sevc_state* s, int32_t n_bins, double hist_max, double* hist);
+
+/* Device time (ms) of the kernels of the last call, by name; returns the
+ * number of entries written.  Used by the benchmark for roofline figures. */
+int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], double* ms,
+                      int64_t* launches);
+int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
+
+/* Device self-test of the shared arithmetic: evaluates op on n inputs on
+ * the GPU (0 atan2(y,x), 1 sin(x), 2 cos(x), 3 exp(x), 4 x/y, 5 sqrt(x),
+ * 6 |(x,y)| as geometry.h:217, 7 wrap_dist(x,y,D=1), 8 Philox word 0 of
+ * counter (x,y,0,0) key (1,2)).  Lets tests compare device and host bits. */
+int sevc_selftest_math(int32_t op, int32_t n, const double* x, const double* y, double* out);
+
+void sevc_destroy(sevc_state* s);
+const char* sevc_last_error(const sevc_state* s);
+const char* sevc_strerror(int code);
+int sevc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEVC_H */

@@ -1,0 +1,73 @@
+/*
+ * sevc_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial FP64 CPU restatement of the reference's vertex-model time step
+ * (src/main.cu:120-394 and the kernels it calls).  It is the checker for
+ * the HIP implementation: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product links
+ * or calls it.
+ *
+ * The state uses the sevc_mesh layout of include/sevc.h so that a GPU
+ * state and an oracle state can be compared slot by slot.
+ */
+#ifndef SEVC_ORACLE_H
+#define SEVC_ORACLE_H
+
+#include "../include/sevc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+/* Deep copy of an initial mesh. */
+orc* orc_create(const sevc_options* o, const sevc_mesh* m);
+void orc_destroy(orc* s);
+
+/* Run up to n_steps steps of main.cu:124-389. */
+int orc_step(orc* s, int32_t n_steps, sevc_step_stats* st);
+
+/* Copy the current state into caller arrays sized like the input mesh. */
+int orc_get_mesh(const orc* s, sevc_mesh* out);
+
+/* Compute the per-step quantities of main.cu:124-134,200-206 only
+ * (arclengths, areas, energies, velocities, t_ext) without moving or
+ * changing topology.  For unit tests of the velocity. */
+int orc_compute_velocities(orc* s);
+
+/* Request the export diagnostics (vertex energy, dA/dt, velocity parts)
+ * on the next step. */
+void orc_request_diagnostics(orc* s);
+
+/* Loader restatement of io.h:66-264 (independent of the product loader). */
+int orc_load_files(const sevc_options* o, sevc_mesh* out);
+int orc_build_mesh(const sevc_options* o, int32_t nv, const double* unit_pos, int32_t nr,
+                   const int32_t* ridges, int32_t ng, const double* ori01, const double* se_raw,
+                   sevc_mesh* out);
+
+/* Host mesh helpers for the oracle loader output. */
+int  orc_mesh_alloc(sevc_mesh* m, int32_t nv, int32_t ne, int32_t ng, int32_t S);
+void orc_mesh_free(sevc_mesh* m);
+
+/* check_data restatement (utils.h:85-336): number of violations. */
+int orc_check_mesh(const sevc_mesh* m, double* total_area);
+
+/* Geometry primitives exposed for closed-form tests. */
+double orc_wrap_dist(double dest, double orig, double D);     /* geometry.h:147 */
+double orc_dom_mod(double a, double D);                       /* geometry.h:133 */
+double orc_mag(double x, double y);                           /* geometry.h:217 */
+double orc_polygon_area(const double* x, const double* y, int32_t n, double D);  /* geometry.h:757 */
+double orc_sum_chunked(const double* v, const uint8_t* mask, int32_t n);
+
+/* Elementary functions of include/sevc_math.h compiled on the host. */
+double orc_atan2(double y, double x);
+double orc_sin(double x);
+double orc_cos(double x);
+double orc_exp(double x);
+void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,629 @@
+// sevc_api.hip -- C ABI of the device state: creation (upload with global
+// slot ids), the per-step launch sequence, statistics and download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sevc_state.h"
+#include "../../include/sevc.h"
+
+namespace sevc {
+// kernels of sevc_kernels.hip
+__global__ void k_begin(Dev d);
+__global__ void k_count_grains(Dev d);
+__global__ void k_count_vertices(Dev d);
+__global__ void k_end_prepare(Dev d);
+__global__ void k_end(Dev d);
+__global__ void k_edge_len_energy(Dev d, int only_nucl);
+__global__ void k_grain_area(Dev d, int only_nucl);
+__global__ void k_vertex_energy(Dev d);
+__global__ void k_velocity(Dev d);
+__global__ void k_grain_dadt(Dev d);
+__global__ void k_translate(Dev d);
+__global__ void k_text(Dev d);
+__global__ void k_label(Dev d);
+__global__ void k_poll(Dev d);
+__global__ void k_fix(Dev d, int stage);
+__global__ void k_fix_clear(Dev d, int stage, int with_area);
+__global__ void k_area_list(Dev d, int which);
+__global__ void k_two_scan(Dev d);
+__global__ void k_two_grain(Dev d);
+__global__ void k_label3(Dev d);
+__global__ void k_t2_split(Dev d);
+__global__ void k_t2_par(Dev d);
+__global__ void k_t2_serial(Dev d);
+__global__ void k_st2(Dev d);
+__global__ void k_flip_split(Dev d);
+__global__ void k_flip_par(Dev d);
+__global__ void k_flip_serial(Dev d);
+__global__ void k_nucl_area_l1(Dev d);
+__global__ void k_nucl_area_up(Dev d);
+__global__ void k_nucl_factor(Dev d);
+__global__ void k_rate_cand(Dev d);
+__global__ void k_block_counts(Dev d, int kind);
+__global__ void k_nucl_manager(Dev d);
+}  // namespace sevc
+
+using namespace sevc;
+
+struct KTimer {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct sevc_state {
+    int B = 0;
+    std::vector<sevc_options> opts;
+    std::vector<int> nv, ne, ng, voff, eoff, goff;
+    int V = 0, E = 0, G = 0, S = 0;
+    Dev d{};
+    std::vector<void*> allocs;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool any_nucl = false, any_events = false;
+    bool diag_next = false;
+    std::vector<MState> ms_host, ms_prev;
+    // graph replay
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    // kernel timing
+    bool timing = false;
+    std::vector<KTimer> pending;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::map<std::string, std::pair<double, long long>> ktimes;
+};
+
+namespace {
+
+int fail(sevc_state* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(s, SEVC_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(sevc_state* s, T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(s, SEVC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return SEVC_OK;
+}
+
+hipEvent_t take_event(sevc_state* s) {
+    if (s->pool_used == s->pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        s->pool.push_back(e);
+    }
+    return s->pool[s->pool_used++];
+}
+
+// Launch helper: optional per-kernel device timing on the state's stream.
+template <typename K, typename... Args>
+void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args... args) {
+    if (grid <= 0) return;
+    if (s->timing) {
+        KTimer t{name, take_event(s), take_event(s)};
+        hipEventRecord(t.a, s->stream);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, args...);
+        hipEventRecord(t.b, s->stream);
+        s->pending.push_back(t);
+    } else {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, args...);
+    }
+}
+
+void collect_timers(sevc_state* s) {
+    for (auto& t : s->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        auto& e = s->ktimes[t.name];
+        e.first += ms;
+        e.second += 1;
+    }
+    s->pending.clear();
+    s->pool_used = 0;
+}
+
+constexpr int kListGrid = 512;
+
+// One time step of every active member (main.cu:124-389).
+void launch_step(sevc_state* s, bool diag) {
+    Dev d = s->d;
+    d.diag = diag ? 1 : 0;
+    const int nb = (s->B + 63) / 64;
+    launch(s, "begin", k_begin, nb, 64, d);
+    launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
+    launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
+    if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
+    if (s->any_nucl) {
+        launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
+        launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
+        launch(s, "block_counts", k_block_counts, d.nvb, kBlock, d, 0);
+        launch(s, "block_counts", k_block_counts, d.neb, kBlock, d, 1);
+        launch(s, "block_counts", k_block_counts, d.ngb, kBlock, d, 2);
+        launch(s, "nucl_manager", k_nucl_manager, s->B, 256, d);
+        launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 1);
+        launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 1);
+        if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
+    }
+    launch(s, "velocity", k_velocity, d.nvb, kBlock, d);
+    if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
+    launch(s, "text", k_text, d.neb, kBlock, d);
+    if (s->any_events) {
+        launch(s, "label", k_label, d.neb, kBlock, d);
+        launch(s, "poll", k_poll, 1, 1024, d);
+        launch(s, "two_scan", k_two_scan, d.ngb, kBlock, d);
+        launch(s, "two_grain", k_two_grain, 1, 64, d);
+        launch(s, "label3", k_label3, d.ngb, kBlock, d);
+        launch(s, "t2_split", k_t2_split, kListGrid, 256, d);
+        launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
+        launch(s, "t2_serial", k_t2_serial, 1, 64, d);
+        launch(s, "fix", k_fix, kListGrid, 256, d, 1);
+        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 1, 0);
+        launch(s, "area_list", k_area_list, kListGrid, 256, d, 0);
+        launch(s, "area_list", k_area_list, kListGrid, 256, d, 1);
+        launch(s, "st2", k_st2, kListGrid, 256, d);
+        launch(s, "flip_split", k_flip_split, kListGrid, 256, d);
+        launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
+        launch(s, "flip_serial", k_flip_serial, 1, 64, d);
+        launch(s, "fix", k_fix, kListGrid, 256, d, 2);
+        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 2, 0);
+    }
+    launch(s, "translate", k_translate, d.nvb, kBlock, d);
+    launch(s, "end_prepare", k_end_prepare, nb, 64, d);
+    launch(s, "count_grains", k_count_grains, d.ngb, kBlock, d);
+    launch(s, "count_vertices", k_count_vertices, d.nvb, kBlock, d);
+    launch(s, "end", k_end, nb, 64, d);
+}
+
+void build_table(std::vector<Blk>& tbl, std::vector<int>& first, const std::vector<int>& off,
+                 const std::vector<int>& n) {
+    tbl.clear();
+    first.assign(off.size() + 1, 0);
+    for (size_t m = 0; m < off.size(); ++m) {
+        first[m] = static_cast<int>(tbl.size());
+        for (int b = 0; b < n[m]; b += kBlock)
+            tbl.push_back(Blk{static_cast<int>(m), off[m] + b, off[m] + std::min(b + kBlock, n[m])});
+    }
+    first[off.size()] = static_cast<int>(tbl.size());
+}
+
+int fill_stats(sevc_state* s, int m, sevc_step_stats* st) {
+    if (!st) return SEVC_OK;
+    const MState& a = s->ms_host[static_cast<size_t>(m)];
+    const MState& p = s->ms_prev[static_cast<size_t>(m)];
+    std::memset(st, 0, sizeof(*st));
+    st->steps_done = static_cast<int32_t>(a.steps - p.steps);
+    st->n = a.n;
+    st->t = a.t;
+    st->dt_last = a.dt;
+    st->n_grains = a.curr_grains;
+    st->n_vertices = a.n_venabled;
+    st->nucleated_area_pct = a.nucl_area_pct;
+    st->done = a.done;
+    st->status = a.status;
+    st->n_flips = a.flips - p.flips;
+    st->n_removals = a.removals - p.removals;
+    st->n_two_grain = a.two - p.two;
+    st->n_nucleations = a.nucl - p.nucl;
+    st->n_vertex_steps = a.vsteps - p.vsteps;
+    return SEVC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sevc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* sevc_last_error(const sevc_state* s) { return s ? s->err.c_str() : "null state"; }
+
+int32_t sevc_n_members(const sevc_state* s) { return s ? s->B : 0; }
+
+int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t B, sevc_state** out) {
+    if (!opts || !meshes || B <= 0 || !out) return SEVC_E_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SEVC_E_NO_DEVICE;
+    sevc_state* s = new sevc_state();
+    s->B = B;
+    s->opts.assign(opts, opts + B);
+    s->S = 0;
+    for (int m = 0; m < B; ++m) {
+        const sevc_mesh& me = meshes[m];
+        if (me.n_vertices <= 0 || me.n_boundaries <= 0 || me.n_grains <= 0 || me.max_vrt_per_grn < 3 ||
+            !me.pos || !me.v_bnd || !me.v_grn || !me.v_enabled || !me.b_vtx || !me.b_enabled || !me.g_vtx ||
+            !me.g_vlen || !me.g_enabled || !me.g_SE || !me.g_orientation) {
+            delete s;
+            return SEVC_E_INVALID_ARG;
+        }
+        // Kernels index through every stored id of an enabled entity, so
+        // an inconsistent input is rejected here (utils.h:85-336 checks).
+        double total_area = 0.0;
+        if (sevc_check_mesh(&me, &total_area) != 0) {
+            delete s;
+            return SEVC_E_TOPOLOGY;
+        }
+        s->S = std::max(s->S, static_cast<int>(me.max_vrt_per_grn));
+        s->voff.push_back(s->V); s->eoff.push_back(s->E); s->goff.push_back(s->G);
+        s->nv.push_back(me.n_vertices); s->ne.push_back(me.n_boundaries); s->ng.push_back(me.n_grains);
+        s->V += me.n_vertices; s->E += me.n_boundaries; s->G += me.n_grains;
+        if (opts[m].dt <= 0) { delete s; return SEVC_E_INVALID_ARG; }
+        if (opts[m].events && opts[m].do_nucleation) s->any_nucl = true;
+        if (opts[m].events) s->any_events = true;
+    }
+    if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
+    const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
+    const size_t S = static_cast<size_t>(s->S);
+    Dev& d = s->d;
+    d.S = s->S;
+    d.B = B;
+    d.V = s->V;
+    d.E = s->E;
+    d.G = s->G;
+    int rc = SEVC_OK;
+#define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
+    DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
+    DA(d.venergy, V); DA(d.vparts, 4 * V);
+    DA(d.mfix, 4 * V); DA(d.mgrn, 4 * V); DA(d.mvrt, 4 * V); DA(d.mprev, 4 * V);
+    DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
+    DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
+    DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
+    DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
+    DA(d.cnt, C_COUNT);
+    DA(d.l_cand, E); DA(d.l_two, E); DA(d.l_t2, G); DA(d.l_t2p, G); DA(d.l_t2c, G);
+    DA(d.l_flipp, E); DA(d.l_flipc, E);
+    for (int k = 0; k < kFixStages; ++k) DA(d.l_fix[k], G);
+    DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
+    DA(d.nscr, 12 * G + 16);
+
+    // block tables
+    std::vector<Blk> vt, et, gt;
+    std::vector<int> vf, ef, gf;
+    build_table(vt, vf, s->voff, s->nv);
+    build_table(et, ef, s->eoff, s->ne);
+    build_table(gt, gf, s->goff, s->ng);
+    d.nvb = static_cast<int>(vt.size()); d.neb = static_cast<int>(et.size()); d.ngb = static_cast<int>(gt.size());
+    Blk *dvt, *det, *dgt;
+    DA(dvt, vt.size()); DA(det, et.size()); DA(dgt, gt.size());
+    DA(d.vblk_first, vf.size()); DA(d.eblk_first, ef.size()); DA(d.gblk_first, gf.size());
+    DA(d.bc_v, vt.size()); DA(d.bc_e, et.size()); DA(d.bc_g, gt.size());
+    DA(d.partial, 2 * gt.size() + 2);
+    d.vblk = dvt; d.eblk = det; d.gblk = dgt;
+    Params* dprm;
+    DA(dprm, static_cast<size_t>(B));
+    DA(d.ms, static_cast<size_t>(B));
+    d.prm = dprm;
+#undef DA
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        sevc_destroy(s);
+        return SEVC_E_HIP;
+    }
+#define UP(dst, src, n) \
+    do { if (hipMemcpy((dst), (src), (n), hipMemcpyHostToDevice) != hipSuccess) { sevc_destroy(s); return SEVC_E_HIP; } } while (0)
+    UP(dvt, vt.data(), vt.size() * sizeof(Blk));
+    UP(det, et.data(), et.size() * sizeof(Blk));
+    UP(dgt, gt.data(), gt.size() * sizeof(Blk));
+    UP(d.vblk_first, vf.data(), vf.size() * sizeof(int));
+    UP(d.eblk_first, ef.data(), ef.size() * sizeof(int));
+    UP(d.gblk_first, gf.data(), gf.size() * sizeof(int));
+
+    // host staging with global ids
+    std::vector<double> pos(2 * V), vel(2 * V, 0.0), ben(E, 0.0), blen(E, 0.0), bt(E, SEVC_MAX_T_EXT);
+    std::vector<double> garea(G, 0.0), gori(G, 0.0), gse(G, 0.0), venergy(V, 0.0), vparts(4 * V, 0.0), gdadt(G, 0.0);
+    std::vector<int> vb(3 * V), vg(3 * V), bvx(2 * E), gvx(G * S, -1), glen(G, 0);
+    std::vector<uint8_t> ven(V), benab(E), bcand(E, 0), genab(G), gtype(G, 0);
+    std::vector<Params> prm(static_cast<size_t>(B));
+    s->ms_host.assign(static_cast<size_t>(B), MState{});
+    auto gid = [](int x, int off) { return x < 0 ? -1 : x + off; };
+    for (int m = 0; m < B; ++m) {
+        const sevc_mesh& me = meshes[m];
+        const sevc_options& o = opts[m];
+        const int vo = s->voff[m], eo = s->eoff[m], go = s->goff[m];
+        const int Sm = me.max_vrt_per_grn;
+        int ngr = 0, nve = 0;
+        for (int i = 0; i < me.n_vertices; ++i) {
+            const size_t I = static_cast<size_t>(vo + i);
+            pos[2 * I] = me.pos[2 * i]; pos[2 * I + 1] = me.pos[2 * i + 1];
+            if (me.vel) { vel[2 * I] = me.vel[2 * i]; vel[2 * I + 1] = me.vel[2 * i + 1]; }
+            for (int k = 0; k < 3; ++k) {
+                vb[3 * I + k] = gid(me.v_bnd[3 * i + k], eo);
+                vg[3 * I + k] = gid(me.v_grn[3 * i + k], go);
+            }
+            ven[I] = me.v_enabled[i] ? 1 : 0;
+            nve += ven[I];
+            if (me.v_energy) venergy[I] = me.v_energy[i];
+            if (me.v_vel_parts) for (int k = 0; k < 4; ++k) vparts[4 * I + k] = me.v_vel_parts[4 * i + k];
+        }
+        for (int b = 0; b < me.n_boundaries; ++b) {
+            const size_t I = static_cast<size_t>(eo + b);
+            bvx[2 * I] = gid(me.b_vtx[2 * b], vo); bvx[2 * I + 1] = gid(me.b_vtx[2 * b + 1], vo);
+            benab[I] = me.b_enabled[b] ? 1 : 0;
+            if (me.b_energy) ben[I] = me.b_energy[b];
+            if (me.b_arclength) blen[I] = me.b_arclength[b];
+            if (me.b_t_ext) bt[I] = me.b_t_ext[b];
+            if (me.b_candidate) bcand[I] = me.b_candidate[b];
+        }
+        for (int g = 0; g < me.n_grains; ++g) {
+            const size_t I = static_cast<size_t>(go + g);
+            glen[I] = me.g_vlen[g];
+            if (glen[I] > s->S || glen[I] < 0) { sevc_destroy(s); return SEVC_E_INVALID_ARG; }
+            for (int k = 0; k < glen[I]; ++k) gvx[I * S + k] = gid(me.g_vtx[static_cast<size_t>(g) * Sm + k], vo);
+            genab[I] = me.g_enabled[g] ? 1 : 0;
+            ngr += genab[I];
+            gori[I] = me.g_orientation[g];
+            gse[I] = me.g_SE[g];
+            if (me.g_area) garea[I] = me.g_area[g];
+            if (me.g_type) gtype[I] = me.g_type[g];
+            if (me.g_dAdt) gdadt[I] = me.g_dAdt[g];
+        }
+        Params& P = prm[static_cast<size_t>(m)];
+        P.D = me.domain_bound;
+        P.dt_max = o.dt; P.dt_min = o.dt_min; P.dt_safety = o.dt_safety; P.flip_thr = o.flip_threshold;
+        P.mobility = o.mobility; P.gb_eps = o.GB_eps; P.gb_scaling = o.GB_scaling; P.mc_k = o.MC_k;
+        P.nucl_rate = o.nucleation_rate; P.nucl_scaling = o.nucleation_scaling; P.tmax = o.tmax;
+        P.seed = o.seed; P.nmax = o.nmax; P.adaptive = o.adaptive_dt; P.nucl_mode = o.nucleation_mode;
+        P.do_nucl = o.do_nucleation; P.gap = o.nucleation_gap; P.events = o.events; P.vel_model = o.velocity_model;
+        P.n_grains0 = ngr;
+        P.min_grains = o.leave_percent < 0 ? -1 : static_cast<int>(std::floor(o.leave_percent * ngr));
+        P.voff = vo; P.nv = me.n_vertices; P.eoff = eo; P.ne = me.n_boundaries; P.goff = go; P.ng = me.n_grains;
+        MState& M = s->ms_host[static_cast<size_t>(m)];
+        M.curr_grains = ngr;
+        M.n_venabled = nve;
+        M.dt = o.dt;
+        M.tau_bits = 0x7FF0000000000000ull;
+    }
+    s->ms_prev = s->ms_host;
+    UP(d.pos, pos.data(), pos.size() * 8); UP(d.vel, vel.data(), vel.size() * 8);
+    UP(d.vb, vb.data(), vb.size() * 4); UP(d.vg, vg.data(), vg.size() * 4); UP(d.ven, ven.data(), V);
+    UP(d.venergy, venergy.data(), V * 8); UP(d.vparts, vparts.data(), 4 * V * 8);
+    UP(d.bvx, bvx.data(), bvx.size() * 4); UP(d.ben, ben.data(), E * 8); UP(d.blen, blen.data(), E * 8);
+    UP(d.bt, bt.data(), E * 8); UP(d.benab, benab.data(), E); UP(d.bcand, bcand.data(), E);
+    UP(d.gvx, gvx.data(), gvx.size() * 4); UP(d.glen, glen.data(), G * 4); UP(d.garea, garea.data(), G * 8);
+    UP(d.gdadt, gdadt.data(), G * 8); UP(d.gori, gori.data(), G * 8); UP(d.gse, gse.data(), G * 8);
+    UP(d.genab, genab.data(), G); UP(d.gtype, gtype.data(), G);
+    UP(dprm, prm.data(), prm.size() * sizeof(Params));
+    UP(d.ms, s->ms_host.data(), s->ms_host.size() * sizeof(MState));
+#undef UP
+    // scratch initial values
+    std::vector<uint8_t> fix_none(4 * V, 3);
+    std::vector<int> minus1(std::max(4 * V, E), -1);
+    if (hipMemcpy(d.mfix, fix_none.data(), 4 * V, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.mgrn, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.mvrt, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.mprev, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.conf, minus1.data(), E * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(d.gfix, 0, G * 4) != hipSuccess || hipMemset(d.gcand, 0, G) != hipSuccess ||
+        hipMemset(d.vnuc, 0, V) != hipSuccess || hipMemset(d.cnt, 0, C_COUNT * 4) != hipSuccess ||
+        hipMemset(d.nucl_r, 0, V * 8) != hipSuccess || hipMemset(d.nucl_ori, 0, V * 8) != hipSuccess ||
+        hipMemset(d.dE, 0, V * 8) != hipSuccess || hipMemset(d.nfac, 0, V * 8) != hipSuccess) {
+        sevc_destroy(s);
+        return SEVC_E_HIP;
+    }
+    *out = s;
+    return SEVC_OK;
+}
+
+int sevc_create(const sevc_options* o, const sevc_mesh* m, sevc_state** out) {
+    return sevc_batch_create(o, m, 1, out);
+}
+
+void sevc_destroy(sevc_state* s) {
+    if (!s) return;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    for (void* p : s->allocs) hipFree(p);
+    for (auto e : s->pool) hipEventDestroy(e);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int sevc_request_diagnostics(sevc_state* s) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    s->diag_next = true;
+    return SEVC_OK;
+}
+
+int sevc_enable_kernel_timing(sevc_state* s, int32_t on) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    s->timing = on != 0;
+    s->ktimes.clear();
+    return SEVC_OK;
+}
+
+int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], double* ms, int64_t* launches) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    int k = 0;
+    for (auto& kv : s->ktimes) {
+        if (k >= max_entries) break;
+        std::snprintf(names[k], 48, "%s", kv.first.c_str());
+        ms[k] = kv.second.first;
+        launches[k] = kv.second.second;
+        ++k;
+    }
+    return k;
+}
+
+static int finish_call(sevc_state* s) {
+    Dev d = s->d;
+    launch(s, "nucl_area_l1", k_nucl_area_l1, (d.ngb + 255) / 256, 256, d);
+    launch(s, "nucl_area_up", k_nucl_area_up, s->B, 256, d);
+    s->ms_prev = s->ms_host;
+    HIPCHK(hipMemcpyAsync(s->ms_host.data(), d.ms, s->ms_host.size() * sizeof(MState), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->timing) collect_timers(s);
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, d.cnt + C_ERR, sizeof(int), hipMemcpyDeviceToHost));
+    if (err != 0) return fail(s, err, std::string("device event error: ") + sevc_strerror(err));
+    return SEVC_OK;
+}
+
+int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
+    if (!s || n_steps < 0) return SEVC_E_INVALID_ARG;
+    for (int k = 0; k < n_steps; ++k) {
+        bool diag = s->diag_next;
+        s->diag_next = false;
+        launch_step(s, diag);
+        if (s->timing && s->pending.size() > 4000) {
+            hipStreamSynchronize(s->stream);
+            collect_timers(s);
+        }
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(s, SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
+    int rc = finish_call(s);
+    fill_stats(s, 0, st);
+    return rc;
+}
+
+int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
+    if (!s || n_steps < 0) return SEVC_E_INVALID_ARG;
+    if (!s->gexec) {
+        bool timing = s->timing;
+        s->timing = false;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        launch_step(s, false);
+        HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
+        HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+        s->timing = timing;
+    }
+    for (int k = 0; k < n_steps; ++k) {
+        if (s->diag_next) {
+            s->diag_next = false;
+            launch_step(s, true);
+        } else {
+            HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+        }
+    }
+    int rc = finish_call(s);
+    fill_stats(s, 0, st);
+    return rc;
+}
+
+int sevc_member_stats(sevc_state* s, int32_t m, sevc_step_stats* st) {
+    if (!s || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    return fill_stats(s, m, st);
+}
+
+int sevc_member_sizes(const sevc_state* s, int32_t m, int32_t* nv, int32_t* ne, int32_t* ng, int32_t* S) {
+    if (!s || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    if (nv) *nv = s->nv[m];
+    if (ne) *ne = s->ne[m];
+    if (ng) *ng = s->ng[m];
+    if (S) *S = s->S;
+    return SEVC_OK;
+}
+
+int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
+    if (!s || !out || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    const int nv = s->nv[m], ne = s->ne[m], ng = s->ng[m], S = s->S;
+    if (out->n_vertices != nv || out->n_boundaries != ne || out->n_grains != ng || out->max_vrt_per_grn != S)
+        return fail(s, SEVC_E_INVALID_ARG, "output mesh sizes differ from the member's");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const Dev& d = s->d;
+    const size_t vo = static_cast<size_t>(s->voff[m]), eo = static_cast<size_t>(s->eoff[m]),
+                 go = static_cast<size_t>(s->goff[m]);
+    auto down = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    bool ok = true;
+    out->domain_bound = s->opts[m].domain_bound;
+    ok &= down(out->pos, d.pos + 2 * vo, 16 * static_cast<size_t>(nv));
+    if (out->vel) ok &= down(out->vel, d.vel + 2 * vo, 16 * static_cast<size_t>(nv));
+    ok &= down(out->v_bnd, d.vb + 3 * vo, 12 * static_cast<size_t>(nv));
+    ok &= down(out->v_grn, d.vg + 3 * vo, 12 * static_cast<size_t>(nv));
+    ok &= down(out->v_enabled, d.ven + vo, static_cast<size_t>(nv));
+    if (out->v_energy) ok &= down(out->v_energy, d.venergy + vo, 8 * static_cast<size_t>(nv));
+    if (out->v_vel_parts) ok &= down(out->v_vel_parts, d.vparts + 4 * vo, 32 * static_cast<size_t>(nv));
+    ok &= down(out->b_vtx, d.bvx + 2 * eo, 8 * static_cast<size_t>(ne));
+    if (out->b_energy) ok &= down(out->b_energy, d.ben + eo, 8 * static_cast<size_t>(ne));
+    if (out->b_arclength) ok &= down(out->b_arclength, d.blen + eo, 8 * static_cast<size_t>(ne));
+    if (out->b_t_ext) ok &= down(out->b_t_ext, d.bt + eo, 8 * static_cast<size_t>(ne));
+    ok &= down(out->b_enabled, d.benab + eo, static_cast<size_t>(ne));
+    if (out->b_candidate) ok &= down(out->b_candidate, d.bcand + eo, static_cast<size_t>(ne));
+    ok &= down(out->g_vtx, d.gvx + go * S, 4 * static_cast<size_t>(ng) * S);
+    ok &= down(out->g_vlen, d.glen + go, 4 * static_cast<size_t>(ng));
+    if (out->g_area) ok &= down(out->g_area, d.garea + go, 8 * static_cast<size_t>(ng));
+    if (out->g_dAdt) ok &= down(out->g_dAdt, d.gdadt + go, 8 * static_cast<size_t>(ng));
+    ok &= down(out->g_orientation, d.gori + go, 8 * static_cast<size_t>(ng));
+    ok &= down(out->g_SE, d.gse + go, 8 * static_cast<size_t>(ng));
+    ok &= down(out->g_enabled, d.genab + go, static_cast<size_t>(ng));
+    if (out->g_type) ok &= down(out->g_type, d.gtype + go, static_cast<size_t>(ng));
+    if (!ok) return fail(s, SEVC_E_HIP, "device to host copy failed");
+    auto loc = [](int32_t* a, size_t n, int off) {
+        for (size_t i = 0; i < n; ++i) if (a[i] >= 0) a[i] -= off;
+    };
+    loc(out->v_bnd, 3 * static_cast<size_t>(nv), s->eoff[m]);
+    loc(out->v_grn, 3 * static_cast<size_t>(nv), s->goff[m]);
+    loc(out->b_vtx, 2 * static_cast<size_t>(ne), s->voff[m]);
+    loc(out->g_vtx, static_cast<size_t>(ng) * S, s->voff[m]);
+    return SEVC_OK;
+}
+
+int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const double* g_SE, double gb_scaling,
+                           double mobility) {
+    if (!s || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const Dev& d = s->d;
+    if (pos) HIPCHK(hipMemcpy(d.pos + 2 * static_cast<size_t>(s->voff[m]), pos, 16 * static_cast<size_t>(s->nv[m]),
+                              hipMemcpyHostToDevice));
+    if (g_SE) HIPCHK(hipMemcpy(d.gse + s->goff[m], g_SE, 8 * static_cast<size_t>(s->ng[m]), hipMemcpyHostToDevice));
+    Params P;
+    HIPCHK(hipMemcpy(&P, d.prm + m, sizeof(Params), hipMemcpyDeviceToHost));
+    if (gb_scaling > 0) { P.gb_scaling = gb_scaling; s->opts[m].GB_scaling = gb_scaling; }
+    if (mobility > 0) { P.mobility = mobility; s->opts[m].mobility = mobility; }
+    HIPCHK(hipMemcpy(const_cast<Params*>(d.prm) + m, &P, sizeof(Params), hipMemcpyHostToDevice));
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return SEVC_OK;
+}
+
+}  // extern "C"
+
+namespace sevc {
+__global__ void k_selftest(int op, int n, const double* x, const double* y, double* out);
+}
+
+extern "C" int sevc_selftest_math(int32_t op, int32_t n, const double* x, const double* y, double* out) {
+    if (n <= 0 || !x || !y || !out) return SEVC_E_INVALID_ARG;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+    if (hipMalloc(&dx, bytes) != hipSuccess || hipMalloc(&dy, bytes) != hipSuccess ||
+        hipMalloc(&dout, bytes) != hipSuccess) {
+        (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dout);
+        return SEVC_E_NOMEM;
+    }
+    int rc = SEVC_OK;
+    if (hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SEVC_E_HIP;
+    if (rc == SEVC_OK) {
+        hipLaunchKernelGGL(sevc::k_selftest, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy, dout);
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SEVC_E_HIP;
+    }
+    (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dout);
+    return rc;
+}

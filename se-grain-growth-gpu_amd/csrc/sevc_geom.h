@@ -1,0 +1,95 @@
+// sevc_geom.h -- device-side periodic geometry of the vertex model.
+//
+// Same formulas and operation order as geometry.h of the reference; every
+// translation unit that includes this must be compiled with
+// -ffp-contract=off so that no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sevc {
+
+// dom_mod, geometry.h:133-137
+__device__ __forceinline__ double dom_mod(double a, double D) {
+    if (a < 0) return a + D;
+    else if (a >= D) return a - D;
+    return a;
+}
+
+// wrap_dist, geometry.h:147-152
+__device__ __forceinline__ double wrapd(double dest, double orig, double D) {
+    double d = dest - orig;
+    if (d > +(D / 2.0)) return d - D;
+    if (d < -(D / 2.0)) return D + d;
+    return d;
+}
+
+// vector2_mag, geometry.h:217-223: p*sqrt(1+(q/p)^2), p = max |.|, q = min |.|
+__device__ __forceinline__ double mag(double x, double y) {
+    double ax = fabs(x), ay = fabs(y);
+    double p = ax > ay ? ax : ay;
+    if (p == 0) return 0;
+    double q = ax < ay ? ax : ay;
+    double r = q / p;
+    return p * sqrt(1.0 + r * r);
+}
+
+// compute_arclength, geometry.h:328-330: |wrap(a - b)|
+__device__ __forceinline__ double arclen(double ax, double ay, double bx, double by, double D) {
+    return mag(wrapd(ax, bx, D), wrapd(ay, by, D));
+}
+
+// Streaming form of compute_area (geometry.h:757-765 after the in-place
+// unwrap of geometry.h:908-931).  The unwrapped point U_k is
+// U_{k-1} + wrap(P_k - P_{k-1}) with U_0 = 0, and the shoelace term
+// U_{k-1} x U_k is accumulated as soon as U_k is known, which reproduces
+// the reference's left-to-right sum term by term; the closing term
+// U_{n-1} x U_0 is added last.
+struct AreaAcc {
+    double px, py;   // previous raw point
+    double ux, uy;   // previous unwrapped point
+    double area;
+    __device__ __forceinline__ void start(double x0, double y0) {
+        px = x0; py = y0; ux = 0; uy = 0; area = 0;
+    }
+    __device__ __forceinline__ void push(double cx, double cy, double D) {
+        double dx = wrapd(cx, px, D);
+        double dy = wrapd(cy, py, D);
+        px = cx; py = cy;
+        double nx = ux + dx, ny = uy + dy;
+        area += ux * ny - uy * nx;
+        ux = nx; uy = ny;
+    }
+    __device__ __forceinline__ double finish() {
+        const double zx = 0, zy = 0;
+        area += ux * zy - uy * zx;
+        area *= 0.5;
+        return area;
+    }
+};
+
+// compute_dAdt (geometry.h:779-787) in the same streaming form: terms
+// (V_k x U_{k+1}) + (U_k x V_{k+1}) in the reference's sign convention.
+struct DadtAcc {
+    double px, py, ux, uy, vx, vy, d;
+    double u0x, u0y, v0x, v0y;
+    __device__ __forceinline__ void start(double x0, double y0, double vx0, double vy0) {
+        px = x0; py = y0; ux = 0; uy = 0; vx = vx0; vy = vy0; d = 0;
+        u0x = 0; u0y = 0; v0x = vx0; v0y = vy0;
+    }
+    __device__ __forceinline__ void push(double cx, double cy, double cvx, double cvy, double D) {
+        double dx = wrapd(cx, px, D), dy = wrapd(cy, py, D);
+        px = cx; py = cy;
+        double nx = ux + dx, ny = uy + dy;
+        d += (vx * ny + ux * cvy);
+        d -= (vy * nx + uy * cvx);
+        ux = nx; uy = ny; vx = cvx; vy = cvy;
+    }
+    __device__ __forceinline__ double finish() {
+        d += (vx * u0y + ux * v0y);
+        d -= (vy * u0x + uy * v0x);
+        return d;
+    }
+};
+
+}  // namespace sevc

@@ -1,0 +1,1535 @@
+// sevc_kernels.hip -- the vertex-model time step on CDNA4 (gfx950), FP64.
+//
+// One launch sequence per step covers every member of a batch.  All control
+// decisions (who is active, dt, whether an event list is empty) are taken on
+// the device, so a step never waits for the host and can be captured in a
+// HIP graph.  Kernel order follows main.cu:124-389.
+//
+// Parity rules (the oracle follows the same):
+//  * events that touch disjoint vertex sets commute, so independent T1 and
+//    T2 events run in parallel; events whose vertices collide run afterwards
+//    on one lane in entity-index order, which is the serial order of the
+//    reference's <<<1,1>>> kernels (main.cu:234,259);
+//  * deferred grain-list edits (topological.h:1209-1227) touch only the
+//    edited grain's own list, so they run one lane per flagged grain.
+#include <hip/hip_runtime.h>
+#include "sevc_state.h"
+#include "sevc_geom.h"
+#include "../../include/sevc.h"
+#include "../../include/sevc_math.h"
+
+namespace sevc {
+
+enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17
+enum { T_COMMON = 0, T_NUCLEATED = 1 };                                  // geometry.h:19
+constexpr unsigned kStreamMcSite = 1u, kStreamMcPick = 2u, kStreamRate = 3u;
+constexpr int kMaxStride = 128;   // largest grain-list stride supported
+
+// ---------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------
+
+// Map the hardware workgroup id so that each of the 8 XCDs (workgroups are
+// dealt round-robin over XCDs) processes one contiguous slice of the table:
+// neighbouring slots then share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ void set_error(const Dev& d, int code) {
+    atomicCAS(&d.cnt[C_ERR], 0, code);
+}
+
+__device__ __forceinline__ int other_end(const Dev& d, int b, int v) {
+    return d.bvx[2 * b] == v ? d.bvx[2 * b + 1] : d.bvx[2 * b];
+}
+
+// geometry.h:837-848
+__device__ __forceinline__ int grains_intersect(const Dev& d, int a, int b, int out[2]) {
+    int k = 0;
+    out[0] = out[1] = -1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (d.vg[3 * a + i] == d.vg[3 * b + j]) {
+                out[k++] = d.vg[3 * a + i];
+                if (k == 2) return 2;
+            }
+    return k;
+}
+
+// geometry.h:886-898 (with grains_union, :863-875)
+__device__ __forceinline__ void grains_symdiff(const Dev& d, int a, int b, int out[2]) {
+    int gic[2], gun[4];
+    out[0] = out[1] = -1;
+    grains_intersect(d, a, b, gic);
+    gun[3] = -1;
+    for (int i = 0; i < 3; i++) gun[i] = d.vg[3 * a + i];
+    for (int i = 0; i < 3; i++) {
+        int g = d.vg[3 * b + i];
+        if (g != gun[0] && g != gun[1] && g != gun[2]) { gun[3] = g; break; }
+    }
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+        if (gun[i] != gic[0] && gun[i] != gic[1]) {
+            out[k++] = gun[i];
+            if (k == 2) return;
+        }
+}
+
+__device__ __forceinline__ int* glist(const Dev& d, int g) { return d.gvx + (size_t)g * d.S; }
+
+// geometry.h:703-710
+__device__ __forceinline__ int grain_contains_vertex(const Dev& d, int g, int v) {
+    const int* L = glist(d, g);
+    int n = d.glen[g];
+    for (int i = 0; i < n; i++) if (L[i] == v) return i;
+    return -1;
+}
+
+// geometry.h:724-734
+__device__ __forceinline__ int grain_contains_consecutive(const Dev& d, int g, int a, int b) {
+    const int* L = glist(d, g);
+    int n = d.glen[g];
+    for (int i = 0; i < n; i++) {
+        int nx = L[(i + 1) % n];
+        if (L[i] == a && nx == b) return 1;
+        if (L[i] == b && nx == a) return 2;
+    }
+    return -1;
+}
+
+// geometry.h:548-550, pow(c,3) as c*c*c
+__device__ __forceinline__ double gb_energy_func(double d_alpha, double eps) {
+    double c = sevc_cos(4 * d_alpha);
+    return (1 + 0.5 * eps * (1 - c * c * c));
+}
+
+// geometry.h:559-564
+__device__ __forceinline__ double boundary_compute_energy(const Dev& d, int b, double eps) {
+    int gic[2];
+    if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) return 1.0;
+    return gb_energy_func(d.gori[gic[0]] - d.gori[gic[1]], eps);
+}
+
+// geometry.h:513-517 plus the flip-threshold extension
+__device__ __forceinline__ bool is_candidate(double te, double len, double dt, double thr) {
+    if (te >= 0 && te <= dt) return true;
+    if (thr > 0 && te >= 0 && len < thr) return true;
+    return false;
+}
+
+__device__ __forceinline__ double member_dt(const Params& P, const MState& M) {
+    if (!P.adaptive) return P.dt_max;
+    unsigned long long bits = M.tau_bits;
+    if (bits == 0x7FF0000000000000ull) return P.dt_max;
+    double tau = __longlong_as_double((long long)bits);
+    double dt = P.dt_safety * tau;
+    if (dt < P.dt_min) dt = P.dt_min;
+    if (dt > P.dt_max) dt = P.dt_max;
+    return dt;
+}
+
+__device__ __forceinline__ double grain_area(const Dev& d, int g, double D) {
+    int n = d.glen[g];
+    if (n <= 0) return 0.0;
+    const int* L = glist(d, g);
+    AreaAcc acc;
+    int w = L[0];
+    acc.start(d.pos[2 * w], d.pos[2 * w + 1]);
+    for (int k = 1; k < n; k++) {
+        w = L[k];
+        acc.push(d.pos[2 * w], d.pos[2 * w + 1], D);
+    }
+    return acc.finish();
+}
+
+// geometry.h:369-376
+__device__ __forceinline__ void reset_mods(const Dev& d, int v) {
+    for (int j = 0; j < 4; j++) {
+        d.mfix[4 * v + j] = FIX_NONE;
+        d.mgrn[4 * v + j] = -1;
+        d.mvrt[4 * v + j] = -1;
+        d.mprev[4 * v + j] = -1;
+    }
+}
+
+__device__ __forceinline__ void set_mod(const Dev& d, int v, int slot, int fix, int grn, int vrt) {
+    d.mfix[4 * v + slot] = (uint8_t)fix;
+    d.mgrn[4 * v + slot] = grn;
+    d.mvrt[4 * v + slot] = vrt;
+}
+
+// Flag grain g for the fix stage `stage` (grain.fix = true) and list it once.
+__device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
+    if (atomicExch(&d.gfix[g], 1) == 0) {
+        int i = atomicAdd(&d.cnt[C_FIX0 + stage], 1);
+        d.l_fix[stage][i] = g;
+    }
+}
+
+#define TABLE_BLOCK(tbl, nblk)                                   \
+    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);        \
+    if (lb_ >= (nblk)) return;                                   \
+    const Blk bd_ = (tbl)[lb_];                                  \
+    const Params& P = d.prm[bd_.m];                              \
+    MState& M = d.ms[bd_.m];                                     \
+    (void)P; (void)M;                                            \
+    const int i = bd_.b + (int)threadIdx.x;
+
+// ---------------------------------------------------------------------
+// step begin / end
+// ---------------------------------------------------------------------
+
+// Loop condition of main.cu:120 (time rule), grain rule, nucleation gate
+// main.cu:137-141; clears the per-step counters.
+__global__ void k_begin(Dev d) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0)
+        for (int c = 0; c < C_COUNT; c++) if (c != C_ERR) d.cnt[c] = 0;
+    if (m >= d.B) return;
+    const Params& P = d.prm[m];
+    MState& M = d.ms[m];
+    M.tau_bits = 0x7FF0000000000000ull;
+    M.nucl_now = 0;
+    if (M.done) { M.active = 0; return; }
+    bool tok = (P.tmax <= 0 && P.nmax <= 0) || (M.t <= P.tmax && P.tmax > 0) || (M.n <= P.nmax && P.nmax > 0);
+    bool gok = (P.min_grains <= 0) || (M.curr_grains >= P.min_grains);
+    if (!tok || !gok) { M.active = 0; M.done = 1; return; }
+    M.active = 1;
+    M.vsteps += M.n_venabled;
+    if (P.events && P.do_nucl) {
+        int gap = P.gap > 0 ? P.gap : 1;
+        if (P.nucl_mode == SEVC_NUCL_MC) {
+            if (M.curr_grains <= 5 * P.n_grains0 / 6) M.allow_nucl = 1;
+            M.nucl_now = ((M.n % gap) == 0 && M.allow_nucl) ? 1 : 0;
+        } else {
+            M.nucl_now = ((M.n % gap) == 0) ? 1 : 0;
+        }
+    }
+}
+
+// Per-block counts of enabled grains and vertices (main.cu:317).
+__global__ void k_count_grains(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active) return;
+    int c = (i < bd_.e && d.genab[i]) ? 1 : 0;
+    unsigned long long bal = __ballot(c);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&M.curr_grains, (int)__popcll(bal));
+}
+
+__global__ void k_count_vertices(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active) return;
+    int c = (i < bd_.e && d.ven[i]) ? 1 : 0;
+    unsigned long long bal = __ballot(c);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&M.n_venabled, (int)__popcll(bal));
+}
+
+__global__ void k_end_prepare(Dev d) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= d.B) return;
+    MState& M = d.ms[m];
+    if (!M.active) return;
+    M.curr_grains = 0;
+    M.n_venabled = 0;
+}
+
+// main.cu:388-393
+__global__ void k_end(Dev d) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= d.B) return;
+    const Params& P = d.prm[m];
+    MState& M = d.ms[m];
+    if (!M.active) return;
+    M.dt = member_dt(P, M);
+    M.n += 1;
+    M.steps += 1;
+    if (P.adaptive) M.t += M.dt;
+    else M.t = M.n * P.dt_max;
+    if (d.cnt[C_ERR] != 0 && M.status == 0) M.status = d.cnt[C_ERR];
+    if (M.status != 0) M.done = 1;
+    if (M.curr_grains <= P.min_grains && P.min_grains > 0) M.done = 1;
+}
+
+// ---------------------------------------------------------------------
+// calculus.h: lengths, energies, areas, velocities, translation
+// ---------------------------------------------------------------------
+
+// compute_boundary_arclengths (calculus.h:83-93) fused with
+// compute_boundary_energies (calculus.h:104-114).
+__global__ void k_edge_len_energy(Dev d, int only_nucl) {
+    TABLE_BLOCK(d.eblk, d.neb)
+    if (!M.active || (only_nucl && !M.nucl_now)) return;
+    if (i >= bd_.e || !d.benab[i]) return;
+    int a = d.bvx[2 * i], b = d.bvx[2 * i + 1];
+    d.blen[i] = arclen(d.pos[2 * a], d.pos[2 * a + 1], d.pos[2 * b], d.pos[2 * b + 1], P.D);
+    double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, i, P.gb_eps);
+    d.ben[i] = e * P.gb_scaling;
+}
+
+// compute_grain_areas, calculus.h:224-234
+__global__ void k_grain_area(Dev d, int only_nucl) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || (only_nucl && !M.nucl_now)) return;
+    if (i >= bd_.e || !d.genab[i]) return;
+    d.garea[i] = grain_area(d, i, P.D);
+}
+
+// compute_vertex_energies, calculus.h:123-133 (diagnostic)
+__global__ void k_vertex_energy(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || i >= bd_.e || !d.ven[i]) return;
+    double e = 0.0;
+    for (int j = 0; j < 3; j++) {
+        int b = d.vb[3 * i + j], g = d.vg[3 * i + j];
+        e += (d.blen[b] * d.ben[b] * 0.5);
+        e += (d.gse[g] * d.garea[g] / d.glen[g]);
+    }
+    d.venergy[i] = e;
+}
+
+// energy_diff, calculus.h:19-73: energy change for a grad_eps shift of one
+// coordinate of vertex v, from 3 perturbed lengths and 3 perturbed areas.
+__device__ __forceinline__ double energy_diff(const Dev& d, const Params& P, int comp, int v,
+                                              double* bnd_out, double* grn_out) {
+    const double D = P.D;
+    double nx = d.pos[2 * v], ny = d.pos[2 * v + 1];
+    if (comp == 0) nx += SEVC_GRAD_EPS;
+    else ny += SEVC_GRAD_EPS;
+    nx = dom_mod(nx, D);
+    ny = dom_mod(ny, D);
+    double arcl[3];
+    for (int j = 0; j < 3; j++) {
+        int b = d.vb[3 * v + j];
+        int nb = other_end(d, b, v);
+        arcl[j] = arclen(nx, ny, d.pos[2 * nb], d.pos[2 * nb + 1], D);
+    }
+    double areas[3];
+    for (int j = 0; j < 3; j++) {
+        int g = d.vg[3 * v + j];
+        int n = d.glen[g];
+        const int* L = glist(d, g);
+        double a = 0.0;
+        if (n > 0) {
+            AreaAcc acc;
+            int w = L[0];
+            double x = d.pos[2 * w], y = d.pos[2 * w + 1];
+            if (w == v) { x = nx; y = ny; }
+            acc.start(x, y);
+            for (int k = 1; k < n; k++) {
+                w = L[k];
+                x = d.pos[2 * w]; y = d.pos[2 * w + 1];
+                if (w == v) { x = nx; y = ny; }
+                acc.push(x, y, D);
+            }
+            a = acc.finish();
+        }
+        areas[j] = a;
+    }
+    double bnd_term = 0.0, grn_term = 0.0;
+    for (int j = 0; j < 3; j++) {
+        int b = d.vb[3 * v + j], g = d.vg[3 * v + j];
+        bnd_term += (d.ben[b] * (arcl[j] - d.blen[b]));
+        grn_term += (d.gse[g] * (fabs(areas[j]) - fabs(d.garea[g])));
+    }
+    *bnd_out = bnd_term;
+    *grn_out = grn_term;
+    return bnd_term + grn_term;
+}
+
+// Analytic -grad E (extension): see the oracle for the formula.
+__device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, int v, double* fx, double* fy) {
+    const double D = P.D;
+    double x = d.pos[2 * v], y = d.pos[2 * v + 1];
+    double ex = 0.0, ey = 0.0, gx = 0.0, gy = 0.0;
+    for (int j = 0; j < 3; j++) {
+        int b = d.vb[3 * v + j];
+        int nb = other_end(d, b, v);
+        double dx = wrapd(x, d.pos[2 * nb], D), dy = wrapd(y, d.pos[2 * nb + 1], D);
+        double L = mag(dx, dy);
+        if (L > 0) {
+            ex += d.ben[b] * (dx / L);
+            ey += d.ben[b] * (dy / L);
+        }
+    }
+    for (int j = 0; j < 3; j++) {
+        int g = d.vg[3 * v + j];
+        int n = d.glen[g];
+        int k = grain_contains_vertex(d, g, v);
+        if (k < 0 || n < 3) continue;
+        const int* L = glist(d, g);
+        int wp = L[(k + n - 1) % n], wn = L[(k + 1) % n];
+        double dpx = wrapd(x, d.pos[2 * wp], D), dpy = wrapd(y, d.pos[2 * wp + 1], D);
+        double dnx = wrapd(d.pos[2 * wn], x, D), dny = wrapd(d.pos[2 * wn + 1], y, D);
+        double sg = d.garea[g] < 0 ? -1.0 : 1.0;
+        double c = d.gse[g] * sg * 0.5;
+        gx += c * (dpy + dny);
+        gy += c * (-(dpx + dnx));
+    }
+    *fx = -(ex + gx);
+    *fy = -(ey + gy);
+}
+
+// compute_vertex_velocities, calculus.h:143-154 (x mobility)
+__global__ void k_velocity(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || i >= bd_.e || !d.ven[i]) return;
+    const int v = i;
+    double vx, vy;
+    if (P.vel_model == SEVC_VEL_ANALYTIC) {
+        double fx, fy;
+        analytic_force(d, P, v, &fx, &fy);
+        vx = fx * P.mobility;
+        vy = fy * P.mobility;
+    } else {
+        double bx, gx, by, gy;
+        double ex = energy_diff(d, P, 0, v, &bx, &gx);
+        double ey = energy_diff(d, P, 1, v, &by, &gy);
+        vx = (-ex / SEVC_GRAD_EPS) * P.mobility;
+        vy = (-ey / SEVC_GRAD_EPS) * P.mobility;
+        if (d.diag) {
+            d.vparts[4 * v + 0] = -bx / SEVC_GRAD_EPS;
+            d.vparts[4 * v + 1] = -gx / SEVC_GRAD_EPS;
+            d.vparts[4 * v + 2] = -by / SEVC_GRAD_EPS;
+            d.vparts[4 * v + 3] = -gy / SEVC_GRAD_EPS;
+        }
+    }
+    d.vel[2 * v] = vx;
+    d.vel[2 * v + 1] = vy;
+}
+
+// compute_grain_dAdts, calculus.h:243-253 (diagnostic)
+__global__ void k_grain_dadt(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || i >= bd_.e || !d.genab[i]) return;
+    int n = d.glen[i];
+    double r = 0.0;
+    if (n > 0) {
+        const int* L = glist(d, i);
+        DadtAcc acc;
+        int w = L[0];
+        acc.start(d.pos[2 * w], d.pos[2 * w + 1], d.vel[2 * w], d.vel[2 * w + 1]);
+        for (int k = 1; k < n; k++) {
+            w = L[k];
+            acc.push(d.pos[2 * w], d.pos[2 * w + 1], d.vel[2 * w], d.vel[2 * w + 1], P.D);
+        }
+        r = acc.finish();
+    }
+    d.gdadt[i] = r;
+}
+
+// update_vertices_positions, calculus.h:391-404
+__global__ void k_translate(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || i >= bd_.e || !d.ven[i]) return;
+    const double dt = member_dt(P, M);
+    double x = d.pos[2 * i], y = d.pos[2 * i + 1];
+    x += d.vel[2 * i] * dt;
+    y += d.vel[2 * i + 1] * dt;
+    d.pos[2 * i] = dom_mod(x, P.D);
+    d.pos[2 * i + 1] = dom_mod(y, P.D);
+}
+
+// ---------------------------------------------------------------------
+// topological.h: extinction times, labels, votes
+// ---------------------------------------------------------------------
+
+// compute_t_ext, topological.h:303-334, plus the adaptive-dt reduction
+// (smallest positive t_ext of edges not shorter than the flip threshold).
+__global__ void k_text(Dev d) {
+    TABLE_BLOCK(d.eblk, d.neb)
+    if (!M.active || i >= bd_.e || !d.benab[i]) return;
+    const double D = P.D;
+    d.bcand[i] = 0;
+    int a = d.bvx[2 * i], b = d.bvx[2 * i + 1];
+    // adjust_origin_for_points on (ini, end): U0 = 0, U1 = 0 + wrap(end - ini)
+    double x1 = 0 + wrapd(d.pos[2 * b], d.pos[2 * a], D);
+    double y1 = 0 + wrapd(d.pos[2 * b + 1], d.pos[2 * a + 1], D);
+    double dx = x1 - 0, dy = y1 - 0;
+    double norm = mag(dx, dy);
+    double tx, ty;
+    if (norm == 0) { tx = 0; ty = 0; }
+    else {
+        double ang = sevc_atan2(dy, dx);
+        tx = sevc_cos(ang);
+        ty = sevc_sin(ang);
+    }
+    double px = d.vel[2 * b] - d.vel[2 * a];
+    double py = d.vel[2 * b + 1] - d.vel[2 * a + 1];
+    double aux = tx * px + ty * py;
+    double te = (aux == 0) ? 100.0 : -norm / aux;
+    d.bt[i] = te;
+    if (P.adaptive && te > 0 && !(d.blen[i] < P.flip_thr))
+        atomicMin(&M.tau_bits, (unsigned long long)__double_as_longlong(te));
+}
+
+// label_boundaries, topological.h:343-353 (every slot, quirk A-6), and the
+// compaction of enabled candidates.
+__global__ void k_label(Dev d) {
+    TABLE_BLOCK(d.eblk, d.neb)
+    if (!M.active || !P.events || i >= bd_.e) return;
+    const double dt = member_dt(P, M);
+    bool c = is_candidate(d.bt[i], d.blen[i], dt, P.flip_thr);
+    d.bcand[i] = c ? 1 : 0;
+    if (c && d.benab[i]) {
+        int k = atomicAdd(&d.cnt[C_CAND], 1);
+        d.l_cand[k] = i;
+    }
+}
+
+// polling_system (topological.h:549-578) as Jacobi rounds to a fixpoint,
+// then check_conflictive_boundaries (:83-121) and
+// unlabel_conflictive_boundaries (:129-155), over the compacted candidate
+// list.  Only vertices that touch a candidate can vote for one, so the
+// rounds are exact on the list.  One workgroup.
+__global__ void __launch_bounds__(1024) k_poll(Dev d) {
+    __shared__ int changed;
+    const int nc = d.cnt[C_CAND];
+    if (nc == 0) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto vote = [&](int v) {
+        double min_t = 99999999;
+        int arg = 0;
+        for (int k = 0; k < 3; k++) {
+            int b = d.vb[3 * v + k];
+            if (d.bt[b] <= min_t && d.bcand[b]) { min_t = d.bt[b]; arg = k; }
+        }
+        return d.vb[3 * v + arg];
+    };
+    for (;;) {
+        if (tid == 0) changed = 0;
+        // phase A: votes on the flags at the start of the round
+        for (int k = tid; k < nc; k += nt) {
+            int c = d.l_cand[k];
+            int two = 0;
+            if (d.bcand[c]) {
+                int nvote = (vote(d.bvx[2 * c]) == c) + (vote(d.bvx[2 * c + 1]) == c);
+                two = (nvote == 2);
+            }
+            d.conf[c] = two;     // scratch: 2-vote flag of this round
+        }
+        __syncthreads();
+        // phase B: a 2-vote edge un-labels its four neighbours
+        for (int k = tid; k < nc; k += nt) {
+            int c = d.l_cand[k];
+            if (d.conf[c] == 1) {
+                for (int e = 0; e < 2; e++) {
+                    int v = d.bvx[2 * c + e];
+                    for (int q = 0; q < 3; q++) {
+                        int x = d.vb[3 * v + q];
+                        if (x != c && d.bcand[x]) { d.bcand[x] = 0; changed = 1; }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        int ch = changed;
+        __syncthreads();
+        if (!ch) break;
+    }
+    for (int k = tid; k < nc; k += nt) d.conf[d.l_cand[k]] = -1;
+    __syncthreads();
+    // check_conflictive_boundaries on the non-candidate neighbours
+    for (int k = tid; k < nc; k += nt) {
+        int c = d.l_cand[k];
+        if (!d.bcand[c]) continue;
+        for (int e = 0; e < 2; e++) {
+            int v = d.bvx[2 * c + e];
+            for (int q = 0; q < 3; q++) {
+                int x = d.vb[3 * v + q];
+                if (x == c || !d.benab[x] || d.bcand[x]) continue;
+                int ini = d.bvx[2 * x], end = d.bvx[2 * x + 1];
+                int bi = -1, be = -1;
+                for (int r = 0; r < 3; r++) {
+                    int y = d.vb[3 * ini + r];
+                    if (y != x && d.bcand[y]) { bi = y; break; }
+                }
+                for (int r = 0; r < 3; r++) {
+                    int y = d.vb[3 * end + r];
+                    if (y != x && d.bcand[y]) { be = y; break; }
+                }
+                if (bi >= 0 && be >= 0) {
+                    int w;
+                    if (d.bt[bi] == d.bt[be]) w = bi < be ? bi : be;
+                    else w = d.bt[bi] < d.bt[be] ? bi : be;
+                    d.conf[x] = w;
+                    int slot = atomicAdd(&d.cnt[C_CONF], 1);
+                    d.l_conf[slot] = x;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // unlabel_conflictive_boundaries: reads conf only, writes its own flag
+    for (int k = tid; k < nc; k += nt) {
+        int c = d.l_cand[k];
+        if (!d.bcand[c]) continue;
+        bool drop = false;
+        for (int e = 0; e < 2 && !drop; e++) {
+            int v = d.bvx[2 * c + e];
+            for (int q = 0; q < 3; q++) {
+                int x = d.vb[3 * v + q];
+                if (x != c && d.conf[x] != -1 && d.conf[x] != c) { drop = true; break; }
+            }
+        }
+        if (drop) d.bcand[c] = 0;
+    }
+    __syncthreads();
+    const int ncf = d.cnt[C_CONF];
+    for (int k = tid; k < ncf; k += nt) d.conf[d.l_conf[k]] = -1;
+}
+
+// ---------------------------------------------------------------------
+// deferred grain-list edits (topological.h:1035-1163)
+// ---------------------------------------------------------------------
+
+// grain_add_vertex(grn, vrt, vrt_prev), geometry.h:629-645
+__device__ __forceinline__ void list_add_after(const Dev& d, int g, int v, int prev) {
+    int* L = glist(d, g);
+    int n = d.glen[g], pos = -1;
+    for (int i = 0; i < n; i++) if (L[i] == prev) { pos = i + 1; break; }
+    if (pos < 0) { set_error(d, SEVC_E_STATE); return; }
+    if (n >= d.S) { set_error(d, SEVC_E_GRAIN_FULL); return; }
+    for (int i = n; i > pos; i--) L[i] = L[i - 1];
+    L[pos] = v;
+    d.glen[g] = n + 1;
+}
+
+// grain_remove_vertex, geometry.h:653-666
+__device__ __forceinline__ void list_remove(const Dev& d, int g, int v) {
+    int* L = glist(d, g);
+    int n = d.glen[g];
+    for (int i = 0; i < n; i++)
+        if (L[i] == v) {
+            for (int j = i; j < n - 1; j++) L[j] = L[j + 1];
+            d.glen[g] = n - 1;
+            L[n - 1] = -1;
+            return;
+        }
+}
+
+// grain_replace_vertex, geometry.h:686-694
+__device__ __forceinline__ void list_replace(const Dev& d, int g, int old_v, int new_v) {
+    int* L = glist(d, g);
+    int n = d.glen[g];
+    for (int i = 0; i < n; i++) if (L[i] == old_v) { L[i] = new_v; return; }
+}
+
+// perform_ADD_ops, perform_REMOVE_ops, perform_REPLACE_ops for one grain.
+__device__ void fix_one_grain(const Dev& d, int g) {
+    if (!d.genab[g]) return;
+    int* L = glist(d, g);
+    for (int i = 0; i < d.glen[g]; i++) {                       // :1040-1067
+        int v = L[i];
+        for (int j = 0; j < 4; j++)
+            if (d.mfix[4 * v + j] == FIX_ADD && d.mgrn[4 * v + j] == g)
+                list_add_after(d, g, d.mvrt[4 * v + j], d.mprev[4 * v + j]);
+    }
+    {                                                           // :1086-1106
+        // The reference gathers the REMOVE commands of the list's vertices
+        // and deletes each named vertex once; deleting a set of distinct
+        // entries leaves the same list in any order, so mark the doomed
+        // positions first and compact once.
+        const int n = d.glen[g];
+        uint64_t doomed[kMaxStride / 64] = {0, 0};
+        for (int i = 0; i < n; i++) {
+            int v = L[i];
+            for (int j = 0; j < 4; j++) {
+                if (d.mfix[4 * v + j] != FIX_REMOVE || d.mgrn[4 * v + j] != g) continue;
+                int w = d.mvrt[4 * v + j];
+                for (int q = 0; q < n; q++)
+                    if (L[q] == w) { doomed[q >> 6] |= 1ull << (q & 63); break; }
+            }
+        }
+        int w = 0;
+        for (int i = 0; i < n; i++)
+            if (!((doomed[i >> 6] >> (i & 63)) & 1ull)) L[w++] = L[i];
+        for (int i = w; i < n; i++) L[i] = -1;
+        d.glen[g] = w;
+    }
+    for (int i = 0; i < d.glen[g]; i++) {                       // :1124-1136
+        int v = L[i];
+        for (int j = 0; j < 4; j++)
+            if (d.mfix[4 * v + j] == FIX_REPLACE && d.mgrn[4 * v + j] == g)
+                list_replace(d, g, d.mprev[4 * v + j], d.mvrt[4 * v + j]);
+    }
+}
+
+__global__ void k_fix(Dev d, int stage) {
+    const int n = d.cnt[C_FIX0 + stage];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        fix_one_grain(d, d.l_fix[stage][k]);
+}
+
+// clear_grains_fix_state (topological.h:1150-1163) for the flagged grains,
+// optionally followed by the area update of main.cu:266 for them.
+__global__ void k_fix_clear(Dev d, int stage, int with_area) {
+    const int n = d.cnt[C_FIX0 + stage];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = d.l_fix[stage][k];
+        d.gfix[g] = 0;
+        if (!d.genab[g]) continue;
+        const int* L = glist(d, g);
+        for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
+    }
+    (void)with_area;
+}
+
+__global__ void k_area_list(Dev d, int which) {
+    const int n = which == 0 ? min(d.cnt[C_DIRTY], d.G) : d.cnt[C_FIX1];
+    const int* lst = which == 0 ? d.l_dirty : d.l_fix[1];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = lst[k];
+        if (!d.genab[g]) continue;
+        int m = 0;
+        while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
+        d.garea[g] = grain_area(d, g, d.prm[m].D);
+    }
+}
+
+// ---------------------------------------------------------------------
+// two adjacent 3-sided grains (topological.h:22-73, 840-985)
+// ---------------------------------------------------------------------
+
+// Edges whose two grains are both 3-sided, found from the 3-sided grains.
+__global__ void k_two_scan(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || !P.events || i >= bd_.e) return;
+    if (!d.genab[i] || d.glen[i] != 3) return;
+    const int* L = glist(d, i);
+    for (int k = 0; k < 3; k++) {
+        int a = L[k], c = L[(k + 1) % 3];
+        for (int q = 0; q < 3; q++) {
+            int e = d.vb[3 * a + q];
+            if (other_end(d, e, a) != c || !d.benab[e]) continue;
+            int gic[2];
+            if (grains_intersect(d, d.bvx[2 * e], d.bvx[2 * e + 1], gic) < 2) continue;
+            if (d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3) {
+                int lo = gic[0] < gic[1] ? gic[0] : gic[1];
+                if (lo == i) {
+                    int s = atomicAdd(&d.cnt[C_TWO], 1);
+                    d.l_two[s] = e;
+                }
+            }
+        }
+    }
+}
+
+__device__ bool two_qualifies(const Dev& d, int e) {
+    if (!d.benab[e]) return false;
+    int gic[2];
+    if (grains_intersect(d, d.bvx[2 * e], d.bvx[2 * e + 1], gic) < 2) return false;
+    return d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3;
+}
+
+// remove_two_grains, topological.h:840-955.  Returns CE (the edge whose
+// endpoint moved) or -1.
+__device__ int remove_two_grains(const Dev& d, int bnd, const int gnic[2]) {
+    int A = d.bvx[2 * bnd], B = d.bvx[2 * bnd + 1];
+    int AC = -1, BD = -1, CE = -1, DF = -1;
+    for (int i = 0; i < 3; i++) if (d.vb[3 * A + i] == bnd) { AC = d.vb[3 * A + (i + 1) % 3]; break; }
+    for (int j = 0; j < 3; j++) if (d.vb[3 * B + j] == bnd) { BD = d.vb[3 * B + (j + 1) % 3]; break; }
+    if (AC < 0 || BD < 0) { set_error(d, SEVC_E_STATE); return -1; }
+    int C = other_end(d, AC, A);
+    int Dv = other_end(d, BD, B);
+    for (int k = 0; k < 3; k++) if (d.vb[3 * C + k] == AC) { CE = d.vb[3 * C + (k + 2) % 3]; break; }
+    for (int l = 0; l < 3; l++) if (d.vb[3 * Dv + l] == BD) { DF = d.vb[3 * Dv + (l + 2) % 3]; break; }
+    if (CE < 0 || DF < 0) { set_error(d, SEVC_E_STATE); return -1; }
+    int F = other_end(d, DF, Dv);
+    if (d.bvx[2 * CE] == C) d.bvx[2 * CE] = F;
+    else d.bvx[2 * CE + 1] = F;
+    for (int p = 0; p < 3; p++) if (d.vb[3 * F + p] == DF) { d.vb[3 * F + p] = CE; break; }
+    int G3, G4;
+    if (grain_contains_vertex(d, gnic[0], A) >= 0) { G3 = gnic[0]; G4 = gnic[1]; }
+    else { G3 = gnic[1]; G4 = gnic[0]; }
+    set_mod(d, A, 0, FIX_REMOVE, G3, A); flag_grain(d, G3, 0);
+    set_mod(d, A, 1, FIX_REMOVE, G3, C);
+    set_mod(d, A, 2, FIX_REMOVE, G3, Dv);
+    set_mod(d, B, 0, FIX_REMOVE, G4, B); flag_grain(d, G4, 0);
+    set_mod(d, B, 1, FIX_REMOVE, G4, C);
+    set_mod(d, B, 2, FIX_REMOVE, G4, Dv);
+    for (int i = 0; i < 3; i++) {
+        d.benab[d.vb[3 * A + i]] = 0;
+        d.benab[d.vb[3 * B + i]] = 0;
+    }
+    d.benab[DF] = 0;
+    d.benab[bnd] = 0;
+    d.ven[A] = 0; d.ven[Dv] = 0; d.ven[C] = 0; d.ven[B] = 0;
+    return CE;
+}
+
+__device__ void sort_ints(int* a, int n) {
+    for (int i = 1; i < n; i++) {
+        int x = a[i], j = i - 1;
+        while (j >= 0 && a[j] > x) { a[j + 1] = a[j]; j--; }
+        a[j + 1] = x;
+    }
+}
+
+__device__ int member_of_edge(const Dev& d, int e) {
+    int m = 0;
+    while (m + 1 < d.B && e >= d.prm[m + 1].eoff) m++;
+    return m;
+}
+
+// The loop of main.cu:227-250 on one lane: serial passes in edge order over
+// the qualifying edges (re-checked when reached, as the <<<1,1>>> scan of
+// topological.h:965-985 sees them), each followed by the list fix-up of the
+// flagged grains.  The next pass's qualifying edges can only be edges of
+// grains whose list was just edited.
+__global__ void k_two_grain(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int n = d.cnt[C_TWO];
+    if (n == 0) return;
+    int* lst = d.l_two;
+    for (int it = 0; it < 20 && n > 0; it++) {
+        sort_ints(lst, n);
+        for (int k = 0; k < n; k++) {
+            int e = lst[k];
+            if (!two_qualifies(d, e)) continue;
+            if (!d.ms[member_of_edge(d, e)].active) continue;
+            int gic[2], gnic[2];
+            grains_intersect(d, d.bvx[2 * e], d.bvx[2 * e + 1], gic);
+            grains_symdiff(d, d.bvx[2 * e], d.bvx[2 * e + 1], gnic);
+            if (gnic[0] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); continue; }
+            int CE = remove_two_grains(d, e, gnic);
+            d.genab[gic[0]] = 0;
+            d.genab[gic[1]] = 0;
+            d.ms[member_of_edge(d, e)].two += 1;
+            // an edge whose endpoint moved may qualify later in this pass
+            if (CE > e && two_qualifies(d, CE)) {
+                bool have = false;
+                for (int q = k + 1; q < n; q++) if (lst[q] == CE) have = true;
+                if (!have) {
+                    int p = n++;
+                    while (p > k + 1 && lst[p - 1] > CE) { lst[p] = lst[p - 1]; p--; }
+                    lst[p] = CE;
+                }
+            }
+        }
+        // fix_grain_lists for the flagged grains
+        const int nf = d.cnt[C_FIX0];
+        for (int q = 0; q < nf; q++) fix_one_grain(d, d.l_fix[0][q]);
+        int nn = 0;
+        for (int q = 0; q < nf; q++) {
+            int g = d.l_fix[0][q];
+            d.gfix[g] = 0;
+            int s = atomicAdd(&d.cnt[C_DIRTY], 1);
+            if (s < d.G) d.l_dirty[s] = g;
+            else set_error(d, SEVC_E_STATE);
+            if (!d.genab[g]) continue;
+            const int* L = glist(d, g);
+            for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
+        }
+        // candidates of the next pass: the edited grains' own edges
+        for (int q = 0; q < nf; q++) {
+            int g = d.l_fix[0][q];
+            if (!d.genab[g]) continue;
+            const int* L = glist(d, g);
+            int len = d.glen[g];
+            for (int k = 0; k < len; k++) {
+                int a = L[k], c = L[(k + 1) % len];
+                for (int r = 0; r < 3; r++) {
+                    int e = d.vb[3 * a + r];
+                    if (other_end(d, e, a) != c) continue;
+                    if (!two_qualifies(d, e)) continue;
+                    bool have = false;
+                    for (int z = 0; z < nn; z++) if (lst[z] == e) have = true;
+                    if (!have) lst[nn++] = e;
+                }
+            }
+        }
+        d.cnt[C_FIX0] = 0;
+        n = nn;
+    }
+}
+
+// ---------------------------------------------------------------------
+// T2: 3-sided grain removal (topological.h:614-668, 687-822, 994-1004)
+// ---------------------------------------------------------------------
+
+// clear_grains_state + label_3sided_grains; candidates are compacted.
+__global__ void k_label3(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
+    const int g = i;
+    uint8_t cand = 0;
+    if (d.glen[g] == 3) {
+        const double dt = member_dt(P, M);
+        const int* L = glist(d, g);
+        int ans = 0;
+        for (int q = 0; q < 3; q++) {
+            int v = L[q];
+            for (int k = 0; k < 3; k++) {
+                int b = d.vb[3 * v + k];
+                int ip = grain_contains_vertex(d, g, d.bvx[2 * b]);
+                int ep = grain_contains_vertex(d, g, d.bvx[2 * b + 1]);
+                if (ip >= 0 && ep >= 0 && d.bcand[b] && d.benab[b]) { cand = 1; break; }
+                if (ip >= 0 && ep >= 0 && is_candidate(d.bt[b], d.blen[b], dt, P.flip_thr) && d.benab[b]) ans++;
+            }
+        }
+        ans /= 2;
+        if (ans == 3) cand = 1;
+    }
+    d.gcand[g] = cand;
+    if (cand) {
+        int s = atomicAdd(&d.cnt[C_T2], 1);
+        d.l_t2[s] = g;
+    }
+}
+
+// Split the T2 candidates: a candidate that shares a vertex with another
+// candidate goes to the serial list.
+__global__ void k_t2_split(Dev d) {
+    const int n = d.cnt[C_T2];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = d.l_t2[k];
+        const int* L = glist(d, g);
+        bool clash = false;
+        for (int q = 0; q < 3 && !clash; q++) {
+            int v = L[q];
+            for (int r = 0; r < 3; r++) {
+                int h = d.vg[3 * v + r];
+                if (h != g && h >= 0 && d.genab[h] && d.gcand[h]) { clash = true; break; }
+            }
+        }
+        if (clash) { int s = atomicAdd(&d.cnt[C_T2C], 1); d.l_t2c[s] = g; }
+        else { int s = atomicAdd(&d.cnt[C_T2P], 1); d.l_t2p[s] = g; }
+    }
+}
+
+// remove_grain, topological.h:687-822
+__device__ void remove_grain(const Dev& d, int g, double D) {
+    const int* Lg = glist(d, g);
+    int A = Lg[0], B = Lg[1], C = Lg[2];
+    double xp0 = d.pos[2 * A], xp1 = d.pos[2 * B], xp2 = d.pos[2 * C];
+    double yp0 = d.pos[2 * A + 1], yp1 = d.pos[2 * B + 1], yp2 = d.pos[2 * C + 1];
+    double nAB = arclen(xp0, yp0, xp1, yp1, D);
+    double nBC = arclen(xp1, yp1, xp2, yp2, D);
+    double nCA = arclen(xp2, yp2, xp0, yp0, D);
+    double denom = nAB + nBC + nCA;
+    // adjust_origin_for_points on (A, B, C)
+    double ux1 = 0 + wrapd(xp1, xp0, D), uy1 = 0 + wrapd(yp1, yp0, D);
+    double ux2 = ux1 + wrapd(xp2, xp1, D), uy2 = uy1 + wrapd(yp2, yp1, D);
+    double ux0 = 0, uy0 = 0;
+    double ax = xp0 + (nBC * ux0 + nCA * ux1 + nAB * ux2) / denom;
+    double ay = yp0 + (nBC * uy0 + nCA * uy1 + nAB * uy2) / denom;
+    d.pos[2 * A] = dom_mod(ax, D);
+    d.pos[2 * A + 1] = dom_mod(ay, D);
+    int gic[2], G1, G2, G3;
+    grains_intersect(d, B, C, gic); G3 = (gic[0] == g) ? gic[1] : gic[0];
+    grains_intersect(d, A, B, gic); G2 = (gic[0] == g) ? gic[1] : gic[0];
+    grains_intersect(d, A, C, gic); G1 = (gic[0] == g) ? gic[1] : gic[0];
+    if (G1 < 0 || G2 < 0 || G3 < 0) { set_error(d, SEVC_E_STATE); return; }
+    d.vel[2 * A] = 0.0;
+    d.vel[2 * A + 1] = 0.0;
+    for (int i = 0; i < 3; i++) {
+        int e = d.vb[3 * C + i];
+        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
+        if ((ei == B && ee == C) || (ee == B && ei == C)) { d.benab[e] = 0; break; }
+    }
+    int BEb = -1, CF = -1;
+    for (int i = 0; i < 3; i++) {
+        int e = d.vb[3 * B + i];
+        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
+        if (!(ei == A || ee == A) && !(ei == C || ee == C)) { BEb = e; break; }
+    }
+    for (int i = 0; i < 3; i++) {
+        int e = d.vb[3 * C + i];
+        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
+        if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = e; break; }
+    }
+    if (BEb < 0 || CF < 0) { set_error(d, SEVC_E_STATE); return; }
+    if (d.bvx[2 * BEb] == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
+    if (d.bvx[2 * CF] == C) d.bvx[2 * CF] = A; else d.bvx[2 * CF + 1] = A;
+    for (int i = 0; i < 3; i++) {
+        int e = d.vb[3 * A + i];
+        if (d.bvx[2 * e] == B || d.bvx[2 * e + 1] == B) { d.benab[e] = 0; d.vb[3 * A + i] = BEb; break; }
+    }
+    for (int i = 0; i < 3; i++) {
+        int e = d.vb[3 * A + i];
+        if (d.bvx[2 * e] == C || d.bvx[2 * e + 1] == C) { d.benab[e] = 0; d.vb[3 * A + i] = CF; break; }
+    }
+    d.ven[B] = 0;
+    d.ven[C] = 0;
+    d.genab[g] = 0;
+    set_mod(d, B, 0, FIX_REMOVE, G2, B); flag_grain(d, G2, 1);
+    set_mod(d, C, 0, FIX_REMOVE, G1, C); flag_grain(d, G1, 1);
+    set_mod(d, B, 1, FIX_REMOVE, G3, B); flag_grain(d, G3, 1);
+    set_mod(d, C, 1, FIX_REPLACE, G3, A);
+    d.mprev[4 * C + 1] = C;
+    for (int i = 0; i < 3; i++) if (d.vg[3 * A + i] == g) d.vg[3 * A + i] = G3;
+}
+
+__device__ int member_of_grain(const Dev& d, int g) {
+    int m = 0;
+    while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
+    return m;
+}
+
+__global__ void k_t2_par(Dev d) {
+    const int n = d.cnt[C_T2P];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = d.l_t2p[k];
+        int m = member_of_grain(d, g);
+        remove_grain(d, g, d.prm[m].D);
+        atomicAdd((unsigned long long*)&d.ms[m].removals, 1ull);
+    }
+}
+
+__global__ void k_t2_serial(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int n = d.cnt[C_T2C];
+    if (n == 0) return;
+    sort_ints(d.l_t2c, n);
+    for (int k = 0; k < n; k++) {
+        int g = d.l_t2c[k];
+        if (!(d.genab[g] && d.gcand[g])) continue;
+        int m = member_of_grain(d, g);
+        remove_grain(d, g, d.prm[m].D);
+        d.ms[m].removals += 1;
+    }
+}
+
+// unlabel_boundaries_st2, topological.h:588-606
+__global__ void k_st2(Dev d) {
+    const int n = d.cnt[C_CAND];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int b = d.l_cand[k];
+        if (!(d.benab[b] && d.bcand[b])) continue;
+        int gic[2];
+        if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) { d.bcand[b] = 0; continue; }
+        if (d.glen[gic[0]] == 3 || d.glen[gic[1]] == 3) d.bcand[b] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------
+// T1: flips (topological.h:174-291, 1016-1026)
+// ---------------------------------------------------------------------
+
+__global__ void k_flip_split(Dev d) {
+    const int n = d.cnt[C_CAND];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int c = d.l_cand[k];
+        if (!(d.benab[c] && d.bcand[c])) continue;
+        bool clash = false;
+        for (int e = 0; e < 2 && !clash; e++) {
+            int v = d.bvx[2 * c + e];
+            for (int q = 0; q < 3; q++) {
+                int x = d.vb[3 * v + q];
+                if (x != c && d.benab[x] && d.bcand[x]) { clash = true; break; }
+            }
+        }
+        if (clash) { int s = atomicAdd(&d.cnt[C_FLIPC], 1); d.l_flipc[s] = c; }
+        else { int s = atomicAdd(&d.cnt[C_FLIPP], 1); d.l_flipp[s] = c; }
+    }
+}
+
+// flip, topological.h:174-291
+__device__ void flip(const Dev& d, int bnd, const Params& P) {
+    const double D = P.D;
+    int A = d.bvx[2 * bnd], B = d.bvx[2 * bnd + 1];
+    int gic[2], gnic[2];
+    grains_intersect(d, A, B, gic);
+    grains_symdiff(d, A, B, gnic);
+    int i = -1, j = -1, AC = -1, BEb = -1;
+    for (int k = 0; k < 3; k++) if (d.vb[3 * A + k] == bnd) { i = (k + 1) % 3; AC = d.vb[3 * A + i]; break; }
+    for (int k = 0; k < 3; k++) if (d.vb[3 * B + k] == bnd) { j = (k + 1) % 3; BEb = d.vb[3 * B + j]; break; }
+    if (i < 0 || j < 0 || gic[1] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); return; }
+    d.vb[3 * A + i] = BEb;
+    d.vb[3 * B + j] = AC;
+    if (d.bvx[2 * AC] == A) d.bvx[2 * AC] = B; else d.bvx[2 * AC + 1] = B;
+    if (d.bvx[2 * BEb] == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
+    int g1, g2, g3, g4;
+    for (g2 = 0; g2 < 2; g2++)
+        if (grain_contains_consecutive(d, gic[g2], A, B) == 1) {
+            set_mod(d, A, 0, FIX_REMOVE, gic[g2], A);
+            flag_grain(d, gic[g2], 2);
+            break;
+        }
+    for (g4 = 0; g4 < 2; g4++)
+        if (grain_contains_consecutive(d, gic[g4], B, A) == 1) {
+            set_mod(d, B, 0, FIX_REMOVE, gic[g4], B);
+            flag_grain(d, gic[g4], 2);
+            break;
+        }
+    for (g3 = 0; g3 < 2; g3++)
+        if (grain_contains_vertex(d, gnic[g3], A) >= 0) {
+            set_mod(d, A, 1, FIX_ADD, gnic[g3], B);
+            d.mprev[4 * A + 1] = A;
+            flag_grain(d, gnic[g3], 2);
+            break;
+        }
+    for (g1 = 0; g1 < 2; g1++)
+        if (grain_contains_vertex(d, gnic[g1], B) >= 0) {
+            set_mod(d, B, 1, FIX_ADD, gnic[g1], A);
+            d.mprev[4 * B + 1] = B;
+            flag_grain(d, gnic[g1], 2);
+            break;
+        }
+    if (g1 > 1 || g2 > 1 || g3 > 1 || g4 > 1) { set_error(d, SEVC_E_STATE); return; }
+    for (int k = 0; k < 3; k++) {
+        if (d.vg[3 * A + k] == gic[g2]) d.vg[3 * A + k] = gnic[g1];
+        if (d.vg[3 * B + k] == gic[g4]) d.vg[3 * B + k] = gnic[g3];
+    }
+    // vector2_mean, geometry.h:311-317
+    double dx = wrapd(d.pos[2 * B], d.pos[2 * A], D);
+    double dy = wrapd(d.pos[2 * B + 1], d.pos[2 * A + 1], D);
+    double cx = d.pos[2 * A] + dx / 2, cy = d.pos[2 * A + 1] + dy / 2;
+    cx = dom_mod(cx, D);
+    cy = dom_mod(cy, D);
+    d.pos[2 * A] = cx; d.pos[2 * A + 1] = cy;
+    d.pos[2 * B] = cx; d.pos[2 * B + 1] = cy;
+    int t = d.vb[3 * A + 1]; d.vb[3 * A + 1] = d.vb[3 * A]; d.vb[3 * A] = t;
+    t = d.vb[3 * B + 1]; d.vb[3 * B + 1] = d.vb[3 * B]; d.vb[3 * B] = t;
+    d.ben[bnd] = boundary_compute_energy(d, bnd, P.gb_eps);   // no GB_scaling, topological.h:286
+    d.vel[2 * A] = 0; d.vel[2 * A + 1] = 0;
+    d.vel[2 * B] = 0; d.vel[2 * B + 1] = 0;
+}
+
+__global__ void k_flip_par(Dev d) {
+    const int n = d.cnt[C_FLIPP];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int c = d.l_flipp[k];
+        int m = member_of_edge(d, c);
+        flip(d, c, d.prm[m]);
+        atomicAdd((unsigned long long*)&d.ms[m].flips, 1ull);
+    }
+}
+
+__global__ void k_flip_serial(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int n = d.cnt[C_FLIPC];
+    if (n == 0) return;
+    sort_ints(d.l_flipc, n);
+    for (int k = 0; k < n; k++) {
+        int c = d.l_flipc[k];
+        if (!(d.benab[c] && d.bcand[c])) continue;
+        int m = member_of_edge(d, c);
+        flip(d, c, d.prm[m]);
+        d.ms[m].flips += 1;
+    }
+}
+
+// ---------------------------------------------------------------------
+// deterministic sums (nucleated area, main.cu:321)
+// ---------------------------------------------------------------------
+
+// Level 1: one lane per 256-grain table block sums its nucleated areas
+// left to right (the chunking of the oracle's sum).
+__global__ void k_nucl_area_l1(Dev d) {
+    int lb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lb >= d.ngb) return;
+    Blk bd = d.gblk[lb];
+    double s = 0.0;
+    for (int g = bd.b; g < bd.e; g++)
+        if (d.genab[g] && d.gtype[g] == T_NUCLEATED) s += d.garea[g];
+    d.partial[lb] = s;
+}
+
+// Upper levels for each member: chunks of 256 partials left to right,
+// repeated until one value remains.  One workgroup per member; levels
+// alternate between the two halves of `partial` so no lane overwrites a
+// value another lane still reads.
+__global__ void k_nucl_area_up(Dev d) {
+    int m = blockIdx.x;
+    if (m >= d.B) return;
+    int first = d.gblk_first[m], cnt = d.gblk_first[m + 1] - first;
+    double* src = d.partial + first;
+    double* dst = d.partial + d.ngb + first;
+    while (cnt > 1) {
+        int nn = (cnt + 255) / 256;
+        for (int c = threadIdx.x; c < nn; c += blockDim.x) {
+            double s = 0.0;
+            int e = (c + 1) * 256 < cnt ? (c + 1) * 256 : cnt;
+            for (int q = c * 256; q < e; q++) s += src[q];
+            dst[c] = s;
+        }
+        __syncthreads();
+        double* t = src; src = dst; dst = t;
+        cnt = nn;
+    }
+    if (threadIdx.x == 0) {
+        const Params& P = d.prm[m];
+        double a = cnt == 1 ? src[0] : 0.0;
+        d.ms[m].nucl_area_pct = 100 * a / (P.D * P.D);
+    }
+}
+
+
+// ---------------------------------------------------------------------
+// nucleation (nucleation2.cu)
+// ---------------------------------------------------------------------
+
+struct NuclGeom {
+    double Apos[2], Bpos[2], Cpos[2];
+    int D, E, F, AD, AE, AF, g1, g2, g3;
+    double dE;
+};
+
+// generate_grain_for_nucleation, nucleation2.cu:116-212
+__device__ int generate_grain(const Dev& d, const Params& P, int A, double eps, double scaling, int do_scale,
+                              NuclGeom* r) {
+    const double Dm = P.D;
+    r->AD = d.vb[3 * A + 0]; r->AE = d.vb[3 * A + 1]; r->AF = d.vb[3 * A + 2];
+    r->D = other_end(d, r->AD, A);
+    r->E = other_end(d, r->AE, A);
+    r->F = other_end(d, r->AF, A);
+    int gAD[2], gAE[2], gAF[2];
+    grains_intersect(d, A, r->D, gAD);
+    grains_intersect(d, A, r->E, gAE);
+    grains_intersect(d, A, r->F, gAF);
+    if (gAD[0] == gAE[0] || gAD[0] == gAE[1]) { r->g2 = gAD[0]; r->g1 = gAD[1]; }
+    else if (gAD[1] == gAE[0] || gAD[1] == gAE[1]) { r->g2 = gAD[1]; r->g1 = gAD[0]; }
+    else return -1;
+    if (gAE[0] == r->g2) r->g3 = gAE[1];
+    else if (gAE[1] == r->g2) r->g3 = gAE[0];
+    else return -1;
+    if (r->g1 < 0 || r->g2 < 0 || r->g3 < 0) return -1;
+    const double ax = d.pos[2 * A], ay = d.pos[2 * A + 1], rr = d.nucl_r[A];
+    const int nb[3] = {r->D, r->E, r->F};
+    double* out[3] = {r->Apos, r->Bpos, r->Cpos};
+    for (int k = 0; k < 3; k++) {
+        double tx = wrapd(d.pos[2 * nb[k]], ax, Dm);
+        double ty = wrapd(d.pos[2 * nb[k] + 1], ay, Dm);
+        double w = rr / mag(tx, ty);
+        if (do_scale) w *= 0.99;
+        double px = w * tx + ax, py = w * ty + ay;
+        out[k][0] = dom_mod(px, Dm);
+        out[k][1] = dom_mod(py, Dm);
+    }
+    double LAB = arclen(r->Apos[0], r->Apos[1], r->Bpos[0], r->Bpos[1], Dm);
+    double LBC = arclen(r->Bpos[0], r->Bpos[1], r->Cpos[0], r->Cpos[1], Dm);
+    double LCA = arclen(r->Cpos[0], r->Cpos[1], r->Apos[0], r->Apos[1], Dm);
+    AreaAcc acc;
+    acc.start(ax, ay); acc.push(r->Cpos[0], r->Cpos[1], Dm); acc.push(r->Apos[0], r->Apos[1], Dm);
+    double A1 = acc.finish();
+    acc.start(ax, ay); acc.push(r->Apos[0], r->Apos[1], Dm); acc.push(r->Bpos[0], r->Bpos[1], Dm);
+    double A2 = acc.finish();
+    acc.start(ax, ay); acc.push(r->Bpos[0], r->Bpos[1], Dm); acc.push(r->Cpos[0], r->Cpos[1], Dm);
+    double A3 = acc.finish();
+    double E0 = 0.0, E1;
+    for (int i = 0; i < 3; i++) {
+        if (do_scale) E0 += 0.99 * rr * d.ben[d.vb[3 * A + i]];
+        else E0 += rr * d.ben[d.vb[3 * A + i]];
+    }
+    E0 += A1 * d.gse[r->g1] + A2 * d.gse[r->g2] + A3 * d.gse[r->g3];
+    const double ori = d.nucl_ori[A];
+    E1 = gb_energy_func(ori - d.gori[r->g1], eps) * scaling * LCA;
+    E1 += gb_energy_func(ori - d.gori[r->g2], eps) * scaling * LAB;
+    E1 += gb_energy_func(ori - d.gori[r->g3], eps) * scaling * LBC;
+    r->dE = E1 - E0;
+    return 0;
+}
+
+// compute_nucleation_factor, nucleation2.cu:239-282, with the random pair
+// Philox(seed; step, local vertex id, stream 1).
+__global__ void k_nucl_factor(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_MC) return;
+    if (i >= bd_.e || !d.ven[i]) return;
+    const int v = i;
+    d.dE[v] = 0;
+    d.nfac[v] = 0;
+    double ALs[3];
+    for (int q = 0; q < 3; q++) ALs[q] = d.blen[d.vb[3 * v + q]];
+    double minAL = ALs[0];
+    for (int q = 0; q < 3; q++) if (minAL > ALs[q]) minAL = ALs[q];
+    double u0, u1;
+    sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(v - P.voff), kStreamMcSite, 0u, &u0, &u1);
+    d.nucl_r[v] = 0.8 * minAL * u0 + 0.1;
+    d.nucl_ori[v] = 2 * SEVC_PI * u1;
+    NuclGeom g0, g1;
+    double de = 0.0, de_ = 0.0;
+    if (generate_grain(d, P, v, P.gb_eps, P.gb_scaling, 0, &g0) == 0) de = g0.dE;
+    d.dE[v] = de;
+    if (generate_grain(d, P, v, P.gb_eps, P.gb_scaling, 1, &g1) == 0) de_ = g1.dE;
+    d.nfac[v] = de - de_;
+}
+
+// Rate mode: grain g of a nucleating member is a candidate with
+// probability nucleation_rate * SE_g (Philox(seed; step, local grain, 3, 0)).
+__global__ void k_rate_cand(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_RATE) return;
+    if (i >= bd_.e || !d.genab[i] || d.glen[i] < 1) return;
+    double p = P.nucl_rate * d.gse[i];
+    double u0, u1;
+    sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(i - P.goff), kStreamRate, 0u, &u0, &u1);
+    if (!(u0 < p)) return;
+    int s = atomicAdd(&d.cnt[C_NUCL], 1);
+    d.l_nucl[s] = i;
+}
+
+// Per table block: number of enabled entities (for free-slot search).
+__global__ void k_block_counts(Dev d, int kind) {
+    const Blk* tbl = kind == 0 ? d.vblk : (kind == 1 ? d.eblk : d.gblk);
+    const int nblk = kind == 0 ? d.nvb : (kind == 1 ? d.neb : d.ngb);
+    const uint8_t* en = kind == 0 ? d.ven : (kind == 1 ? d.benab : d.genab);
+    int* out = kind == 0 ? d.bc_v : (kind == 1 ? d.bc_e : d.bc_g);
+    int lb = blockIdx.x;
+    if (lb >= nblk) return;
+    Blk bd = tbl[lb];
+    if (!d.ms[bd.m].nucl_now) return;
+    int i = bd.b + (int)threadIdx.x;
+    int c = (i < bd.e && en[i]) ? 1 : 0;
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    unsigned long long bal = __ballot(c);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&acc, (int)__popcll(bal));
+    __syncthreads();
+    if (threadIdx.x == 0) out[lb] = acc;
+}
+
+// Collect, in index order, the first R slots of a member whose enable flag
+// equals `want`, cooperatively in one workgroup.  Returns how many were
+// found (<= R).  `sh` is LDS scratch of blockDim.x + 1 ints.
+__device__ int collect_slots(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want,
+                             int R, int* out, int* sh) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int per = (nblk + nt - 1) / nt;
+    const int b0 = first + tid * per;
+    const int b1 = min(first + nblk, b0 + per);
+    int mine = 0;
+    for (int b = b0; b < b1; b++) {
+        int sz = tbl[b].e - tbl[b].b;
+        mine += want ? bc[b] : sz - bc[b];
+    }
+    sh[tid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int t = 0; t < nt; t++) { int x = sh[t]; sh[t] = run; run += x; }
+        sh[nt] = run;
+    }
+    __syncthreads();
+    int pos = sh[tid];
+    const int total = sh[nt];
+    if (pos < R) {
+        for (int b = b0; b < b1 && pos < R; b++) {
+            int sz = tbl[b].e - tbl[b].b;
+            int cntb = want ? bc[b] : sz - bc[b];
+            if (cntb == 0) continue;
+            for (int i = tbl[b].b; i < tbl[b].e && pos < R; i++)
+                if ((en[i] != 0) == (want != 0)) out[pos++] = i;
+        }
+    }
+    __syncthreads();
+    return total < R ? total : R;
+}
+
+// nucleate, nucleation2.cu:474-676 (after its checks), with given slots.
+__device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, int AB, int BC, int CA, int g) {
+    NuclGeom r;
+    // nucleation2.cu:526 passes (scaling, eps) swapped; only dE is affected
+    if (generate_grain(d, P, A, P.gb_scaling, P.gb_eps, 1, &r) != 0) return -1;
+    // init_vertex (geometry.h:346-361) for B and C
+    const int nv2[2] = {B, C};
+    const double* np2[2] = {r.Bpos, r.Cpos};
+    for (int q = 0; q < 2; q++) {
+        int v = nv2[q];
+        d.pos[2 * v] = np2[q][0]; d.pos[2 * v + 1] = np2[q][1];
+        d.vel[2 * v] = 0.0; d.vel[2 * v + 1] = 0.0;
+        d.venergy[v] = 0.0;
+        d.ven[v] = 1;
+        d.vnuc[v] = 0;
+        for (int k = 0; k < 3; k++) { d.vg[3 * v + k] = -1; d.vb[3 * v + k] = -1; }
+        reset_mods(d, v);
+    }
+    d.pos[2 * A] = r.Apos[0]; d.pos[2 * A + 1] = r.Apos[1];
+    for (int k = 0; k < 3; k++) d.vb[3 * A + k] = -1;
+    auto add_bnd = [&](int v, int b) {
+        for (int k = 0; k < 3; k++) if (d.vb[3 * v + k] < 0) { d.vb[3 * v + k] = b; return; }
+        set_error(d, SEVC_E_STATE);
+    };
+    auto add_grn = [&](int v, int gg) {
+        for (int k = 0; k < 3; k++) if (d.vg[3 * v + k] < 0) { d.vg[3 * v + k] = gg; return; }
+        set_error(d, SEVC_E_STATE);
+    };
+    auto init_bnd = [&](int b, int ini, int end) {          // geometry.h:491-504
+        d.bvx[2 * b] = ini; add_bnd(ini, b);
+        d.bvx[2 * b + 1] = end; add_bnd(end, b);
+        d.blen[b] = 0;
+        d.bt[b] = SEVC_MAX_T_EXT;
+        d.benab[b] = 1;
+        d.bcand[b] = 0;
+        d.conf[b] = -1;
+    };
+    init_bnd(AB, A, B);
+    init_bnd(BC, B, C);
+    init_bnd(CA, C, A);
+    add_bnd(A, r.AD);
+    int* Lg = glist(d, g);                                  // nucleate_grain, geometry.h:596-606
+    for (int k = 0; k < d.S; k++) Lg[k] = -1;
+    d.glen[g] = 0;
+    d.garea[g] = 0;
+    d.gse[g] = 0;
+    d.gori[g] = 0;
+    d.genab[g] = 1;
+    d.gcand[g] = 0;
+    d.gfix[g] = 0;
+    d.gtype[g] = T_NUCLEATED;
+    d.gori[g] = d.nucl_ori[A];
+    Lg[0] = A; Lg[1] = B; Lg[2] = C;
+    d.glen[g] = 3;
+    for (int k = 0; k < 3; k++) if (d.vg[3 * A + k] == r.g3) { d.vg[3 * A + k] = g; break; }
+    add_grn(B, g); add_grn(B, r.g2); add_grn(B, r.g3);
+    add_grn(C, g); add_grn(C, r.g3); add_grn(C, r.g1);
+    list_add_after(d, r.g1, C, A);
+    if (d.bvx[2 * r.AE] == A) list_add_after(d, r.g2, B, d.bvx[2 * r.AE + 1]);
+    else if (d.bvx[2 * r.AE + 1] == A) list_add_after(d, r.g2, B, d.bvx[2 * r.AE]);
+    list_add_after(d, r.g3, C, A);
+    list_add_after(d, r.g3, B, C);
+    list_remove(d, r.g3, A);
+    int t = d.vb[3 * B + 1]; d.vb[3 * B + 1] = d.vb[3 * B]; d.vb[3 * B] = t;
+    t = d.vb[3 * C + 1]; d.vb[3 * C + 1] = d.vb[3 * C]; d.vb[3 * C] = t;
+    if (d.bvx[2 * r.AE] == A) d.bvx[2 * r.AE] = B;
+    else if (d.bvx[2 * r.AE + 1] == A) d.bvx[2 * r.AE + 1] = B;
+    d.vb[3 * B + 2] = r.AE;
+    if (d.bvx[2 * r.AF] == A) d.bvx[2 * r.AF] = C;
+    else d.bvx[2 * r.AF + 1] = C;
+    d.vb[3 * C + 2] = r.AF;
+    return 0;
+}
+
+// One workgroup per member: picks the nucleation events of this step,
+// assigns free slots and applies the events.
+//  MC mode (nucleation2.cu:292-443, 33-72, 474): one uniformly chosen alive
+//  vertex, Metropolis acceptance, first free grain / 3 boundaries / 2
+//  vertices.
+//  RATE mode: candidates in grain order; a candidate is dropped when an
+//  earlier candidate's site shares a grain with it; kept events take the
+//  free slots in rank order and, being grain-disjoint, are applied in
+//  parallel.
+// Scratch: 12 ints per grain slot of the member, at 12*goff in d.nscr:
+//  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
+//  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free vertices.
+__global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
+    __shared__ int sh[257];
+    __shared__ int nkeep, nmade, fg_n, fe_n, fv_n;
+    const int m = blockIdx.x;
+    if (m >= d.B) return;
+    const Params& P = d.prm[m];
+    MState& M = d.ms[m];
+    if (!M.active || !M.nucl_now) return;
+    const int tid = threadIdx.x;
+    const int ng = P.ng;
+    int* base = d.nscr + 12 * (size_t)P.goff;
+    int* cand = base;
+    int* sgr = base + ng;
+    int* kept = base + 4 * (size_t)ng;
+    int* fg = base + 5 * (size_t)ng;
+    int* fe = base + 6 * (size_t)ng;
+    int* fv = base + 9 * (size_t)ng;
+    if (tid == 0) {
+        nkeep = 0;
+        nmade = 0;
+        if (P.nucl_mode == SEVC_NUCL_MC) {
+            const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
+            int n_alive = 0;
+            for (int b = first; b < first + nb; b++) n_alive += d.bc_v[b];
+            if (n_alive > 0) {
+                double u0, u1;
+                sevc_uniform2(P.seed, (uint32_t)M.n, 0u, kStreamMcPick, 0u, &u0, &u1);
+                int k = (int)(u0 * (double)n_alive);
+                if (k >= n_alive) k = n_alive - 1;
+                int A = -1;
+                for (int b = first; b < first + nb && A < 0; b++) {
+                    if (k >= d.bc_v[b]) { k -= d.bc_v[b]; continue; }
+                    for (int i = d.vblk[b].b; i < d.vblk[b].e; i++)
+                        if (d.ven[i]) { if (k == 0) { A = i; break; } k--; }
+                }
+                if (A >= 0) {
+                    d.vnuc[A] = 0;
+                    if (d.dE[A] < 0) d.vnuc[A] = 1;
+                    else if (d.nfac[A] < 0) {
+                        double thr = sevc_exp(-P.mc_k * d.dE[A]);
+                        if (u1 < thr) d.vnuc[A] = 1;
+                    }
+                    if (d.vnuc[A]) { kept[0] = A; nkeep = 1; }
+                }
+            }
+        } else {
+            const int total = d.cnt[C_NUCL];
+            int nc = 0;
+            for (int q = 0; q < total; q++) {
+                int g = d.l_nucl[q];
+                if (g >= P.goff && g < P.goff + ng) cand[nc++] = g;
+            }
+            sort_ints(cand, nc);
+            int nk = 0;
+            for (int e = 0; e < nc; e++) {
+                const int g = cand[e];
+                double u0, u1, u2, u3;
+                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
+                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
+                const int len = d.glen[g];
+                int k = (int)(u1 * (double)len);
+                if (k >= len) k = len - 1;
+                const int A = glist(d, g)[k];
+                for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
+                bool clash = false;
+                for (int f = 0; f < e && !clash; f++)
+                    for (int a = 0; a < 3 && !clash; a++)
+                        for (int b = 0; b < 3; b++)
+                            if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
+                if (clash) continue;
+                double minAL = d.blen[d.vb[3 * A]];
+                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+                d.nucl_r[A] = P.nucl_scaling * minAL * u2;
+                d.nucl_ori[A] = 2 * SEVC_PI * u3;
+                kept[nk++] = A;
+            }
+            nkeep = nk;
+        }
+    }
+    __syncthreads();
+    const int K = nkeep;
+    if (K == 0) return;
+    // free slots in rank order, taken before any insertion of this step
+    int a = collect_slots(d.gblk, d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], d.bc_g, d.genab, 0,
+                          K, fg, sh);
+    int b = collect_slots(d.eblk, d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], d.bc_e, d.benab, 0,
+                          3 * K, fe, sh);
+    int c = collect_slots(d.vblk, d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], d.bc_v, d.ven, 0,
+                          2 * K, fv, sh);
+    if (tid == 0) { fg_n = a; fe_n = b; fv_n = c; }
+    __syncthreads();
+    for (int r = tid; r < K; r += blockDim.x) {
+        if (r + 1 > fg_n || 3 * r + 3 > fe_n || 2 * r + 2 > fv_n) continue;
+        if (nucleate_at(d, P, kept[r], fv[2 * r], fv[2 * r + 1], fe[3 * r], fe[3 * r + 1], fe[3 * r + 2], fg[r]) == 0)
+            atomicAdd(&nmade, 1);
+    }
+    __syncthreads();
+    if (tid == 0) M.nucl += nmade;
+}
+
+}  // namespace sevc
+
+namespace sevc {
+__global__ void k_selftest(int op, int n, const double* x, const double* y, double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = x[i], b = y[i], r = 0.0;
+    switch (op) {
+        case 0: r = sevc_atan2(a, b); break;
+        case 1: r = sevc_sin(a); break;
+        case 2: r = sevc_cos(a); break;
+        case 3: r = sevc_exp(a); break;
+        case 4: r = a / b; break;
+        case 5: r = sqrt(a); break;
+        case 6: r = mag(a, b); break;
+        case 7: r = wrapd(a, b, 1.0); break;
+        case 8: {
+            uint32_t c[4] = {(uint32_t)a, (uint32_t)b, 0u, 0u}, k[2] = {1u, 2u}, o[4];
+            sevc_philox4x32_10(c, k, o);
+            r = (double)o[0];
+            break;
+        }
+        default: r = 0.0;
+    }
+    out[i] = r;
+}
+}  // namespace sevc

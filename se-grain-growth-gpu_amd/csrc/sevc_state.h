@@ -1,0 +1,71 @@
+// sevc_state.h -- device state of a (batched) vertex-model simulation.
+//
+// All members of a batch live in one set of structure-of-arrays buffers;
+// topology indices are global slot ids (member offset added at upload), so
+// every per-entity kernel covers all members in one launch.  Per-member
+// parameters and loop state are small device arrays read once per
+// workgroup through the block tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sevc {
+
+constexpr int kBlock = 256;          // elements per table block (one workgroup)
+constexpr int kFixStages = 3;        // 0: two-grain, 1: T2 removal, 2: flips
+
+// Per-member run parameters (from sevc_options).
+struct Params {
+    double D, dt_max, dt_min, dt_safety, flip_thr, mobility;
+    double gb_eps, gb_scaling, mc_k, nucl_rate, nucl_scaling, tmax;
+    unsigned long long seed;
+    int nmax, adaptive, nucl_mode, do_nucl, gap, events, vel_model, min_grains, n_grains0;
+    int voff, nv, eoff, ne, goff, ng;
+};
+
+// Per-member loop state (main.cu:30-113), updated on the device.
+struct MState {
+    int n, active, done, curr_grains, allow_nucl, nucl_now, status, n_venabled;
+    double t, dt;
+    unsigned long long tau_bits;     // min positive t_ext (as ordered bits)
+    long long flips, removals, two, nucl, vsteps, steps;
+    double nucl_area_pct;
+};
+
+// A workgroup's range of one member's entities: [b, e).
+struct Blk { int m, b, e; };
+
+enum Counter {
+    C_CAND = 0, C_TWO, C_T2, C_T2P, C_T2C, C_FLIPP, C_FLIPC,
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_ANYNUCL, C_ERR, C_COUNT
+};
+
+struct Dev {
+    // vertices
+    double* pos; double* vel; int* vb; int* vg; uint8_t* ven;
+    double* venergy; double* vparts;
+    uint8_t* mfix; int* mgrn; int* mvrt; int* mprev;
+    double* nucl_r; double* nucl_ori; double* dE; double* nfac; uint8_t* vnuc;
+    // boundaries
+    int* bvx; double* ben; double* blen; double* bt; uint8_t* benab; uint8_t* bcand; int* conf;
+    // grains
+    int* gvx; int* glen; double* garea; double* gdadt; double* gori; double* gse;
+    uint8_t* genab; uint8_t* gtype; uint8_t* gcand; int* gfix;
+    int S;
+    int V, E, G;                          // total slots over members
+    // members
+    const Params* prm; MState* ms; int B;
+    const Blk* vblk; const Blk* eblk; const Blk* gblk;
+    int nvb, neb, ngb;
+    int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
+    // work lists and scratch
+    int* cnt;
+    int* l_cand; int* l_two; int* l_t2; int* l_t2p; int* l_t2c; int* l_flipp; int* l_flipc;
+    int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf;
+    int* bc_v; int* bc_e; int* bc_g;      // per table block counts (k-th selection)
+    double* partial;                      // per grain block partial sums (2 levels)
+    int* nscr;                            // nucleation scratch, 12 ints per grain slot
+    int diag;                             // compute export diagnostics this step
+};
+
+}  // namespace sevc

@@ -1,0 +1,6 @@
+"""sevc: MI355X-native vertex-model time step (HIP/CDNA4, FP64) behind the
+C ABI of include/sevc.h.  This package is the thin Python host binding."""
+from ._abi import (HostMesh, Mesh, Options, StepStats, NUCL_MC, NUCL_RATE, VEL_FD,  # noqa: F401
+                   VEL_ANALYTIC)
+from .lib import (SevcError, State, build_mesh, check_mesh, default_options, lib,  # noqa: F401
+                  load_files, options_from_ini, selftest_math, voronoi)

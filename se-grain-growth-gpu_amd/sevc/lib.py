@@ -1,0 +1,222 @@
+"""ctypes binding of libsevc.so (the C ABI of include/sevc.h).
+
+The product path is the native library: importing this module fails loudly
+when it has not been built, and nothing here falls back to Python or CPU
+arithmetic."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import HostMesh, Mesh, Options, StepStats
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libsevc.so")
+
+_lib = None
+
+
+class SevcError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"sevc error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libsevc.so (built in-tree by sevc.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "sevc_options_default": (None, [P(Options)]),
+        "sevc_options_from_ini": (C.c_int, [C.c_char_p, P(Options)]),
+        "sevc_mesh_alloc": (C.c_int, [P(Mesh), C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "sevc_mesh_free": (None, [P(Mesh)]),
+        "sevc_mesh_load_files": (C.c_int, [P(Options), P(Mesh)]),
+        "sevc_mesh_build": (C.c_int, [P(Options), C.c_int32, P(C.c_double), C.c_int32, P(C.c_int32),
+                                      C.c_int32, P(C.c_double), P(C.c_double), P(Mesh)]),
+        "sevc_voronoi_generate": (C.c_int, [C.c_int32, C.c_uint64, P(C.c_int32), P(P(C.c_double)),
+                                            P(C.c_int32), P(P(C.c_int32))]),
+        "sevc_free": (None, [C.c_void_p]),
+        "sevc_check_mesh": (C.c_int, [P(Mesh), P(C.c_double)]),
+        "sevc_export_snapshot": (C.c_int, [P(Mesh), P(Options), C.c_char_p, C.c_int32, C.c_double]),
+        "sevc_create": (C.c_int, [P(Options), P(Mesh), P(C.c_void_p)]),
+        "sevc_batch_create": (C.c_int, [P(Options), P(Mesh), C.c_int32, P(C.c_void_p)]),
+        "sevc_step": (C.c_int, [C.c_void_p, C.c_int32, P(StepStats)]),
+        "sevc_step_graph": (C.c_int, [C.c_void_p, C.c_int32, P(StepStats)]),
+        "sevc_member_stats": (C.c_int, [C.c_void_p, C.c_int32, P(StepStats)]),
+        "sevc_member_sizes": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                        P(C.c_int32)]),
+        "sevc_n_members": (C.c_int32, [C.c_void_p]),
+        "sevc_get_mesh": (C.c_int, [C.c_void_p, C.c_int32, P(Mesh)]),
+        "sevc_set_member_params": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
+                                             C.c_double, C.c_double]),
+        "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
+        "sevc_enable_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+        "sevc_kernel_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double), P(C.c_int64)]),
+        "sevc_destroy": (None, [C.c_void_p]),
+        "sevc_last_error": (C.c_char_p, [C.c_void_p]),
+        "sevc_strerror": (C.c_char_p, [C.c_int]),
+        "sevc_device_count": (C.c_int, []),
+        "sevc_selftest_math": (C.c_int, [C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, state=None):
+    if rc != 0:
+        msg = lib().sevc_strerror(rc).decode()
+        if state:
+            msg += " / " + lib().sevc_last_error(state).decode()
+        raise SevcError(rc, msg)
+
+
+def default_options():
+    o = Options()
+    lib().sevc_options_default(C.byref(o))
+    return o
+
+
+def options_from_ini(path, base=None):
+    o = base.copy() if base is not None else default_options()
+    _check(lib().sevc_options_from_ini(path.encode(), C.byref(o)))
+    return o
+
+
+def _take_mesh(m):
+    h = HostMesh.from_struct(m)
+    lib().sevc_mesh_free(C.byref(m))
+    return h
+
+
+def load_files(opts):
+    m = Mesh()
+    _check(lib().sevc_mesh_load_files(C.byref(opts), C.byref(m)))
+    return _take_mesh(m)
+
+
+def build_mesh(opts, unit_pos, ridges, orientations01, se_raw, n_grains_expected=0):
+    unit_pos = np.ascontiguousarray(unit_pos, dtype=np.float64).reshape(-1)
+    ridges = np.ascontiguousarray(ridges, dtype=np.int32).reshape(-1)
+    ori = np.ascontiguousarray(orientations01, dtype=np.float64)
+    se = np.ascontiguousarray(se_raw, dtype=np.float64)
+    m = Mesh()
+    P = C.POINTER
+    _check(lib().sevc_mesh_build(C.byref(opts), unit_pos.size // 2, unit_pos.ctypes.data_as(P(C.c_double)),
+                                 ridges.size // 2, ridges.ctypes.data_as(P(C.c_int32)), int(n_grains_expected),
+                                 ori.ctypes.data_as(P(C.c_double)), se.ctypes.data_as(P(C.c_double)),
+                                 C.byref(m)))
+    return _take_mesh(m)
+
+
+def voronoi(n_seeds, seed):
+    """Periodic Voronoi tessellation: (unit vertex positions (nv,2), ridges (nr,2))."""
+    nv, nr = C.c_int32(), C.c_int32()
+    pp, rp = C.POINTER(C.c_double)(), C.POINTER(C.c_int32)()
+    _check(lib().sevc_voronoi_generate(int(n_seeds), int(seed), C.byref(nv), C.byref(pp), C.byref(nr),
+                                       C.byref(rp)))
+    pos = np.ctypeslib.as_array(pp, shape=(nv.value * 2,)).copy().reshape(-1, 2)
+    rid = np.ctypeslib.as_array(rp, shape=(nr.value * 2,)).copy().reshape(-1, 2)
+    lib().sevc_free(C.cast(pp, C.c_void_p))
+    lib().sevc_free(C.cast(rp, C.c_void_p))
+    return pos, rid
+
+
+def check_mesh(h):
+    ta = C.c_double()
+    n = lib().sevc_check_mesh(C.byref(h.struct), C.byref(ta))
+    return n, ta.value
+
+
+class State:
+    """A device state holding one or more members (sevc_batch_create)."""
+
+    def __init__(self, opts, meshes):
+        if isinstance(meshes, HostMesh):
+            meshes, opts = [meshes], [opts]
+        self.meshes_in = meshes
+        arr_o = (Options * len(meshes))(*[o.copy() for o in opts])
+        self._structs = [m.struct for m in meshes]
+        arr_m = (Mesh * len(meshes))(*self._structs)
+        h = C.c_void_p()
+        rc = lib().sevc_batch_create(arr_o, arr_m, len(meshes), C.byref(h))
+        _check(rc)
+        self.h = h
+        self.n_members = len(meshes)
+
+    def step(self, n, graph=False):
+        st = StepStats()
+        f = lib().sevc_step_graph if graph else lib().sevc_step
+        _check(f(self.h, int(n), C.byref(st)), self.h)
+        return st
+
+    def member_stats(self, m):
+        st = StepStats()
+        _check(lib().sevc_member_stats(self.h, int(m), C.byref(st)), self.h)
+        return st
+
+    def sizes(self, m=0):
+        a = [C.c_int32() for _ in range(4)]
+        _check(lib().sevc_member_sizes(self.h, int(m), *[C.byref(x) for x in a]), self.h)
+        return tuple(x.value for x in a)
+
+    def get_mesh(self, m=0):
+        nv, ne, ng, S = self.sizes(m)
+        h = HostMesh(nv, ne, ng, S)
+        _check(lib().sevc_get_mesh(self.h, int(m), C.byref(h.struct)), self.h)
+        h.D = h.struct.domain_bound
+        return h
+
+    def request_diagnostics(self):
+        _check(lib().sevc_request_diagnostics(self.h), self.h)
+
+    def set_params(self, m=0, pos=None, g_SE=None, gb_scaling=-1.0, mobility=-1.0):
+        P = C.POINTER(C.c_double)
+        pp = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64)
+        sp = None if g_SE is None else np.ascontiguousarray(g_SE, dtype=np.float64)
+        _check(lib().sevc_set_member_params(self.h, int(m), None if pp is None else pp.ctypes.data_as(P),
+                                            None if sp is None else sp.ctypes.data_as(P), float(gb_scaling),
+                                            float(mobility)), self.h)
+
+    def enable_timing(self, on=True):
+        _check(lib().sevc_enable_kernel_timing(self.h, 1 if on else 0), self.h)
+
+    def kernel_times(self):
+        n = 64
+        names = (C.c_char * 48 * n)()
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        k = lib().sevc_kernel_times(self.h, n, C.cast(names, C.c_void_p), ms, cnt)
+        return {bytes(names[i]).split(b"\0")[0].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sevc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def selftest_math(op, x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros_like(x)
+    P = C.POINTER(C.c_double)
+    _check(lib().sevc_selftest_math(int(op), x.size, x.ctypes.data_as(P), y.ctypes.data_as(P),
+                                    out.ctypes.data_as(P)))
+    return out

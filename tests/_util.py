@@ -1,0 +1,143 @@
+"""Test helpers: the oracle binding (test infrastructure) and state
+comparison utilities."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+REF2K = os.path.join(GOLDEN, "ref2k")
+ORACLE_SO = os.path.join(REPO, "oracle", "build", "liboracle.so")
+
+import sevc  # noqa: E402
+from sevc._abi import HostMesh, Mesh, Options, StepStats  # noqa: E402
+
+_orc = None
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
+        L = C.CDLL(ORACLE_SO)
+        P = C.POINTER
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [P(Options), P(Mesh)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int32, P(StepStats)]
+        L.orc_get_mesh.argtypes = [C.c_void_p, P(Mesh)]
+        L.orc_compute_velocities.argtypes = [C.c_void_p]
+        L.orc_request_diagnostics.argtypes = [C.c_void_p]
+        L.orc_load_files.argtypes = [P(Options), P(Mesh)]
+        L.orc_build_mesh.argtypes = [P(Options), C.c_int32, P(C.c_double), C.c_int32, P(C.c_int32), C.c_int32,
+                                     P(C.c_double), P(C.c_double), P(Mesh)]
+        L.orc_mesh_free.argtypes = [P(Mesh)]
+        L.orc_check_mesh.argtypes = [P(Mesh), P(C.c_double)]
+        for f in ("orc_wrap_dist", "orc_dom_mod"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_mag.restype = C.c_double
+        L.orc_mag.argtypes = [C.c_double, C.c_double]
+        L.orc_polygon_area.restype = C.c_double
+        L.orc_polygon_area.argtypes = [P(C.c_double), P(C.c_double), C.c_int32, C.c_double]
+        L.orc_sum_chunked.restype = C.c_double
+        L.orc_sum_chunked.argtypes = [P(C.c_double), P(C.c_uint8), C.c_int32]
+        L.orc_atan2.restype = C.c_double
+        L.orc_atan2.argtypes = [C.c_double, C.c_double]
+        for f in ("orc_sin", "orc_cos", "orc_exp"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.c_double]
+        L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        _orc = L
+    return _orc
+
+
+class Oracle:
+    """The serial FP64 restatement of the reference step."""
+
+    def __init__(self, opts, mesh):
+        self.L = oracle_lib()
+        self.template = mesh
+        self.h = self.L.orc_create(C.byref(opts), C.byref(mesh.struct))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+
+    def step(self, n):
+        st = StepStats()
+        rc = self.L.orc_step(self.h, int(n), C.byref(st))
+        return rc, st
+
+    def velocities(self):
+        return self.L.orc_compute_velocities(self.h)
+
+    def request_diagnostics(self):
+        self.L.orc_request_diagnostics(self.h)
+
+    def get_mesh(self):
+        t = self.template
+        out = HostMesh(t.nv, t.ne, t.ng, t.S, t.D)
+        rc = self.L.orc_get_mesh(self.h, C.byref(out.struct))
+        assert rc == 0
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def oracle_load(opts):
+    m = Mesh()
+    rc = oracle_lib().orc_load_files(C.byref(opts), C.byref(m))
+    if rc != 0:
+        return rc, None
+    h = HostMesh.from_struct(m)
+    oracle_lib().orc_mesh_free(C.byref(m))
+    return 0, h
+
+
+def ref2k_options(se="uniform", **kw):
+    o = sevc.default_options()
+    o.vertex_fname = os.path.join(REF2K, "vertices.txt").encode()
+    o.boundary_fname = os.path.join(REF2K, "ridges.txt").encode()
+    o.orientation_fname = os.path.join(REF2K, "oris.txt").encode()
+    o.SE_fname = os.path.join(REF2K, "SE", se + ".txt").encode()
+    o.n_grains = 2000
+    o.GB_eps = 0.0
+    o.SE_eps = 0.5
+    o.do_nucleation = 0
+    o.dt = 1e-5
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# Fields compared bit for bit after a step.  Diagnostic-only arrays are
+# compared where the test asks for them.
+STATE_FIELDS = ["pos", "vel", "v_bnd", "v_grn", "v_enabled", "b_vtx", "b_enabled", "b_candidate", "b_t_ext",
+                "b_arclength", "b_energy", "g_vtx", "g_vlen", "g_enabled", "g_area", "g_SE", "g_orientation",
+                "g_type"]
+
+
+def diff_fields(a, b, fields=STATE_FIELDS):
+    bad = {}
+    for f in fields:
+        x, y = a.a[f], b.a[f]
+        if x.dtype.kind == "f":
+            neq = ~((x == y) | (np.isnan(x) & np.isnan(y)))
+            # +0.0 and -0.0 compare equal; require identical bits as well
+            neq |= (x.view(np.uint64) != y.view(np.uint64))
+        else:
+            neq = x != y
+        if neq.any():
+            bad[f] = int(neq.sum())
+    return bad
