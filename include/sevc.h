@@ -257,24 +257,11 @@ int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
  * dA/dt and velocity parts, calculus.h:123-133,243-253,62-69). */
 int sevc_request_diagnostics(sevc_state* s);
 
-/* Adjoint over a topology-fixed window (extension).  Runs `steps` steps of
- * velocity+translation with the analytic velocity from the current state,
- * checkpointing every ckpt_every steps, evaluates the area-histogram loss
- * against `target_hist` (n_bins values, smooth linear binning of A/<A> on
- * [0, hist_max)) and returns the loss and its gradients with respect to
- * the global boundary-energy scale, the mobility and every grain's stored
- * energy (g_SE_grad has n_grains slots of member 0). */
-int sevc_adjoint_window(sevc_state* s, int32_t steps, int32_t ckpt_every, double dt,
-                        int32_t n_bins, double hist_max, const double* target_hist,
-                        double* loss, double* d_gamma, double* d_mobility, double* g_SE_grad);
-
-/* Area histogram of member 0 with the binning used by the adjoint loss. */
-int sevc_area_histogram(sevc_state* s, int32_t n_bins, double hist_max, double* hist);
-
 /* Device time (ms) of the kernels of the last call, by name; returns the
  * number of entries written.  Used by the benchmark for roofline figures. */
 int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], double* ms,
                       int64_t* launches);
+/* on: 0 off, 1 every kernel, 2 only velocity / translate / text. */
 int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
 
 /* Device self-test of the shared arithmetic: evaluates op on n inputs on

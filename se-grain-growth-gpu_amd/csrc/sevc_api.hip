@@ -18,7 +18,8 @@ namespace sevc {
 __global__ void k_begin(Dev d);
 __global__ void k_count_grains(Dev d);
 __global__ void k_count_vertices(Dev d);
-__global__ void k_end_prepare(Dev d);
+__global__ void k_tau_partial(Dev d);
+__global__ void k_member_reduce(Dev d, int what);
 __global__ void k_end(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
@@ -48,6 +49,7 @@ __global__ void k_nucl_factor(Dev d);
 __global__ void k_rate_cand(Dev d);
 __global__ void k_block_counts(Dev d, int kind);
 __global__ void k_nucl_manager(Dev d);
+__global__ void k_nucl_refresh(Dev d);
 }  // namespace sevc
 
 using namespace sevc;
@@ -66,7 +68,7 @@ struct sevc_state {
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
     std::string err;
-    bool any_nucl = false, any_events = false;
+    bool any_nucl = false, any_events = false, any_adaptive = false;
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
     // graph replay
@@ -74,6 +76,7 @@ struct sevc_state {
     hipGraph_t graph = nullptr;
     // kernel timing
     bool timing = false;
+    bool timing_hot_only = false;
     std::vector<KTimer> pending;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -108,7 +111,7 @@ int dalloc(sevc_state* s, T** p, size_t n) {
 hipEvent_t take_event(sevc_state* s) {
     if (s->pool_used == s->pool.size()) {
         hipEvent_t e;
-        hipEventCreate(&e);
+        (void)hipEventCreate(&e);
         s->pool.push_back(e);
     }
     return s->pool[s->pool_used++];
@@ -118,11 +121,15 @@ hipEvent_t take_event(sevc_state* s) {
 template <typename K, typename... Args>
 void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args... args) {
     if (grid <= 0) return;
-    if (s->timing) {
+    bool timed = s->timing;
+    if (timed && s->timing_hot_only)
+        timed = std::strcmp(name, "velocity") == 0 || std::strcmp(name, "translate") == 0 ||
+                std::strcmp(name, "text") == 0;
+    if (timed) {
         KTimer t{name, take_event(s), take_event(s)};
-        hipEventRecord(t.a, s->stream);
+        (void)hipEventRecord(t.a, s->stream);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, args...);
-        hipEventRecord(t.b, s->stream);
+        (void)hipEventRecord(t.b, s->stream);
         s->pending.push_back(t);
     } else {
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, args...);
@@ -132,7 +139,7 @@ void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args
 void collect_timers(sevc_state* s) {
     for (auto& t : s->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, t.a, t.b);
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
         auto& e = s->ktimes[t.name];
         e.first += ms;
         e.second += 1;
@@ -159,13 +166,16 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "block_counts", k_block_counts, d.neb, kBlock, d, 1);
         launch(s, "block_counts", k_block_counts, d.ngb, kBlock, d, 2);
         launch(s, "nucl_manager", k_nucl_manager, s->B, 256, d);
-        launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 1);
-        launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 1);
+        launch(s, "nucl_refresh", k_nucl_refresh, kListGrid, 256, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     }
     launch(s, "velocity", k_velocity, d.nvb, kBlock, d);
     if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
     launch(s, "text", k_text, d.neb, kBlock, d);
+    if (s->any_adaptive) {
+        launch(s, "tau_partial", k_tau_partial, d.neb, kBlock, d);
+        launch(s, "member_reduce", k_member_reduce, s->B, 256, d, 0);
+    }
     if (s->any_events) {
         launch(s, "label", k_label, d.neb, kBlock, d);
         launch(s, "poll", k_poll, 1, 1024, d);
@@ -187,9 +197,9 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 2, 0);
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
-    launch(s, "end_prepare", k_end_prepare, nb, 64, d);
     launch(s, "count_grains", k_count_grains, d.ngb, kBlock, d);
     launch(s, "count_vertices", k_count_vertices, d.nvb, kBlock, d);
+    launch(s, "member_reduce", k_member_reduce, s->B, 256, d, 1);
     launch(s, "end", k_end, nb, 64, d);
 }
 
@@ -272,6 +282,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         if (opts[m].dt <= 0) { delete s; return SEVC_E_INVALID_ARG; }
         if (opts[m].events && opts[m].do_nucleation) s->any_nucl = true;
         if (opts[m].events) s->any_events = true;
+        if (opts[m].adaptive_dt) s->any_adaptive = true;
     }
     if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
     const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
@@ -297,6 +308,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     for (int k = 0; k < kFixStages; ++k) DA(d.l_fix[k], G);
     DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
     DA(d.nscr, 12 * G + 16);
+    DA(d.l_nde, 6 * G + 16); DA(d.l_ndg, 4 * G + 16);
 
     // block tables
     std::vector<Blk> vt, et, gt;
@@ -310,6 +322,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.vblk_first, vf.size()); DA(d.eblk_first, ef.size()); DA(d.gblk_first, gf.size());
     DA(d.bc_v, vt.size()); DA(d.bc_e, et.size()); DA(d.bc_g, gt.size());
     DA(d.partial, 2 * gt.size() + 2);
+    DA(d.bmin, et.size()); DA(d.bcnt_g, gt.size()); DA(d.bcnt_v, vt.size());
     d.vblk = dvt; d.eblk = det; d.gblk = dgt;
     Params* dprm;
     DA(dprm, static_cast<size_t>(B));
@@ -431,12 +444,12 @@ int sevc_create(const sevc_options* o, const sevc_mesh* m, sevc_state** out) {
 
 void sevc_destroy(sevc_state* s) {
     if (!s) return;
-    if (s->stream) hipStreamSynchronize(s->stream);
-    if (s->gexec) hipGraphExecDestroy(s->gexec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    for (void* p : s->allocs) hipFree(p);
-    for (auto e : s->pool) hipEventDestroy(e);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (void* p : s->allocs) (void)hipFree(p);
+    for (auto e : s->pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -449,6 +462,7 @@ int sevc_request_diagnostics(sevc_state* s) {
 int sevc_enable_kernel_timing(sevc_state* s, int32_t on) {
     if (!s) return SEVC_E_INVALID_ARG;
     s->timing = on != 0;
+    s->timing_hot_only = on == 2;
     s->ktimes.clear();
     return SEVC_OK;
 }
@@ -488,7 +502,7 @@ int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
         s->diag_next = false;
         launch_step(s, diag);
         if (s->timing && s->pending.size() > 4000) {
-            hipStreamSynchronize(s->stream);
+            (void)hipStreamSynchronize(s->stream);
             collect_timers(s);
         }
     }
@@ -596,8 +610,8 @@ int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const do
     if (gb_scaling > 0) { P.gb_scaling = gb_scaling; s->opts[m].GB_scaling = gb_scaling; }
     if (mobility > 0) { P.mobility = mobility; s->opts[m].mobility = mobility; }
     HIPCHK(hipMemcpy(const_cast<Params*>(d.prm) + m, &P, sizeof(Params), hipMemcpyHostToDevice));
-    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
     return SEVC_OK;
 }
 

@@ -209,30 +209,72 @@ __global__ void k_begin(Dev d) {
     }
 }
 
-// Per-block counts of enabled grains and vertices (main.cu:317).
-__global__ void k_count_grains(Dev d) {
-    TABLE_BLOCK(d.gblk, d.ngb)
-    if (!M.active) return;
-    int c = (i < bd_.e && d.genab[i]) ? 1 : 0;
+// Per-block counts of enabled grains and vertices (main.cu:317), written as
+// one partial per table block and summed per member by k_member_reduce.
+__device__ __forceinline__ int block_count(int c) {
+    __shared__ int wsum[kBlock / 64];
     unsigned long long bal = __ballot(c);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&M.curr_grains, (int)__popcll(bal));
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (int)__popcll(bal);
+    __syncthreads();
+    int r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += wsum[w];
+    return r;
+}
+
+__global__ void k_count_grains(Dev d) {
+    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb_ >= d.ngb) return;
+    const Blk bd_ = d.gblk[lb_];
+    const int i = bd_.b + (int)threadIdx.x;
+    int c = (d.ms[bd_.m].active && i < bd_.e && d.genab[i]) ? 1 : 0;
+    int r = block_count(c);
+    if (threadIdx.x == 0) d.bcnt_g[lb_] = r;
 }
 
 __global__ void k_count_vertices(Dev d) {
-    TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active) return;
-    int c = (i < bd_.e && d.ven[i]) ? 1 : 0;
-    unsigned long long bal = __ballot(c);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&M.n_venabled, (int)__popcll(bal));
+    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb_ >= d.nvb) return;
+    const Blk bd_ = d.vblk[lb_];
+    const int i = bd_.b + (int)threadIdx.x;
+    int c = (d.ms[bd_.m].active && i < bd_.e && d.ven[i]) ? 1 : 0;
+    int r = block_count(c);
+    if (threadIdx.x == 0) d.bcnt_v[lb_] = r;
 }
 
-__global__ void k_end_prepare(Dev d) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per member: sums / minimises the per-block partials.
+// what = 0: tau (adaptive dt); 1: grain and vertex counts.
+__global__ void __launch_bounds__(256) k_member_reduce(Dev d, int what) {
+    __shared__ unsigned long long sm[256];
+    __shared__ long long sa[256], sb[256];
+    const int m = blockIdx.x;
     if (m >= d.B) return;
     MState& M = d.ms[m];
     if (!M.active) return;
-    M.curr_grains = 0;
-    M.n_venabled = 0;
+    const int tid = threadIdx.x;
+    if (what == 0) {
+        unsigned long long v = 0x7FF0000000000000ull;
+        for (int b = d.eblk_first[m] + tid; b < d.eblk_first[m + 1]; b += blockDim.x) v = d.bmin[b] < v ? d.bmin[b] : v;
+        sm[tid] = v;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st && sm[tid + st] < sm[tid]) sm[tid] = sm[tid + st];
+            __syncthreads();
+        }
+        if (tid == 0) M.tau_bits = sm[0];
+    } else {
+        long long a = 0, b2 = 0;
+        for (int b = d.gblk_first[m] + tid; b < d.gblk_first[m + 1]; b += blockDim.x) a += d.bcnt_g[b];
+        for (int b = d.vblk_first[m] + tid; b < d.vblk_first[m + 1]; b += blockDim.x) b2 += d.bcnt_v[b];
+        sa[tid] = a;
+        sb[tid] = b2;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st) { sa[tid] += sa[tid + st]; sb[tid] += sb[tid + st]; }
+            __syncthreads();
+        }
+        if (tid == 0) { M.curr_grains = (int)sa[0]; M.n_venabled = (int)sb[0]; }
+    }
 }
 
 // main.cu:388-393
@@ -460,8 +502,35 @@ __global__ void k_text(Dev d) {
     double aux = tx * px + ty * py;
     double te = (aux == 0) ? 100.0 : -norm / aux;
     d.bt[i] = te;
-    if (P.adaptive && te > 0 && !(d.blen[i] < P.flip_thr))
-        atomicMin(&M.tau_bits, (unsigned long long)__double_as_longlong(te));
+}
+
+// Adaptive dt: smallest positive t_ext over enabled edges not shorter than
+// the flip threshold; one partial per table block (no contended atomics),
+// reduced per member by k_member_reduce.
+__global__ void k_tau_partial(Dev d) {
+    __shared__ unsigned long long wmin[kBlock / 64];
+    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb_ >= d.neb) return;
+    const Blk bd_ = d.eblk[lb_];
+    const Params& P = d.prm[bd_.m];
+    const MState& M = d.ms[bd_.m];
+    const int i = bd_.b + (int)threadIdx.x;
+    unsigned long long v = 0x7FF0000000000000ull;
+    if (M.active && P.adaptive && i < bd_.e && d.benab[i]) {
+        double te = d.bt[i];
+        if (te > 0 && !(d.blen[i] < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = wmin[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = wmin[w] < r ? wmin[w] : r;
+        d.bmin[lb_] = r;
+    }
 }
 
 // label_boundaries, topological.h:343-353 (every slot, quirk A-6), and the
@@ -1391,7 +1460,48 @@ __device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, i
     if (d.bvx[2 * r.AF] == A) d.bvx[2 * r.AF] = C;
     else d.bvx[2 * r.AF + 1] = C;
     d.vb[3 * C + 2] = r.AF;
+    const int de[6] = {AB, BC, CA, r.AD, r.AE, r.AF};
+    int k0 = atomicAdd(&d.cnt[C_NDE], 6);
+    for (int q = 0; q < 6; q++) if (k0 + q < 6 * d.G) d.l_nde[k0 + q] = de[q];
+    const int dg[4] = {g, r.g1, r.g2, r.g3};
+    int k1 = atomicAdd(&d.cnt[C_NDG], 4);
+    for (int q = 0; q < 4; q++) if (k1 + q < 4 * d.G) d.l_ndg[k1 + q] = dg[q];
     return 0;
+}
+
+// Cooperative search, in one workgroup, of the k-th (0-based) slot of a
+// member whose enable flag equals `want`; returns -1 if there is none.
+__device__ int find_kth(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want, int k,
+                        int* sh) {
+    __shared__ int found;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int per = (nblk + nt - 1) / nt;
+    const int b0 = first + tid * per, b1 = min(first + nblk, b0 + per);
+    int mine = 0;
+    for (int b = b0; b < b1; b++) mine += want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
+    sh[tid] = mine;
+    if (tid == 0) found = -1;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int t = 0; t < nt; t++) { int x = sh[t]; sh[t] = run; run += x; }
+    }
+    __syncthreads();
+    int pos = sh[tid];
+    if (k >= pos && k < pos + mine) {
+        int r = k - pos;
+        for (int b = b0; b < b1; b++) {
+            int c = want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
+            if (r >= c) { r -= c; continue; }
+            for (int i = tbl[b].b; i < tbl[b].e; i++)
+                if ((en[i] != 0) == (want != 0)) { if (r == 0) { found = i; break; } r--; }
+            break;
+        }
+    }
+    __syncthreads();
+    int f = found;
+    __syncthreads();
+    return f;
 }
 
 // One workgroup per member: picks the nucleation events of this step,
@@ -1405,16 +1515,17 @@ __device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, i
 //  parallel.
 // Scratch: 12 ints per grain slot of the member, at 12*goff in d.nscr:
 //  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
-//  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free vertices.
+//  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free
+//  vertices, [11ng,12ng) candidate sites.
 __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
     __shared__ int sh[257];
-    __shared__ int nkeep, nmade, fg_n, fe_n, fv_n;
+    __shared__ int nkeep, nmade, fg_n, fe_n, fv_n, ncand;
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
     MState& M = d.ms[m];
     if (!M.active || !M.nucl_now) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int ng = P.ng;
     int* base = d.nscr + 12 * (size_t)P.goff;
     int* cand = base;
@@ -1423,66 +1534,83 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
     int* fg = base + 5 * (size_t)ng;
     int* fe = base + 6 * (size_t)ng;
     int* fv = base + 9 * (size_t)ng;
-    if (tid == 0) {
-        nkeep = 0;
-        nmade = 0;
-        if (P.nucl_mode == SEVC_NUCL_MC) {
-            const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
-            int n_alive = 0;
-            for (int b = first; b < first + nb; b++) n_alive += d.bc_v[b];
-            if (n_alive > 0) {
-                double u0, u1;
-                sevc_uniform2(P.seed, (uint32_t)M.n, 0u, kStreamMcPick, 0u, &u0, &u1);
-                int k = (int)(u0 * (double)n_alive);
-                if (k >= n_alive) k = n_alive - 1;
-                int A = -1;
-                for (int b = first; b < first + nb && A < 0; b++) {
-                    if (k >= d.bc_v[b]) { k -= d.bc_v[b]; continue; }
-                    for (int i = d.vblk[b].b; i < d.vblk[b].e; i++)
-                        if (d.ven[i]) { if (k == 0) { A = i; break; } k--; }
-                }
-                if (A >= 0) {
-                    d.vnuc[A] = 0;
-                    if (d.dE[A] < 0) d.vnuc[A] = 1;
-                    else if (d.nfac[A] < 0) {
-                        double thr = sevc_exp(-P.mc_k * d.dE[A]);
-                        if (u1 < thr) d.vnuc[A] = 1;
-                    }
-                    if (d.vnuc[A]) { kept[0] = A; nkeep = 1; }
-                }
+    int* site = base + 11 * (size_t)ng;
+    if (tid == 0) { nkeep = 0; nmade = 0; ncand = 0; }
+    __syncthreads();
+    if (P.nucl_mode == SEVC_NUCL_MC) {
+        const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
+        int part = 0;
+        for (int b = first + tid; b < first + nb; b += nt) part += d.bc_v[b];
+        sh[tid] = part;
+        __syncthreads();
+        if (tid == 0) { int t = 0; for (int q = 0; q < nt; q++) t += sh[q]; sh[nt] = t; }
+        __syncthreads();
+        const int n_alive = sh[nt];
+        __syncthreads();
+        if (n_alive == 0) return;
+        double u0, u1;
+        sevc_uniform2(P.seed, (uint32_t)M.n, 0u, kStreamMcPick, 0u, &u0, &u1);
+        int k = (int)(u0 * (double)n_alive);
+        if (k >= n_alive) k = n_alive - 1;
+        const int A = find_kth(d.vblk, first, nb, d.bc_v, d.ven, 1, k, sh);
+        if (tid == 0 && A >= 0) {
+            d.vnuc[A] = 0;
+            if (d.dE[A] < 0) d.vnuc[A] = 1;
+            else if (d.nfac[A] < 0) {
+                double thr = sevc_exp(-P.mc_k * d.dE[A]);
+                if (u1 < thr) d.vnuc[A] = 1;
             }
-        } else {
-            const int total = d.cnt[C_NUCL];
-            int nc = 0;
-            for (int q = 0; q < total; q++) {
-                int g = d.l_nucl[q];
-                if (g >= P.goff && g < P.goff + ng) cand[nc++] = g;
-            }
-            sort_ints(cand, nc);
-            int nk = 0;
-            for (int e = 0; e < nc; e++) {
-                const int g = cand[e];
-                double u0, u1, u2, u3;
-                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
-                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
-                const int len = d.glen[g];
-                int k = (int)(u1 * (double)len);
-                if (k >= len) k = len - 1;
-                const int A = glist(d, g)[k];
-                for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
-                bool clash = false;
-                for (int f = 0; f < e && !clash; f++)
-                    for (int a = 0; a < 3 && !clash; a++)
-                        for (int b = 0; b < 3; b++)
-                            if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
-                if (clash) continue;
-                double minAL = d.blen[d.vb[3 * A]];
-                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
-                d.nucl_r[A] = P.nucl_scaling * minAL * u2;
-                d.nucl_ori[A] = 2 * SEVC_PI * u3;
-                kept[nk++] = A;
-            }
-            nkeep = nk;
+            if (d.vnuc[A]) { kept[0] = A; nkeep = 1; }
+        }
+    } else {
+        // this member's candidates, then grain order
+        const int total = d.cnt[C_NUCL];
+        for (int q = tid; q < total; q += nt) {
+            int g = d.l_nucl[q];
+            if (g >= P.goff && g < P.goff + ng) cand[atomicAdd(&ncand, 1)] = g;
+        }
+        __syncthreads();
+        const int nc = ncand;
+        if (tid == 0) sort_ints(cand, nc);
+        __syncthreads();
+        for (int e = tid; e < nc; e += nt) {
+            const int g = cand[e];
+            double u0, u1;
+            sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
+            const int len = d.glen[g];
+            int k = (int)(u1 * (double)len);
+            if (k >= len) k = len - 1;
+            const int A = glist(d, g)[k];
+            site[e] = A;
+            for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
+        }
+        __syncthreads();
+        // kept iff no earlier candidate touches one of its grains
+        for (int e = tid; e < nc; e += nt) {
+            bool clash = false;
+            for (int f = 0; f < e && !clash; f++)
+                for (int a = 0; a < 3 && !clash; a++)
+                    for (int b = 0; b < 3; b++)
+                        if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
+            kept[e] = clash ? 0 : 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int w = 0;
+            for (int e = 0; e < nc; e++) if (kept[e]) cand[w++] = cand[e] | 0, site[w - 1] = site[e];
+            nkeep = w;
+        }
+        __syncthreads();
+        for (int r = tid; r < nkeep; r += nt) {
+            const int g = cand[r], A = site[r];
+            double u0, u1, u2, u3;
+            sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
+            (void)u0; (void)u1;
+            double minAL = d.blen[d.vb[3 * A]];
+            for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+            d.nucl_r[A] = P.nucl_scaling * minAL * u2;
+            d.nucl_ori[A] = 2 * SEVC_PI * u3;
+            kept[r] = A;
         }
     }
     __syncthreads();
@@ -1497,13 +1625,35 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
                           2 * K, fv, sh);
     if (tid == 0) { fg_n = a; fe_n = b; fv_n = c; }
     __syncthreads();
-    for (int r = tid; r < K; r += blockDim.x) {
+    for (int r = tid; r < K; r += nt) {
         if (r + 1 > fg_n || 3 * r + 3 > fe_n || 2 * r + 2 > fv_n) continue;
         if (nucleate_at(d, P, kept[r], fv[2 * r], fv[2 * r + 1], fe[3 * r], fe[3 * r + 1], fe[3 * r + 2], fg[r]) == 0)
             atomicAdd(&nmade, 1);
     }
     __syncthreads();
     if (tid == 0) M.nucl += nmade;
+}
+
+// Lengths, energies (calculus.h:83-114) and areas (:224-234) of the
+// entities a nucleation changed -- the recomputation of main.cu:171-177
+// restricted to the only values it can change.
+__global__ void k_nucl_refresh(Dev d) {
+    const int ne = min(d.cnt[C_NDE], 6 * d.G), ng = min(d.cnt[C_NDG], 4 * d.G);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ne + ng; k += gridDim.x * blockDim.x) {
+        if (k < ne) {
+            int b = d.l_nde[k];
+            if (!d.benab[b]) continue;
+            const Params& P = d.prm[member_of_edge(d, b)];
+            int a0 = d.bvx[2 * b], a1 = d.bvx[2 * b + 1];
+            d.blen[b] = arclen(d.pos[2 * a0], d.pos[2 * a0 + 1], d.pos[2 * a1], d.pos[2 * a1 + 1], P.D);
+            double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
+            d.ben[b] = e * P.gb_scaling;
+        } else {
+            int g = d.l_ndg[k - ne];
+            if (!d.genab[g]) continue;
+            d.garea[g] = grain_area(d, g, d.prm[member_of_grain(d, g)].D);
+        }
+    }
 }
 
 }  // namespace sevc

@@ -36,9 +36,10 @@ def oracle_lib():
                                      P(C.c_double), P(C.c_double), P(Mesh)]
         L.orc_mesh_free.argtypes = [P(Mesh)]
         L.orc_check_mesh.argtypes = [P(Mesh), P(C.c_double)]
-        for f in ("orc_wrap_dist", "orc_dom_mod"):
-            getattr(L, f).restype = C.c_double
-            getattr(L, f).argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_wrap_dist.restype = C.c_double
+        L.orc_wrap_dist.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_dom_mod.restype = C.c_double
+        L.orc_dom_mod.argtypes = [C.c_double, C.c_double]
         L.orc_mag.restype = C.c_double
         L.orc_mag.argtypes = [C.c_double, C.c_double]
         L.orc_polygon_area.restype = C.c_double

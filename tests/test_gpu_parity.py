@@ -1,19 +1,45 @@
 """GPU parity: the HIP time step against the serial FP64 oracle.
 
-Integer topology and every FP64 field must match bit for bit (the oracle
-and the kernels evaluate the same operations in the same order with no
-fused multiply-add)."""
+Every test runs the same seeded input through the C ABI on the GPU and
+through the oracle, then compares the whole state.  Integer topology and
+every FP64 field must match bit for bit: both sides evaluate the same
+operations in the same order with no fused multiply-add, and the device's
+division, square root and elementary functions are checked against the
+host's bits first."""
 import numpy as np
 import pytest
 
 import sevc
-from _util import Oracle, diff_fields, ref2k_options
+from _util import Oracle, diff_fields, oracle_lib, ref2k_options
 
 pytestmark = pytest.mark.gpu
 
+STAT_KEYS = ("n", "n_grains", "n_vertices", "n_flips", "n_removals", "n_two_grain", "n_nucleations",
+             "n_vertex_steps", "steps_done", "done")
 
-def _pair(opts, mesh):
-    return sevc.State(opts, mesh), Oracle(opts, mesh)
+
+def _run_pair(opts, mesh, chunks, graph=False):
+    """Step GPU and oracle in the given chunk sizes; compare after each."""
+    st, orc = sevc.State(opts, mesh), Oracle(opts, mesh)
+    tot = {"flips": 0, "removals": 0, "nucl": 0, "two": 0}
+    for n in chunks:
+        print(f"  chunk of {n} steps", flush=True)
+        s1 = st.step(n, graph=graph)
+        rc, s2 = orc.step(n)
+        assert rc == 0 and s1.status == 0, (rc, s1.status)
+        a, b = st.get_mesh(), orc.get_mesh()
+        bad = diff_fields(a, b)
+        assert bad == {}, f"state differs after chunk: {bad}"
+        for k in STAT_KEYS:
+            assert getattr(s1, k) == getattr(s2, k), (k, getattr(s1, k), getattr(s2, k))
+        assert s1.t == s2.t and s1.dt_last == s2.dt_last
+        assert s1.nucleated_area_pct == s2.nucleated_area_pct
+        tot["flips"] += s1.n_flips
+        tot["removals"] += s1.n_removals
+        tot["nucl"] += s1.n_nucleations
+        tot["two"] += s1.n_two_grain
+    st.close()
+    return tot
 
 
 def test_selftest_math_bits():
@@ -21,7 +47,6 @@ def test_selftest_math_bits():
     n = 20000
     x = rng.uniform(-4, 4, n)
     y = rng.uniform(-4, 4, n)
-    from _util import oracle_lib
     L = oracle_lib()
     cases = {0: lambda a, b: L.orc_atan2(a, b), 1: lambda a, b: L.orc_sin(a), 2: lambda a, b: L.orc_cos(a),
              3: lambda a, b: L.orc_exp(a), 4: lambda a, b: a / b, 6: lambda a, b: L.orc_mag(a, b),
@@ -30,30 +55,113 @@ def test_selftest_math_bits():
         dev = sevc.selftest_math(op, x, y)
         host = np.array([f(a, b) for a, b in zip(x, y)])
         assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), f"op {op} differs"
-    xs = np.abs(x) * 10.0 ** rng.integers(-20, 20, n)
-    dev = sevc.selftest_math(5, xs, y)
-    assert np.array_equal(dev.view(np.uint64), np.sqrt(xs).view(np.uint64))
+    # correctly rounded sqrt over many binades, and division by tiny/huge numbers
+    xs = np.abs(x) * 10.0 ** rng.integers(-300, 300, n).astype(np.float64)
+    assert np.array_equal(sevc.selftest_math(5, xs, y).view(np.uint64), np.sqrt(xs).view(np.uint64))
+    ys = y * 10.0 ** rng.integers(-150, 150, n).astype(np.float64)
+    assert np.array_equal(sevc.selftest_math(4, xs, ys).view(np.uint64), (xs / ys).view(np.uint64))
+    # Philox word 0 on the device equals the host restatement
+    ctr = rng.integers(0, 2 ** 32, n).astype(np.float64)
+    dev = sevc.selftest_math(8, ctr, ctr[::-1].copy())
+    import ctypes as C
+    host = []
+    for a, b in zip(ctr, ctr[::-1]):
+        c = (C.c_uint32 * 4)(int(a), int(b), 0, 0)
+        k = (C.c_uint32 * 2)(1, 2)
+        o = (C.c_uint32 * 4)()
+        L.orc_philox(c, k, o)
+        host.append(float(o[0]))
+    assert np.array_equal(dev, np.array(host))
 
 
 def test_step0_velocities_bitwise():
     o = ref2k_options(events=0)
     mesh = sevc.load_files(o)
-    st, orc = _pair(o, mesh)
+    _run_pair(o, mesh, [1])
+
+
+def test_run_until_half_removed():
+    """Config-1 style run of the 2,000-grain fixture to 50 % removal."""
+    o = ref2k_options(dt=2e-4, leave_percent=0.5)
+    mesh = sevc.load_files(o)
+    tot = _run_pair(o, mesh, [50, 150, 400])
+    assert tot["removals"] >= 900 and tot["flips"] >= 1000, tot
+
+
+def test_mc_nucleation_recrystallisation():
+    """Reference Metropolis nucleation (nucleation2.cu) at the example.ini
+    scale: domain 40, SE in [3, 6], gate at 5/6 of the grains."""
+    o = ref2k_options(domain_bound=40.0, dt=1e-3, dtau=1e-3, do_nucleation=1, leave_percent=0.05,
+                      SE_eps=1.0, MC_k=1.0)
+    o.SE_min, o.SE_max = 3.0, 6.0
+    mesh = sevc.load_files(o)
+    tot = _run_pair(o, mesh, [400, 400, 400])
+    assert tot["nucl"] >= 20, tot
+
+
+def _generated(n, seed, **kw):
+    pos, rid = sevc.voronoi(n, seed)
+    rng = np.random.default_rng(seed)
+    se = np.exp(np.log(0.3) - 0.125 + 0.5 * rng.standard_normal(n))
+    o = sevc.default_options()
+    o.GB_eps = 0.0
+    o.n_grains = n
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o, sevc.build_mesh(o, pos, rid, rng.random(n), se, n)
+
+
+def test_rate_nucleation_adaptive_threshold():
+    """The benchmark's model options (rate nucleation, adaptive dt, flip
+    threshold) on a 3,000-grain tessellation."""
+    o, mesh = _generated(3000, 5, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3,
+                         do_nucleation=1, nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02,
+                         nucleation_scaling=0.9, seed=11, extra_capacity=0.1)
+    tot = _run_pair(o, mesh, [20, 80, 100])
+    assert tot["nucl"] >= 10 and tot["flips"] >= 50, tot
+
+
+def test_graph_replay():
+    o = ref2k_options(dt=2e-4)
+    mesh = sevc.load_files(o)
+    tot = _run_pair(o, mesh, [30, 30, 60], graph=True)
+    assert tot["flips"] > 0
+
+
+def test_diagnostics_fields():
+    o = ref2k_options(dt=1e-4)
+    mesh = sevc.load_files(o)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    st.step(7)
+    orc.step(7)
+    st.request_diagnostics()
+    orc.request_diagnostics()
     st.step(1)
     orc.step(1)
     a, b = st.get_mesh(), orc.get_mesh()
-    assert diff_fields(a, b) == {}
+    assert diff_fields(a, b, ["v_energy", "v_vel_parts", "g_dAdt", "pos", "g_area"]) == {}
 
 
-@pytest.mark.parametrize("steps", [50, 400])
-def test_steps_with_events_bitwise(steps):
-    o = ref2k_options()
+def test_analytic_velocity_model():
+    o = ref2k_options(dt=1e-5, velocity_model=sevc.VEL_ANALYTIC, events=0)
     mesh = sevc.load_files(o)
-    st, orc = _pair(o, mesh)
-    s1 = st.step(steps)
-    rc, s2 = orc.step(steps)
-    assert rc == 0 and s1.status == 0
-    a, b = st.get_mesh(), orc.get_mesh()
-    assert diff_fields(a, b) == {}
-    for k in ("n", "n_grains", "n_vertices", "n_flips", "n_removals", "n_two_grain", "n_vertex_steps"):
-        assert getattr(s1, k) == getattr(s2, k), k
+    _run_pair(o, mesh, [10])
+
+
+def test_batch_members_match_separate_oracles():
+    o1 = ref2k_options(dt=2e-4)
+    m1 = sevc.load_files(o1)
+    o2, m2 = _generated(1500, 3, dt=5e-5, do_nucleation=1, nucleation_mode=sevc.NUCL_RATE,
+                        nucleation_rate=0.02, seed=5, extra_capacity=0.1)
+    o3, m3 = _generated(800, 4, dt=1e-4, adaptive_dt=1, flip_threshold=3e-3)
+    st = sevc.State([o1, o2, o3], [m1, m2, m3])
+    orcs = [Oracle(o1, m1), Oracle(o2, m2), Oracle(o3, m3)]
+    for chunk in (40, 80):
+        st.step(chunk)
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            s1 = st.member_stats(k)
+            assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
+            for key in STAT_KEYS:
+                assert getattr(s1, key) == getattr(s2, key), (k, key)

@@ -1,0 +1,170 @@
+"""Host side of the C ABI: options / ini semantics (options.h:69-206), the
+loader (io.h:66-264) against the oracle's independent restatement, error
+paths, exported symbols, the Voronoi generator and the snapshot format."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sevc
+from sevc import _abi
+from _util import GOLDEN, REF2K, REPO, diff_fields, oracle_load, ref2k_options
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(REPO, "include", "sevc.h")).read()
+    names = set(re.findall(r"\b(sevc_[a-z_0-9]+)\s*\(", hdr))
+    L = sevc.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_default_options_match_python_mirror():
+    a, b = sevc.default_options(), _abi.default_options()
+    for name, _ in _abi.Options._fields_:
+        assert getattr(a, name) == getattr(b, name), name
+
+
+def test_example_ini_semantics(tmp_path):
+    o = sevc.options_from_ini(os.path.join(GOLDEN, "example.ini"))
+    assert (o.dt, o.dtau, o.tmax, o.nmax, o.snap_gap) == (1e-4, 1e-4, 10.0, -1, 100)
+    assert o.leave_percent == 0.05 and o.n_grains == 2000
+    assert o.vertex_fname == b"voronoi/2K/vertices.txt" and o.SE_fname == b"voronoi/2K/SE/uniform.txt"
+    assert (o.GB_eps, o.GB_scaling, o.SE_eps, o.SE_min, o.SE_max) == (0.0, 1.0, 1.0, 3.0, 6.0)
+    assert o.domain_bound == 40.0 and o.do_nucleation == 0 and o.nucleation_gap == 1
+    assert (o.n_blocks, o.n_threads, o.debug_lvl) == (32, 256, 0)
+    assert o.output_folder == b"/home/user/example"
+    p = tmp_path / "q.ini"
+    p.write_text("dt=0.5\ndtau=2.0\ntmax=3\nnmax=7\ndo_nucleation=True\n dt=9\n# dt=8\n[x]\nMC_k=2.5\n")
+    q = sevc.options_from_ini(str(p))
+    assert q.dt == 0.5 and q.nucleation_gap == 4 and q.MC_k == 2.5
+    assert q.do_nucleation == 0                       # only the exact value "true"
+    assert q.tmax == 1.0 and q.nmax == -1              # both set -> options.h:177-180
+    assert sevc.lib().sevc_options_from_ini(b"/nonexistent.ini", C.byref(_abi.Options())) == _abi.E_IO
+
+
+@pytest.mark.parametrize("se,semin,semax,seeps", [("uniform", 999, 999, 0.5), ("triangular", 3.0, 3.0, 2.0),
+                                                  ("uniform", 3.0, 6.0, 1.0)])
+def test_loader_matches_oracle_loader(se, semin, semax, seeps):
+    o = ref2k_options(se=se, SE_min=semin, SE_max=semax, SE_eps=seeps)
+    a = sevc.load_files(o)
+    rc, b = oracle_load(o)
+    assert rc == 0
+    assert diff_fields(a, b, list(a.a.keys())) == {}
+    assert sevc.check_mesh(a)[0] == 0
+    assert (a.nv, a.ne, a.ng) == (4000, 6000, 2000)
+    hist = np.bincount(a["g_vlen"])
+    assert hist[3:13].sum() == 2000 and abs(np.average(np.arange(len(hist)), weights=hist) - 6.0) < 1e-12
+
+
+def test_loader_error_paths(tmp_path):
+    o = ref2k_options(n_grains=1999)
+    with pytest.raises(sevc.SevcError) as e:
+        sevc.load_files(o)
+    assert e.value.code == _abi.E_TOPOLOGY
+    o = ref2k_options()
+    o.vertex_fname = str(tmp_path / "missing.txt").encode()
+    with pytest.raises(sevc.SevcError) as e:
+        sevc.load_files(o)
+    assert e.value.code == _abi.E_IO
+    # a vertex with four ridges is not trivalent
+    rid = open(os.path.join(REF2K, "ridges.txt")).read().splitlines()
+    bad = tmp_path / "ridges.txt"
+    bad.write_text("\n".join(rid + [rid[0]]) + "\n")
+    o = ref2k_options()
+    o.boundary_fname = str(bad).encode()
+    with pytest.raises(sevc.SevcError) as e:
+        sevc.load_files(o)
+    assert e.value.code == _abi.E_TOPOLOGY
+
+
+def test_checker_flags_corruption_and_create_rejects_it():
+    o = ref2k_options()
+    m = sevc.load_files(o)
+    assert sevc.check_mesh(m)[0] == 0
+    m2 = m.copy()
+    m2["v_grn"][0], m2["v_grn"][1] = m2["v_grn"][1], m2["v_grn"][0]
+    m2["v_grn"][3 * 5] = m2["v_grn"][3 * 5 + 1]          # repeated grain at vertex 5
+    assert sevc.check_mesh(m2)[0] > 0
+
+
+def test_voronoi_generator_invariants():
+    for n, seed in [(40, 1), (500, 2), (5000, 3)]:
+        pos, rid = sevc.voronoi(n, seed)
+        assert pos.shape == (2 * n, 2) and rid.shape == (3 * n, 2)
+        assert np.all((pos >= 0) & (pos < 1))
+        o = sevc.default_options()
+        o.GB_eps = 0.0
+        m = sevc.build_mesh(o, pos, rid, np.zeros(n), np.zeros(n), n)
+        assert sevc.check_mesh(m)[0] == 0
+    with pytest.raises(sevc.SevcError):
+        sevc.voronoi(3, 1)   # a cell would border its own periodic image
+
+
+def test_voronoi_matches_scipy_periodic_tessellation():
+    spatial = pytest.importorskip("scipy.spatial")
+    n = 60
+    pos, rid = sevc.voronoi(n, 9)
+    # seeds of the generator, recovered as the cell centroids' nearest seeds
+    # is not possible; instead rebuild scipy's diagram from the same vertices:
+    # every vertex must be equidistant to exactly three seeds of a 3x3 tiling,
+    # and scipy's diagram of those seeds has the same vertex set.
+    import ctypes
+    L = sevc.lib()
+    # regenerate the seeds exactly as sevc_voronoi_generate does (Philox)
+    from _util import oracle_lib
+    O = oracle_lib()
+    seeds = []
+    for k in range(n):
+        c = (ctypes.c_uint32 * 4)(k, 0, 4, 0)
+        key = (ctypes.c_uint32 * 2)(9, 0)
+        o = (ctypes.c_uint32 * 4)()
+        O.orc_philox(c, key, o)
+        x = (((o[0] << 32) | o[1]) >> 11) * 2.0 ** -53
+        y = (((o[2] << 32) | o[3]) >> 11) * 2.0 ** -53
+        seeds.append((x, y))
+    s = np.array(seeds)
+    tiles = np.concatenate([s + [dx, dy] for dx in (-1, 0, 1) for dy in (-1, 0, 1)])
+    vor = spatial.Voronoi(tiles)
+    vv = vor.vertices
+    inside = vv[(vv[:, 0] >= 0) & (vv[:, 0] < 1) & (vv[:, 1] >= 0) & (vv[:, 1] < 1)]
+    assert len(inside) == len(pos)
+    tree = spatial.cKDTree(inside)
+    dist, idx = tree.query(pos)
+    assert dist.max() < 1e-9 and len(set(idx)) == len(pos)
+    # edges: scipy ridges between inside vertices map onto ours
+    ours = set(tuple(sorted((int(idx[a]), int(idx[b])))) for a, b in rid)
+    theirs = set()
+    vidx = {}
+    for i, v in enumerate(vv):
+        vidx[i] = tree.query(np.mod(v, 1.0))[1]
+    for a, b in vor.ridge_vertices:
+        if a < 0 or b < 0:
+            continue
+        va, vb = vv[a], vv[b]
+        if (0 <= va[0] < 1 and 0 <= va[1] < 1) or (0 <= vb[0] < 1 and 0 <= vb[1] < 1):
+            theirs.add(tuple(sorted((int(vidx[a]), int(vidx[b])))))
+    assert ours == theirs
+
+
+def test_snapshot_format(tmp_path):
+    o = ref2k_options()
+    m = sevc.load_files(o)
+    L = sevc.lib()
+    rc = L.sevc_export_snapshot(C.byref(m.struct), C.byref(o), str(tmp_path).encode(), 3, C.c_double(0.25))
+    assert rc == 0
+    lines = open(tmp_path / "000003.txt").read().splitlines()
+    assert lines[0] == "TIME 0.2500000000000000"
+    assert lines[12] == "MAX VERTICES PER GRN 100"
+    assert lines[13:16] == ["VERTICES 4000", "BOUNDARIES 6000", "GRAINS 2000"]
+    assert lines[19].startswith("#VERTEX ENABLED ENERGY X Y")
+    v0 = lines[20].split()
+    assert len(v0) == 17 and v0[0] == "0" and v0[1] == "1"
+    assert float(v0[3]) == float("%.16f" % m["pos"][0])
+    hb = lines.index("#BOUNDARY ENABLED INI END ENERGY ARCLEN EXTTIME CANDIDATE")
+    assert hb == 20 + 4000
+    hg = lines.index("#GRAIN ID ENABLED NUCLEATED ORIENT SE AREA dAdt VLEN VERTICES")
+    g0 = lines[hg + 1].split()
+    assert len(g0) == 8 + 100 and int(g0[7]) == m["g_vlen"][0]
