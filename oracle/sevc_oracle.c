@@ -1171,6 +1171,9 @@ static int nucleation_rate(orc* s) {
         int A = GV(s, g, k);
         double minAL = s->blen[VB(s, A, 0)];
         for (int i = 0; i < 3; i++) if (minAL > s->blen[VB(s, A, i)]) minAL = s->blen[VB(s, A, i)];
+        /* a vertex on a zero-length boundary (both ends of a fresh flip sit
+         * at one point) cannot host the nucleus: it is not a candidate */
+        if (!(minAL > 0)) continue;
         ev_v[ne] = A;
         ev_r[ne] = s->o.nucleation_scaling * minAL * u2;
         ev_o[ne] = 2 * SEVC_PI * u3;

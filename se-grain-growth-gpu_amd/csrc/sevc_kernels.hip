@@ -1581,23 +1581,33 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             int k = (int)(u1 * (double)len);
             if (k >= len) k = len - 1;
             const int A = glist(d, g)[k];
-            site[e] = A;
+            double minAL = d.blen[d.vb[3 * A]];
+            for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+            // a site on a zero-length boundary is not a candidate
+            site[e] = (minAL > 0) ? A : -1;
             for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
         }
         __syncthreads();
         // kept iff no earlier candidate touches one of its grains
         for (int e = tid; e < nc; e += nt) {
-            bool clash = false;
-            for (int f = 0; f < e && !clash; f++)
+            bool clash = site[e] < 0;
+            for (int f = 0; f < e && !clash; f++) {
+                if (site[f] < 0) continue;
                 for (int a = 0; a < 3 && !clash; a++)
                     for (int b = 0; b < 3; b++)
                         if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
+            }
             kept[e] = clash ? 0 : 1;
         }
         __syncthreads();
         if (tid == 0) {
             int w = 0;
-            for (int e = 0; e < nc; e++) if (kept[e]) cand[w++] = cand[e] | 0, site[w - 1] = site[e];
+            for (int e = 0; e < nc; e++) {
+                if (!kept[e]) continue;
+                cand[w] = cand[e];
+                site[w] = site[e];
+                w++;
+            }
             nkeep = w;
         }
         __syncthreads();
