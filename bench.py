@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the step as a HIP graph (no per-kernel timing)")
+    ap.add_argument("--graph-steps", type=int, default=200, help="extra steps timed as HIP graph replays (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,6 +150,16 @@ def main():
                                  "achieved_GBs": (B_VELOCITY + B_TRANSLATE) * verts_per_step / (vel_avg + tr_avg) / 1e9},
         "events": {"flips": s.n_flips, "removals": s.n_removals, "nucleations": s.n_nucleations},
     }
+    if args.graph_steps > 0:
+        # same workload continued with the step replayed as a HIP graph
+        st.enable_timing(0)
+        st.step(5, graph=True)
+        g0 = time.perf_counter()
+        sg = st.step(args.graph_steps, graph=True)
+        g1 = time.perf_counter()
+        line["graph_replay"] = {"value": sg.n_vertex_steps / (g1 - g0),
+                                "ms_per_step": (g1 - g0) / max(1, sg.steps_done) * 1e3,
+                                "steps": sg.steps_done}
     st.close()
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)

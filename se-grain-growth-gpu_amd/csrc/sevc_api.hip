@@ -25,6 +25,9 @@ __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
 __global__ void k_velocity(Dev d);
+__global__ void k_velocity_generic(Dev d);
+__global__ void k_velocity_fallback(Dev d);
+__global__ void k_velocity_analytic(Dev d);
 __global__ void k_grain_dadt(Dev d);
 __global__ void k_translate(Dev d);
 __global__ void k_text(Dev d);
@@ -68,7 +71,7 @@ struct sevc_state {
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
     std::string err;
-    bool any_nucl = false, any_events = false, any_adaptive = false;
+    bool any_nucl = false, any_events = false, any_adaptive = false, any_analytic = false;
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
     // graph replay
@@ -169,7 +172,13 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "nucl_refresh", k_nucl_refresh, kListGrid, 256, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     }
-    launch(s, "velocity", k_velocity, d.nvb, kBlock, d);
+    if (diag) {
+        launch(s, "velocity_generic", k_velocity_generic, d.nvb, kBlock, d);
+    } else {
+        launch(s, "velocity", k_velocity, d.nvb, kBlock, d);
+        launch(s, "velocity_fallback", k_velocity_fallback, 64, 256, d);
+    }
+    if (s->any_analytic) launch(s, "velocity_analytic", k_velocity_analytic, d.nvb, kBlock, d);
     if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
     launch(s, "text", k_text, d.neb, kBlock, d);
     if (s->any_adaptive) {
@@ -283,6 +292,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         if (opts[m].events && opts[m].do_nucleation) s->any_nucl = true;
         if (opts[m].events) s->any_events = true;
         if (opts[m].adaptive_dt) s->any_adaptive = true;
+        if (opts[m].velocity_model == SEVC_VEL_ANALYTIC) s->any_analytic = true;
     }
     if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
     const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
@@ -308,7 +318,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     for (int k = 0; k < kFixStages; ++k) DA(d.l_fix[k], G);
     DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
     DA(d.nscr, 12 * G + 16);
-    DA(d.l_nde, 6 * G + 16); DA(d.l_ndg, 4 * G + 16);
+    DA(d.l_nde, 6 * G + 16); DA(d.l_ndg, 4 * G + 16); DA(d.l_vfall, V + 16);
 
     // block tables
     std::vector<Blk> vt, et, gt;

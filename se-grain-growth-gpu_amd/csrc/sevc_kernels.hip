@@ -413,32 +413,149 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
     *fy = -(ey + gy);
 }
 
-// compute_vertex_velocities, calculus.h:143-154 (x mobility)
-__global__ void k_velocity(Dev d) {
+// Both perturbed areas of grain g around vertex v in one pass over the
+// polygon (the reference's compute_area on the x-shifted and on the
+// y-shifted copy, geometry.h:757-765 after :908-931).
+//
+// With the shifted point (px, y) the Y track of the unwrap is the
+// unperturbed one and the X track differs from it only in the deltas that
+// enter and leave position k (the slot of v); likewise for (x, qy).  Those
+// four deltas are computed before the loop from v's two neighbours in the
+// list, and every other delta is shared.  Each track is still the
+// reference's sequential sum U_i = U_{i-1} + delta_i, and each area the
+// reference's left-to-right shoelace sum, so the results are bit-identical.
+// Requires dom_mod(y) == y and dom_mod(x) == x (checked by the caller).
+__device__ __forceinline__ bool shifted_areas(const Dev& d, int g, int v, double x, double y, double px, double qy,
+                                              double D, double* ax, double* ay) {
+    const int* L = glist(d, g);
+    const int n = d.glen[g];
+    int k = -1;
+    for (int i = 0; i < n; i++) if (L[i] == v) { k = i; break; }
+    if (k < 0) return false;
+    double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
+    if (k > 0) {
+        const int w = L[k - 1];
+        exk = wrapd(px, d.pos[2 * w], D);
+        eyk = wrapd(qy, d.pos[2 * w + 1], D);
+    }
+    if (k < n - 1) {
+        const int w = L[k + 1];
+        exk1 = wrapd(d.pos[2 * w], px, D);
+        eyk1 = wrapd(d.pos[2 * w + 1], qy, D);
+    }
+    int w0 = L[0];
+    double prx = d.pos[2 * w0], pry = d.pos[2 * w0 + 1];
+    double ux = 0, uy = 0, xx = 0, yy = 0, sx = 0, sy = 0;
+    for (int i = 1; i < n; i++) {
+        const int w = L[i];
+        const double cx = d.pos[2 * w], cy = d.pos[2 * w + 1];
+        const double dux = wrapd(cx, prx, D), duy = wrapd(cy, pry, D);
+        prx = cx;
+        pry = cy;
+        const double dxx = (i == k) ? exk : ((i == k + 1) ? exk1 : dux);
+        const double dyy = (i == k) ? eyk : ((i == k + 1) ? eyk1 : duy);
+        const double nux = ux + dux, nuy = uy + duy;
+        const double nxx = xx + dxx, nyy = yy + dyy;
+        sx += xx * nuy - uy * nxx;
+        sy += ux * nyy - yy * nux;
+        ux = nux; uy = nuy; xx = nxx; yy = nyy;
+    }
+    const double z = 0;
+    sx += xx * z - uy * z;
+    sy += ux * z - yy * z;
+    *ax = sx * 0.5;
+    *ay = sy * 0.5;
+    return true;
+}
+
+// compute_vertex_velocities, calculus.h:143-154 (x mobility).  The two
+// energy_diff calls (calculus.h:19-73) are evaluated together: six
+// perturbed lengths and the two perturbed areas of each of the three
+// grains, accumulated in the reference's order.  Vertices the single pass
+// cannot take (a coordinate equal to D, so that dom_mod moves the unshifted
+// coordinate) are listed for k_velocity_generic.
+__global__ void __launch_bounds__(256) k_velocity(Dev d) {
     TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active || i >= bd_.e || !d.ven[i]) return;
+    if (!M.active || P.vel_model != SEVC_VEL_FD || i >= bd_.e || !d.ven[i]) return;
     const int v = i;
-    double vx, vy;
-    if (P.vel_model == SEVC_VEL_ANALYTIC) {
-        double fx, fy;
-        analytic_force(d, P, v, &fx, &fy);
-        vx = fx * P.mobility;
-        vy = fy * P.mobility;
-    } else {
-        double bx, gx, by, gy;
-        double ex = energy_diff(d, P, 0, v, &bx, &gx);
-        double ey = energy_diff(d, P, 1, v, &by, &gy);
-        vx = (-ex / SEVC_GRAD_EPS) * P.mobility;
-        vy = (-ey / SEVC_GRAD_EPS) * P.mobility;
-        if (d.diag) {
-            d.vparts[4 * v + 0] = -bx / SEVC_GRAD_EPS;
-            d.vparts[4 * v + 1] = -gx / SEVC_GRAD_EPS;
-            d.vparts[4 * v + 2] = -by / SEVC_GRAD_EPS;
-            d.vparts[4 * v + 3] = -gy / SEVC_GRAD_EPS;
+    const double D = P.D;
+    const double x = d.pos[2 * v], y = d.pos[2 * v + 1];
+    const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
+    const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
+    bool fast = (pyy == y) && (qx == x);
+    double bx = 0.0, by = 0.0, gx = 0.0, gy = 0.0;
+    if (fast) {
+        const bool uni = (P.gb_eps == 0.0);
+        for (int j = 0; j < 3; j++) {
+            const int b = d.vb[3 * v + j];
+            const int nb = other_end(d, b, v);
+            const double nbx = d.pos[2 * nb], nby = d.pos[2 * nb + 1];
+            const double lx = arclen(px, pyy, nbx, nby, D);
+            const double ly = arclen(qx, qy, nbx, nby, D);
+            const double L0 = d.blen[b];
+            const double gam = uni ? P.gb_scaling : d.ben[b];
+            bx += (gam * (lx - L0));
+            by += (gam * (ly - L0));
+        }
+        for (int j = 0; j < 3; j++) {
+            const int g = d.vg[3 * v + j];
+            double ax, ay;
+            if (!shifted_areas(d, g, v, x, y, px, qy, D, &ax, &ay)) { fast = false; break; }
+            const double a0 = fabs(d.garea[g]), se = d.gse[g];
+            gx += (se * (fabs(ax) - a0));
+            gy += (se * (fabs(ay) - a0));
         }
     }
-    d.vel[2 * v] = vx;
-    d.vel[2 * v + 1] = vy;
+    if (!fast) {
+        int s = atomicAdd(&d.cnt[C_VFALL], 1);
+        d.l_vfall[s] = v;
+        return;
+    }
+    d.vel[2 * v] = (-(bx + gx) / SEVC_GRAD_EPS) * P.mobility;
+    d.vel[2 * v + 1] = (-(by + gy) / SEVC_GRAD_EPS) * P.mobility;
+}
+
+// The reference's two energy_diff passes for one vertex (also stores the
+// velocity parts exported by export_data, calculus.h:62-69).
+__device__ void velocity_generic(const Dev& d, const Params& P, int v, bool parts) {
+    double bx, gx, by, gy;
+    double ex = energy_diff(d, P, 0, v, &bx, &gx);
+    double ey = energy_diff(d, P, 1, v, &by, &gy);
+    d.vel[2 * v] = (-ex / SEVC_GRAD_EPS) * P.mobility;
+    d.vel[2 * v + 1] = (-ey / SEVC_GRAD_EPS) * P.mobility;
+    if (parts) {
+        d.vparts[4 * v + 0] = -bx / SEVC_GRAD_EPS;
+        d.vparts[4 * v + 1] = -gx / SEVC_GRAD_EPS;
+        d.vparts[4 * v + 2] = -by / SEVC_GRAD_EPS;
+        d.vparts[4 * v + 3] = -gy / SEVC_GRAD_EPS;
+    }
+}
+
+// Every FD vertex through the two-pass path (diagnostic steps).
+__global__ void k_velocity_generic(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || P.vel_model != SEVC_VEL_FD || i >= bd_.e || !d.ven[i]) return;
+    velocity_generic(d, P, i, d.diag != 0);
+}
+
+__global__ void k_velocity_fallback(Dev d) {
+    const int n = d.cnt[C_VFALL];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int v = d.l_vfall[k];
+        int m = 0;
+        while (m + 1 < d.B && v >= d.prm[m + 1].voff) m++;
+        velocity_generic(d, d.prm[m], v, false);
+    }
+}
+
+// Analytic -grad E velocities (extension).
+__global__ void k_velocity_analytic(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || P.vel_model != SEVC_VEL_ANALYTIC || i >= bd_.e || !d.ven[i]) return;
+    double fx, fy;
+    analytic_force(d, P, i, &fx, &fy);
+    d.vel[2 * i] = fx * P.mobility;
+    d.vel[2 * i + 1] = fy * P.mobility;
 }
 
 // compute_grain_dAdts, calculus.h:243-253 (diagnostic)
@@ -1352,41 +1469,83 @@ __global__ void k_block_counts(Dev d, int kind) {
     if (threadIdx.x == 0) out[lb] = acc;
 }
 
+// Exclusive prefix sum of one int per thread over the workgroup (LDS).
+// Returns the prefix; *total receives the sum.  All threads must call it.
+__device__ int wg_exclusive_scan(int x, int* sh, int* total) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    sh[tid] = x;
+    __syncthreads();
+    for (int off = 1; off < nt; off <<= 1) {
+        int y = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += y;
+        __syncthreads();
+    }
+    int incl = sh[tid];
+    *total = sh[nt - 1];
+    __syncthreads();
+    return incl - x;
+}
+
+__device__ __forceinline__ int block_matches(const Blk* tbl, const int* bc, int b, int want) {
+    return want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
+}
+
 // Collect, in index order, the first R slots of a member whose enable flag
-// equals `want`, cooperatively in one workgroup.  Returns how many were
-// found (<= R).  `sh` is LDS scratch of blockDim.x + 1 ints.
+// equals `want`, cooperatively in one workgroup: block counts are scanned
+// 256 at a time, and each table block that holds wanted ranks is swept by
+// the whole workgroup with one lane per slot.  Returns how many were found
+// (<= R).  `sh` is LDS scratch of blockDim.x + 1 ints.
 __device__ int collect_slots(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want,
                              int R, int* out, int* sh) {
+    __shared__ int blist[256];
+    __shared__ int bstart[256];
+    __shared__ int nlist;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int per = (nblk + nt - 1) / nt;
-    const int b0 = first + tid * per;
-    const int b1 = min(first + nblk, b0 + per);
-    int mine = 0;
-    for (int b = b0; b < b1; b++) {
-        int sz = tbl[b].e - tbl[b].b;
-        mine += want ? bc[b] : sz - bc[b];
-    }
-    sh[tid] = mine;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int t = 0; t < nt; t++) { int x = sh[t]; sh[t] = run; run += x; }
-        sh[nt] = run;
-    }
-    __syncthreads();
-    int pos = sh[tid];
-    const int total = sh[nt];
-    if (pos < R) {
-        for (int b = b0; b < b1 && pos < R; b++) {
-            int sz = tbl[b].e - tbl[b].b;
-            int cntb = want ? bc[b] : sz - bc[b];
-            if (cntb == 0) continue;
-            for (int i = tbl[b].b; i < tbl[b].e && pos < R; i++)
-                if ((en[i] != 0) == (want != 0)) out[pos++] = i;
+    int found = 0;
+    for (int base = first; base < first + nblk && found < R; base += nt) {
+        const int b = base + tid;
+        const int c = (b < first + nblk) ? block_matches(tbl, bc, b, want) : 0;
+        int total;
+        const int pre = wg_exclusive_scan(c, sh, &total);
+        if (tid == 0) nlist = 0;
+        __syncthreads();
+        // blocks of this window that hold ranks in [found, R), in order
+        const bool need = c > 0 && found + pre < R;
+        const unsigned long long bal = __ballot(need);
+        const int wave = tid >> 6, lane = tid & 63;
+        __shared__ int wcnt[4];
+        if (lane == 0) wcnt[wave] = (int)__popcll(bal);
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wave; w++) woff += wcnt[w];
+        if (need) {
+            const int r = woff + (int)__popcll(bal & ((1ull << lane) - 1ull));
+            blist[r] = b;
+            bstart[r] = found + pre;
         }
+        if (tid == 0) { int t = 0; for (int w = 0; w < (nt >> 6); w++) t += wcnt[w]; nlist = t; }
+        __syncthreads();
+        const int nl = nlist;
+        for (int q = 0; q < nl; q++) {
+            const int bb = blist[q], st = bstart[q];
+            const int idx = tbl[bb].b + tid;
+            const bool hit = idx < tbl[bb].e && ((en[idx] != 0) == (want != 0));
+            const unsigned long long hb = __ballot(hit);
+            if (lane == 0) wcnt[wave] = (int)__popcll(hb);
+            __syncthreads();
+            int off = 0;
+            for (int w = 0; w < wave; w++) off += wcnt[w];
+            if (hit) {
+                const int r = st + off + (int)__popcll(hb & ((1ull << lane) - 1ull));
+                if (r < R) out[r] = idx;
+            }
+            __syncthreads();
+        }
+        found += total;
     }
     __syncthreads();
-    return total < R ? total : R;
+    return found < R ? found : R;
 }
 
 // nucleate, nucleation2.cu:474-676 (after its checks), with given slots.
@@ -1473,33 +1632,36 @@ __device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, i
 // member whose enable flag equals `want`; returns -1 if there is none.
 __device__ int find_kth(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want, int k,
                         int* sh) {
-    __shared__ int found;
+    __shared__ int fb, fr, found;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int per = (nblk + nt - 1) / nt;
-    const int b0 = first + tid * per, b1 = min(first + nblk, b0 + per);
-    int mine = 0;
-    for (int b = b0; b < b1; b++) mine += want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
-    sh[tid] = mine;
-    if (tid == 0) found = -1;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int t = 0; t < nt; t++) { int x = sh[t]; sh[t] = run; run += x; }
+    if (tid == 0) { fb = -1; fr = 0; found = -1; }
+    int seen = 0;
+    for (int base = first; base < first + nblk; base += nt) {
+        const int b = base + tid;
+        const int c = (b < first + nblk) ? block_matches(tbl, bc, b, want) : 0;
+        int total;
+        const int pre = wg_exclusive_scan(c, sh, &total);
+        if (c > 0 && k >= seen + pre && k < seen + pre + c) { fb = b; fr = k - seen - pre; }
+        __syncthreads();
+        if (fb >= 0) break;
+        seen += total;
     }
     __syncthreads();
-    int pos = sh[tid];
-    if (k >= pos && k < pos + mine) {
-        int r = k - pos;
-        for (int b = b0; b < b1; b++) {
-            int c = want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
-            if (r >= c) { r -= c; continue; }
-            for (int i = tbl[b].b; i < tbl[b].e; i++)
-                if ((en[i] != 0) == (want != 0)) { if (r == 0) { found = i; break; } r--; }
-            break;
-        }
+    const int bb = fb;
+    if (bb >= 0) {
+        const int wave = tid >> 6, lane = tid & 63;
+        __shared__ int wcnt[4];
+        const int idx = tbl[bb].b + tid;
+        const bool hit = idx < tbl[bb].e && ((en[idx] != 0) == (want != 0));
+        const unsigned long long hb = __ballot(hit);
+        if (lane == 0) wcnt[wave] = (int)__popcll(hb);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; w++) off += wcnt[w];
+        if (hit && off + (int)__popcll(hb & ((1ull << lane) - 1ull)) == fr) found = idx;
     }
     __syncthreads();
-    int f = found;
+    const int f = found;
     __syncthreads();
     return f;
 }
@@ -1571,7 +1733,15 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
         }
         __syncthreads();
         const int nc = ncand;
-        if (tid == 0) sort_ints(cand, nc);
+        // rank sort (distinct grain ids), through the kept region
+        for (int e = tid; e < nc; e += nt) {
+            const int g = cand[e];
+            int r = 0;
+            for (int f = 0; f < nc; f++) r += cand[f] < g ? 1 : 0;
+            kept[r] = g;
+        }
+        __syncthreads();
+        for (int e = tid; e < nc; e += nt) cand[e] = kept[e];
         __syncthreads();
         for (int e = tid; e < nc; e += nt) {
             const int g = cand[e];

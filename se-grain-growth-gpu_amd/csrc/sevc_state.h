@@ -37,7 +37,7 @@ struct Blk { int m, b, e; };
 
 enum Counter {
     C_CAND = 0, C_TWO, C_T2, C_T2P, C_T2C, C_FLIPP, C_FLIPC,
-    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NDE, C_NDG, C_ERR, C_COUNT
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NDE, C_NDG, C_VFALL, C_ERR, C_COUNT
 };
 
 struct Dev {
@@ -61,7 +61,7 @@ struct Dev {
     // work lists and scratch
     int* cnt;
     int* l_cand; int* l_two; int* l_t2; int* l_t2p; int* l_t2c; int* l_flipp; int* l_flipc;
-    int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_nde; int* l_ndg;
+    int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_nde; int* l_ndg; int* l_vfall;
     int* bc_v; int* bc_e; int* bc_g;      // per table block counts (k-th selection)
     double* partial;                      // per grain block partial sums (2 levels)
     unsigned long long* bmin;             // per edge block min t_ext bits
