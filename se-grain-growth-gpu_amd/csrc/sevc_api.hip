@@ -25,6 +25,8 @@ __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
 __global__ void k_velocity(Dev d);
+__global__ void k_vel_grain(Dev d);
+__global__ void k_vel_vertex(Dev d);
 __global__ void k_velocity_generic(Dev d);
 __global__ void k_velocity_fallback(Dev d);
 __global__ void k_velocity_analytic(Dev d);
@@ -160,7 +162,7 @@ void launch_step(sevc_state* s, bool diag) {
     const int nb = (s->B + 63) / 64;
     launch(s, "begin", k_begin, nb, 64, d);
     launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
-    launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
+    if (diag || s->any_analytic) launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
     if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     if (s->any_nucl) {
         launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
@@ -175,7 +177,8 @@ void launch_step(sevc_state* s, bool diag) {
     if (diag) {
         launch(s, "velocity_generic", k_velocity_generic, d.nvb, kBlock, d);
     } else {
-        launch(s, "velocity", k_velocity, d.nvb, kBlock, d);
+        launch(s, "vel_grain", k_vel_grain, d.ntile, 256, d);
+        launch(s, "velocity", k_vel_vertex, d.nvb, kBlock, d);
         launch(s, "velocity_fallback", k_velocity_fallback, 64, 256, d);
     }
     if (s->any_analytic) launch(s, "velocity_analytic", k_velocity_analytic, d.nvb, kBlock, d);
@@ -334,6 +337,21 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.partial, 2 * gt.size() + 2);
     DA(d.bmin, et.size()); DA(d.bcnt_g, gt.size()); DA(d.bcnt_v, vt.size());
     d.vblk = dvt; d.eblk = det; d.gblk = dgt;
+    {
+        std::vector<Blk> tt;
+        for (int m = 0; m < B; ++m)
+            for (int b = 0; b < s->ng[m]; b += kTile)
+                tt.push_back(Blk{m, s->goff[m] + b, s->goff[m] + std::min(b + kTile, s->ng[m])});
+        Blk* dtt;
+        DA(dtt, tt.size());
+        if (hipMemcpy(dtt, tt.data(), tt.size() * sizeof(Blk), hipMemcpyHostToDevice) != hipSuccess) {
+            sevc_destroy(s);
+            return SEVC_E_HIP;
+        }
+        d.gtile = dtt;
+        d.ntile = static_cast<int>(tt.size());
+    }
+    DA(d.vterm, 6 * V);
     Params* dprm;
     DA(dprm, static_cast<size_t>(B));
     DA(d.ms, static_cast<size_t>(B));

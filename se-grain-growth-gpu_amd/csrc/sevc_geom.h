@@ -9,29 +9,32 @@
 
 namespace sevc {
 
-// dom_mod, geometry.h:133-137
+// dom_mod, geometry.h:133-137.  Written with selects so that the compiler
+// emits v_cndmask rather than divergent branches; the candidate values are
+// the reference's a + D and a - D.
 __device__ __forceinline__ double dom_mod(double a, double D) {
-    if (a < 0) return a + D;
-    else if (a >= D) return a - D;
-    return a;
+    const double lo = a + D, hi = a - D;
+    const double r = (a >= D) ? hi : a;
+    return (a < 0) ? lo : r;
 }
 
-// wrap_dist, geometry.h:147-152
+// wrap_dist, geometry.h:147-152 (select form, same candidate values).
 __device__ __forceinline__ double wrapd(double dest, double orig, double D) {
-    double d = dest - orig;
-    if (d > +(D / 2.0)) return d - D;
-    if (d < -(D / 2.0)) return D + d;
-    return d;
+    const double d = dest - orig;
+    const double h = D / 2.0;
+    const double a = d - D, b = D + d;
+    const double r = (d > +h) ? a : d;
+    return (d < -h) ? b : r;
 }
 
 // vector2_mag, geometry.h:217-223: p*sqrt(1+(q/p)^2), p = max |.|, q = min |.|
 __device__ __forceinline__ double mag(double x, double y) {
-    double ax = fabs(x), ay = fabs(y);
-    double p = ax > ay ? ax : ay;
-    if (p == 0) return 0;
-    double q = ax < ay ? ax : ay;
-    double r = q / p;
-    return p * sqrt(1.0 + r * r);
+    const double ax = fabs(x), ay = fabs(y);
+    const double p = ax > ay ? ax : ay;
+    const double q = ax < ay ? ax : ay;
+    const double r = q / p;
+    const double m = p * sqrt(1.0 + r * r);
+    return (p == 0) ? 0.0 : m;
 }
 
 // compute_arclength, geometry.h:328-330: |wrap(a - b)|

@@ -515,6 +515,163 @@ __global__ void __launch_bounds__(256) k_velocity(Dev d) {
     d.vel[2 * v + 1] = (-(by + gy) / SEVC_GRAD_EPS) * P.mobility;
 }
 
+// Velocity, pass 1 of 2 (stored-energy part of energy_diff,
+// calculus.h:39-61, and compute_grain_areas, calculus.h:224-234): one
+// workgroup per tile of consecutive grains.  The tile's vertex lists and
+// the positions they name are staged in LDS with one wide pass over
+// memory; each grain's area is computed once (and stored, as
+// compute_grain_areas does); then every (grain, list slot) pair computes
+// the two shifted areas of its vertex (same single-pass arithmetic as
+// shifted_areas) and writes SE*(|A'|-|A|) for x and y into the vertex's
+// term slot of that grain (slot j where vertex.grains[j] == grain).
+constexpr int kTilePts = 1024;
+static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
+__global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
+    __shared__ double2 Ps[kTilePts];
+    __shared__ int Ls[kTilePts];
+    __shared__ unsigned char pgl[kTilePts];
+    __shared__ int off[kTile + 1];
+    __shared__ double area_s[kTile];
+    __shared__ int gend_s;
+    const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb >= d.ntile) return;
+    const Blk bd = d.gtile[lb];
+    const Params& P = d.prm[bd.m];
+    const MState& M = d.ms[bd.m];
+    if (!M.active || P.vel_model != SEVC_VEL_FD) return;
+    const double D = P.D;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int gs = bd.b; gs < bd.e;) {
+        if (tid < kTile) {
+            // wave 0: inclusive scan of list lengths; the tile takes the
+            // longest prefix of grains whose lists fit in kTilePts points
+            const int g = gs + tid;
+            const int n = (g < bd.e && d.genab[g]) ? d.glen[g] : 0;
+            int inc = n;
+            for (int o = 1; o < kTile; o <<= 1) {
+                const int t = __shfl_up(inc, o, kTile);
+                if (tid >= o) inc += t;
+            }
+            const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
+            const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
+            if (tid < cnt) off[tid] = inc - n;
+            if (tid == cnt - 1) off[cnt] = inc;
+            if (tid == 0) {
+                gend_s = gs + cnt;
+                if (cnt == 0) off[0] = 0;
+            }
+        }
+        __syncthreads();
+        const int gend = gend_s, ngr = gend - gs, npts = off[ngr];
+        for (int g = tid; g < ngr; g += nt)
+            for (int q = off[g]; q < off[g + 1]; q++) pgl[q] = (unsigned char)g;
+        __syncthreads();
+        for (int p = tid; p < npts; p += nt) {
+            const int g = pgl[p];
+            const int w = glist(d, gs + g)[p - off[g]];
+            Ls[p] = w;
+            Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
+        }
+        __syncthreads();
+        for (int g = tid; g < ngr; g += nt) {
+            const int b0 = off[g], n = off[g + 1] - b0;
+            double a = 0.0;
+            if (n > 0) {
+                AreaAcc acc;
+                acc.start(Ps[b0].x, Ps[b0].y);
+                for (int k = 1; k < n; k++) acc.push(Ps[b0 + k].x, Ps[b0 + k].y, D);
+                a = acc.finish();
+            }
+            if (d.genab[gs + g]) d.garea[gs + g] = a;
+            area_s[g] = a;
+        }
+        __syncthreads();
+        for (int p = tid; p < npts; p += nt) {
+            const int g = pgl[p], b0 = off[g], n = off[g + 1] - b0, k = p - b0;
+            const int v = Ls[p], gg = gs + g;
+            const double x = Ps[p].x, y = Ps[p].y;
+            const double px = dom_mod(x + SEVC_GRAD_EPS, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
+            double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
+            if (k > 0) {
+                exk = wrapd(px, Ps[p - 1].x, D);
+                eyk = wrapd(qy, Ps[p - 1].y, D);
+            }
+            if (k < n - 1) {
+                exk1 = wrapd(Ps[p + 1].x, px, D);
+                eyk1 = wrapd(Ps[p + 1].y, qy, D);
+            }
+            double prx = Ps[b0].x, pry = Ps[b0].y;
+            double ux = 0, uy = 0, xx = 0, yy = 0, sx = 0, sy = 0;
+            for (int i = 1; i < n; i++) {
+                const double cx = Ps[b0 + i].x, cy = Ps[b0 + i].y;
+                const double dux = wrapd(cx, prx, D), duy = wrapd(cy, pry, D);
+                prx = cx;
+                pry = cy;
+                const double dxx = (i == k) ? exk : ((i == k + 1) ? exk1 : dux);
+                const double dyy = (i == k) ? eyk : ((i == k + 1) ? eyk1 : duy);
+                const double nux = ux + dux, nuy = uy + duy;
+                const double nxx = xx + dxx, nyy = yy + dyy;
+                sx += xx * nuy - uy * nxx;
+                sy += ux * nyy - yy * nux;
+                ux = nux; uy = nuy; xx = nxx; yy = nyy;
+            }
+            const double z = 0;
+            sx += xx * z - uy * z;
+            sy += ux * z - yy * z;
+            const double ax = sx * 0.5, ay = sy * 0.5;
+            const double a0 = fabs(area_s[g]), se = d.gse[gg];
+            const double tx = se * (fabs(ax) - a0), ty = se * (fabs(ay) - a0);
+            for (int j = 0; j < 3; j++)
+                if (d.vg[3 * v + j] == gg) {
+                    d.vterm[6 * (size_t)v + 2 * j] = tx;
+                    d.vterm[6 * (size_t)v + 2 * j + 1] = ty;
+                }
+        }
+        __syncthreads();
+        gs = gend;
+    }
+}
+
+// Velocity, pass 2 of 2: the line-tension part of energy_diff
+// (calculus.h:28-37,57-60) -- six shifted lengths -- plus the three
+// stored-energy terms of pass 1 summed in vertex.grains order, then
+// v = -dE/grad_eps * mobility (calculus.h:149-150).  Vertices whose
+// unshifted coordinate dom_mod would move (a coordinate equal to D) are
+// listed for the two-pass path.
+__global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (!M.active || P.vel_model != SEVC_VEL_FD || i >= bd_.e || !d.ven[i]) return;
+    const int v = i;
+    const double D = P.D;
+    const double x = d.pos[2 * v], y = d.pos[2 * v + 1];
+    const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
+    const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
+    if (!((pyy == y) && (qx == x))) {
+        int s = atomicAdd(&d.cnt[C_VFALL], 1);
+        d.l_vfall[s] = v;
+        return;
+    }
+    const bool uni = (P.gb_eps == 0.0);
+    double bx = 0.0, by = 0.0, gx = 0.0, gy = 0.0;
+    for (int j = 0; j < 3; j++) {
+        const int b = d.vb[3 * v + j];
+        const int nb = other_end(d, b, v);
+        const double nbx = d.pos[2 * nb], nby = d.pos[2 * nb + 1];
+        const double lx = arclen(px, pyy, nbx, nby, D);
+        const double ly = arclen(qx, qy, nbx, nby, D);
+        const double L0 = d.blen[b];
+        const double gam = uni ? P.gb_scaling : d.ben[b];
+        bx += (gam * (lx - L0));
+        by += (gam * (ly - L0));
+    }
+    const double* t = d.vterm + 6 * (size_t)v;
+    gx += t[0]; gy += t[1];
+    gx += t[2]; gy += t[3];
+    gx += t[4]; gy += t[5];
+    d.vel[2 * v] = (-(bx + gx) / SEVC_GRAD_EPS) * P.mobility;
+    d.vel[2 * v + 1] = (-(by + gy) / SEVC_GRAD_EPS) * P.mobility;
+}
+
 // The reference's two energy_diff passes for one vertex (also stores the
 // velocity parts exported by export_data, calculus.h:62-69).
 __device__ void velocity_generic(const Dev& d, const Params& P, int v, bool parts) {

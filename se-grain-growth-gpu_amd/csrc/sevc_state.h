@@ -13,6 +13,7 @@ namespace sevc {
 
 constexpr int kBlock = 256;          // elements per table block (one workgroup)
 constexpr int kFixStages = 3;        // 0: two-grain, 1: T2 removal, 2: flips
+constexpr int kTile = 64;            // grains per velocity tile (pass 1)
 
 // Per-member run parameters (from sevc_options).
 struct Params {
@@ -57,6 +58,8 @@ struct Dev {
     const Params* prm; MState* ms; int B;
     const Blk* vblk; const Blk* eblk; const Blk* gblk;
     int nvb, neb, ngb;
+    const Blk* gtile; int ntile;         // 64-grain tiles for the velocity pass 1
+    double* vterm;                       // 6 per vertex: stored-energy terms (x,y) per grain slot
     int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
     // work lists and scratch
     int* cnt;

@@ -12,7 +12,7 @@ from . import _abi
 from ._abi import HostMesh, Mesh, Options, StepStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libsevc.so")
+LIB_PATH = os.environ.get("SEVC_LIB_PATH") or os.path.join(os.path.dirname(_HERE), "build", "libsevc.so")
 
 _lib = None
 
