@@ -24,7 +24,6 @@ __global__ void k_end(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
-__global__ void k_velocity(Dev d);
 __global__ void k_vel_grain(Dev d);
 __global__ void k_vel_vertex(Dev d);
 __global__ void k_velocity_generic(Dev d);
@@ -352,6 +351,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         d.ntile = static_cast<int>(tt.size());
     }
     DA(d.vterm, 6 * V);
+    DA(d.slist, static_cast<size_t>(d.ngb) + d.neb + d.nvb);
     Params* dprm;
     DA(dprm, static_cast<size_t>(B));
     DA(d.ms, static_cast<size_t>(B));

@@ -413,117 +413,27 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
     *fy = -(ey + gy);
 }
 
-// Both perturbed areas of grain g around vertex v in one pass over the
-// polygon (the reference's compute_area on the x-shifted and on the
-// y-shifted copy, geometry.h:757-765 after :908-931).
-//
-// With the shifted point (px, y) the Y track of the unwrap is the
-// unperturbed one and the X track differs from it only in the deltas that
-// enter and leave position k (the slot of v); likewise for (x, qy).  Those
-// four deltas are computed before the loop from v's two neighbours in the
-// list, and every other delta is shared.  Each track is still the
-// reference's sequential sum U_i = U_{i-1} + delta_i, and each area the
-// reference's left-to-right shoelace sum, so the results are bit-identical.
-// Requires dom_mod(y) == y and dom_mod(x) == x (checked by the caller).
-__device__ __forceinline__ bool shifted_areas(const Dev& d, int g, int v, double x, double y, double px, double qy,
-                                              double D, double* ax, double* ay) {
-    const int* L = glist(d, g);
-    const int n = d.glen[g];
-    int k = -1;
-    for (int i = 0; i < n; i++) if (L[i] == v) { k = i; break; }
-    if (k < 0) return false;
-    double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
-    if (k > 0) {
-        const int w = L[k - 1];
-        exk = wrapd(px, d.pos[2 * w], D);
-        eyk = wrapd(qy, d.pos[2 * w + 1], D);
-    }
-    if (k < n - 1) {
-        const int w = L[k + 1];
-        exk1 = wrapd(d.pos[2 * w], px, D);
-        eyk1 = wrapd(d.pos[2 * w + 1], qy, D);
-    }
-    int w0 = L[0];
-    double prx = d.pos[2 * w0], pry = d.pos[2 * w0 + 1];
-    double ux = 0, uy = 0, xx = 0, yy = 0, sx = 0, sy = 0;
-    for (int i = 1; i < n; i++) {
-        const int w = L[i];
-        const double cx = d.pos[2 * w], cy = d.pos[2 * w + 1];
-        const double dux = wrapd(cx, prx, D), duy = wrapd(cy, pry, D);
-        prx = cx;
-        pry = cy;
-        const double dxx = (i == k) ? exk : ((i == k + 1) ? exk1 : dux);
-        const double dyy = (i == k) ? eyk : ((i == k + 1) ? eyk1 : duy);
-        const double nux = ux + dux, nuy = uy + duy;
-        const double nxx = xx + dxx, nyy = yy + dyy;
-        sx += xx * nuy - uy * nxx;
-        sy += ux * nyy - yy * nux;
-        ux = nux; uy = nuy; xx = nxx; yy = nyy;
-    }
-    const double z = 0;
-    sx += xx * z - uy * z;
-    sy += ux * z - yy * z;
-    *ax = sx * 0.5;
-    *ay = sy * 0.5;
-    return true;
-}
-
-// compute_vertex_velocities, calculus.h:143-154 (x mobility).  The two
-// energy_diff calls (calculus.h:19-73) are evaluated together: six
-// perturbed lengths and the two perturbed areas of each of the three
-// grains, accumulated in the reference's order.  Vertices the single pass
-// cannot take (a coordinate equal to D, so that dom_mod moves the unshifted
-// coordinate) are listed for k_velocity_generic.
-__global__ void __launch_bounds__(256) k_velocity(Dev d) {
-    TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active || P.vel_model != SEVC_VEL_FD || i >= bd_.e || !d.ven[i]) return;
-    const int v = i;
-    const double D = P.D;
-    const double x = d.pos[2 * v], y = d.pos[2 * v + 1];
-    const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
-    const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
-    bool fast = (pyy == y) && (qx == x);
-    double bx = 0.0, by = 0.0, gx = 0.0, gy = 0.0;
-    if (fast) {
-        const bool uni = (P.gb_eps == 0.0);
-        for (int j = 0; j < 3; j++) {
-            const int b = d.vb[3 * v + j];
-            const int nb = other_end(d, b, v);
-            const double nbx = d.pos[2 * nb], nby = d.pos[2 * nb + 1];
-            const double lx = arclen(px, pyy, nbx, nby, D);
-            const double ly = arclen(qx, qy, nbx, nby, D);
-            const double L0 = d.blen[b];
-            const double gam = uni ? P.gb_scaling : d.ben[b];
-            bx += (gam * (lx - L0));
-            by += (gam * (ly - L0));
-        }
-        for (int j = 0; j < 3; j++) {
-            const int g = d.vg[3 * v + j];
-            double ax, ay;
-            if (!shifted_areas(d, g, v, x, y, px, qy, D, &ax, &ay)) { fast = false; break; }
-            const double a0 = fabs(d.garea[g]), se = d.gse[g];
-            gx += (se * (fabs(ax) - a0));
-            gy += (se * (fabs(ay) - a0));
-        }
-    }
-    if (!fast) {
-        int s = atomicAdd(&d.cnt[C_VFALL], 1);
-        d.l_vfall[s] = v;
-        return;
-    }
-    d.vel[2 * v] = (-(bx + gx) / SEVC_GRAD_EPS) * P.mobility;
-    d.vel[2 * v + 1] = (-(by + gy) / SEVC_GRAD_EPS) * P.mobility;
-}
-
 // Velocity, pass 1 of 2 (stored-energy part of energy_diff,
 // calculus.h:39-61, and compute_grain_areas, calculus.h:224-234): one
 // workgroup per tile of consecutive grains.  The tile's vertex lists and
 // the positions they name are staged in LDS with one wide pass over
 // memory; each grain's area is computed once (and stored, as
 // compute_grain_areas does); then every (grain, list slot) pair computes
-// the two shifted areas of its vertex (same single-pass arithmetic as
-// shifted_areas) and writes SE*(|A'|-|A|) for x and y into the vertex's
-// term slot of that grain (slot j where vertex.grains[j] == grain).
+// the two shifted areas of its vertex and writes SE*(|A'|-|A|) for x and
+// y into the vertex's term slot of that grain (slot j where
+// vertex.grains[j] == grain).
+//
+// Both shifted areas come from one pass over the polygon (the reference's
+// compute_area on the x-shifted and on the y-shifted copy,
+// geometry.h:757-765 after :908-931).  With the shifted point (px, y) the Y
+// track of the unwrap is the unperturbed one and the X track differs from
+// it only in the deltas that enter and leave position k (the vertex's
+// slot); likewise for (x, qy).  Those four deltas are computed first from
+// the two list neighbours and every other delta is shared.  Each track is
+// still the reference's sequential sum U_i = U_{i-1} + delta_i, and each
+// area the reference's left-to-right shoelace sum, so the results are
+// bit-identical.  Pass 2 sends vertices with dom_mod(x) != x or
+// dom_mod(y) != y to the two-pass path, which this form does not cover.
 constexpr int kTilePts = 1024;
 static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
 __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
@@ -1605,12 +1515,13 @@ __global__ void k_rate_cand(Dev d) {
     d.l_nucl[s] = i;
 }
 
-// Per table block: number of enabled entities (for free-slot search).
+// Per table block: numbers of enabled and of free slots (for the slot
+// searches of the nucleation manager).
 __global__ void k_block_counts(Dev d, int kind) {
     const Blk* tbl = kind == 0 ? d.vblk : (kind == 1 ? d.eblk : d.gblk);
     const int nblk = kind == 0 ? d.nvb : (kind == 1 ? d.neb : d.ngb);
     const uint8_t* en = kind == 0 ? d.ven : (kind == 1 ? d.benab : d.genab);
-    int* out = kind == 0 ? d.bc_v : (kind == 1 ? d.bc_e : d.bc_g);
+    int2* out = kind == 0 ? d.bc_v : (kind == 1 ? d.bc_e : d.bc_g);
     int lb = blockIdx.x;
     if (lb >= nblk) return;
     Blk bd = tbl[lb];
@@ -1623,7 +1534,7 @@ __global__ void k_block_counts(Dev d, int kind) {
     unsigned long long bal = __ballot(c);
     if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&acc, (int)__popcll(bal));
     __syncthreads();
-    if (threadIdx.x == 0) out[lb] = acc;
+    if (threadIdx.x == 0) out[lb] = make_int2(acc, (bd.e - bd.b) - acc);
 }
 
 // Exclusive prefix sum of one int per thread over the workgroup (LDS).
@@ -1644,65 +1555,85 @@ __device__ int wg_exclusive_scan(int x, int* sh, int* total) {
     return incl - x;
 }
 
-__device__ __forceinline__ int block_matches(const Blk* tbl, const int* bc, int b, int want) {
-    return want ? bc[b] : (tbl[b].e - tbl[b].b) - bc[b];
+__device__ __forceinline__ int block_matches(const int2* bc, int b, int want) {
+    const int2 c = bc[b];
+    return want ? c.x : c.y;
+}
+
+// Split of a member's table blocks into one run of consecutive blocks per
+// lane, with the number of wanted slots in each run and its exclusive
+// prefix over the workgroup.  The searches below walk only the runs that
+// hold the ranks they need.
+struct RunScan {
+    int lo, hi;     // this lane's blocks [lo, hi)
+    int cnt, pre;   // wanted slots in the run, and before it
+    int total;      // wanted slots of the member
+};
+
+__device__ RunScan run_scan(const Blk* tbl, int first, int nblk, const int2* bc, int want, int* sh) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int C = (nblk + nt - 1) / nt;
+    RunScan r;
+    r.lo = first + min(nblk, tid * C);
+    r.hi = first + min(nblk, (tid + 1) * C);
+    int c = 0, b = r.lo;
+    for (; b + 8 <= r.hi; b += 8) {
+        int t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = block_matches(bc, b + q, want);
+#pragma unroll
+        for (int q = 0; q < 8; q++) c += t[q];
+    }
+    for (; b < r.hi; b++) c += block_matches(bc, b, want);
+    r.cnt = c;
+    r.pre = wg_exclusive_scan(c, sh, &r.total);
+    return r;
 }
 
 // Collect, in index order, the first R slots of a member whose enable flag
-// equals `want`, cooperatively in one workgroup: block counts are scanned
-// 256 at a time, and each table block that holds wanted ranks is swept by
-// the whole workgroup with one lane per slot.  Returns how many were found
-// (<= R).  `sh` is LDS scratch of blockDim.x + 1 ints.
-__device__ int collect_slots(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want,
-                             int R, int* out, int* sh) {
-    __shared__ int blist[256];
-    __shared__ int bstart[256];
-    __shared__ int nlist;
+// equals `want`, cooperatively in one workgroup.  Lanes whose run holds
+// ranks below R list (block, first rank) for each block with wanted slots;
+// the waves then sweep the listed blocks 64 slots at a time and write each
+// wanted slot at its rank.  `blist` holds at least nblk entries; `sh` is
+// LDS scratch of blockDim.x + 1 ints.  Returns min(R, wanted slots).
+__device__ int collect_slots(const Blk* tbl, int first, int nblk, const int2* bc, const uint8_t* en, int want,
+                             int R, int* out, int* sh, int2* blist) {
+    __shared__ int nl;
     const int tid = threadIdx.x, nt = blockDim.x;
-    int found = 0;
-    for (int base = first; base < first + nblk && found < R; base += nt) {
-        const int b = base + tid;
-        const int c = (b < first + nblk) ? block_matches(tbl, bc, b, want) : 0;
-        int total;
-        const int pre = wg_exclusive_scan(c, sh, &total);
-        if (tid == 0) nlist = 0;
-        __syncthreads();
-        // blocks of this window that hold ranks in [found, R), in order
-        const bool need = c > 0 && found + pre < R;
-        const unsigned long long bal = __ballot(need);
-        const int wave = tid >> 6, lane = tid & 63;
-        __shared__ int wcnt[4];
-        if (lane == 0) wcnt[wave] = (int)__popcll(bal);
-        __syncthreads();
-        int woff = 0;
-        for (int w = 0; w < wave; w++) woff += wcnt[w];
-        if (need) {
-            const int r = woff + (int)__popcll(bal & ((1ull << lane) - 1ull));
-            blist[r] = b;
-            bstart[r] = found + pre;
-        }
-        if (tid == 0) { int t = 0; for (int w = 0; w < (nt >> 6); w++) t += wcnt[w]; nlist = t; }
-        __syncthreads();
-        const int nl = nlist;
-        for (int q = 0; q < nl; q++) {
-            const int bb = blist[q], st = bstart[q];
-            const int idx = tbl[bb].b + tid;
-            const bool hit = idx < tbl[bb].e && ((en[idx] != 0) == (want != 0));
-            const unsigned long long hb = __ballot(hit);
-            if (lane == 0) wcnt[wave] = (int)__popcll(hb);
-            __syncthreads();
-            int off = 0;
-            for (int w = 0; w < wave; w++) off += wcnt[w];
-            if (hit) {
-                const int r = st + off + (int)__popcll(hb & ((1ull << lane) - 1ull));
-                if (r < R) out[r] = idx;
+    const RunScan rs = run_scan(tbl, first, nblk, bc, want, sh);
+    if (tid == 0) nl = 0;
+    __syncthreads();
+    if (rs.cnt > 0 && rs.pre < R) {
+        int rank = rs.pre;
+        for (int b = rs.lo; b < rs.hi && rank < R; b++) {
+            const int c = block_matches(bc, b, want);
+            if (c > 0) {
+                blist[atomicAdd(&nl, 1)] = make_int2(b, rank);
+                rank += c;
             }
-            __syncthreads();
         }
-        found += total;
     }
     __syncthreads();
-    return found < R ? found : R;
+    const int n = nl;
+    const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int q = wave; q < n; q += nw) {
+        const int2 e = blist[q];
+        const int hi = tbl[e.x].e;
+        int rank = e.y;
+        for (int i = tbl[e.x].b; i < hi && rank < R; i += 64) {
+            const int idx = i + lane;
+            const bool hit = idx < hi && ((en[idx] != 0) == (want != 0));
+            const unsigned long long hb = __ballot(hit);
+            if (hit) {
+                const int r = rank + (int)__popcll(hb & below);
+                if (r < R) out[r] = idx;
+            }
+            rank += (int)__popcll(hb);
+        }
+    }
+    __syncthreads();
+    return rs.total < R ? rs.total : R;
 }
 
 // nucleate, nucleation2.cu:474-676 (after its checks), with given slots.
@@ -1785,37 +1716,35 @@ __device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, i
     return 0;
 }
 
-// Cooperative search, in one workgroup, of the k-th (0-based) slot of a
-// member whose enable flag equals `want`; returns -1 if there is none.
-__device__ int find_kth(const Blk* tbl, int first, int nblk, const int* bc, const uint8_t* en, int want, int k,
-                        int* sh) {
+// The k-th (0-based) slot of a member whose enable flag equals `want`,
+// given the member's run scan; -1 if there is none.  The lane whose run
+// holds rank k finds the block, wave 0 sweeps that block.
+__device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const uint8_t* en, int want, int k) {
     __shared__ int fb, fr, found;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x;
     if (tid == 0) { fb = -1; fr = 0; found = -1; }
-    int seen = 0;
-    for (int base = first; base < first + nblk; base += nt) {
-        const int b = base + tid;
-        const int c = (b < first + nblk) ? block_matches(tbl, bc, b, want) : 0;
-        int total;
-        const int pre = wg_exclusive_scan(c, sh, &total);
-        if (c > 0 && k >= seen + pre && k < seen + pre + c) { fb = b; fr = k - seen - pre; }
-        __syncthreads();
-        if (fb >= 0) break;
-        seen += total;
+    __syncthreads();
+    if (k >= rs.pre && k < rs.pre + rs.cnt) {
+        int rank = rs.pre;
+        for (int b = rs.lo; b < rs.hi; b++) {
+            const int c = block_matches(bc, b, want);
+            if (k < rank + c) { fb = b; fr = k - rank; break; }
+            rank += c;
+        }
     }
     __syncthreads();
     const int bb = fb;
-    if (bb >= 0) {
-        const int wave = tid >> 6, lane = tid & 63;
-        __shared__ int wcnt[4];
-        const int idx = tbl[bb].b + tid;
-        const bool hit = idx < tbl[bb].e && ((en[idx] != 0) == (want != 0));
-        const unsigned long long hb = __ballot(hit);
-        if (lane == 0) wcnt[wave] = (int)__popcll(hb);
-        __syncthreads();
-        int off = 0;
-        for (int w = 0; w < wave; w++) off += wcnt[w];
-        if (hit && off + (int)__popcll(hb & ((1ull << lane) - 1ull)) == fr) found = idx;
+    if (bb >= 0 && tid < 64) {
+        const int hi = tbl[bb].e, lane = tid;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int rank = 0;
+        for (int i = tbl[bb].b; i < hi && rank <= fr; i += 64) {
+            const int idx = i + lane;
+            const bool hit = idx < hi && ((en[idx] != 0) == (want != 0));
+            const unsigned long long hb = __ballot(hit);
+            if (hit && rank + (int)__popcll(hb & below) == fr) found = idx;
+            rank += (int)__popcll(hb);
+        }
     }
     __syncthreads();
     const int f = found;
@@ -1858,20 +1787,14 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
     __syncthreads();
     if (P.nucl_mode == SEVC_NUCL_MC) {
         const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
-        int part = 0;
-        for (int b = first + tid; b < first + nb; b += nt) part += d.bc_v[b];
-        sh[tid] = part;
-        __syncthreads();
-        if (tid == 0) { int t = 0; for (int q = 0; q < nt; q++) t += sh[q]; sh[nt] = t; }
-        __syncthreads();
-        const int n_alive = sh[nt];
-        __syncthreads();
+        const RunScan rs = run_scan(d.vblk, first, nb, d.bc_v, 1, sh);
+        const int n_alive = rs.total;
         if (n_alive == 0) return;
         double u0, u1;
         sevc_uniform2(P.seed, (uint32_t)M.n, 0u, kStreamMcPick, 0u, &u0, &u1);
         int k = (int)(u0 * (double)n_alive);
         if (k >= n_alive) k = n_alive - 1;
-        const int A = find_kth(d.vblk, first, nb, d.bc_v, d.ven, 1, k, sh);
+        const int A = find_kth(d.vblk, rs, d.bc_v, d.ven, 1, k);
         if (tid == 0 && A >= 0) {
             d.vnuc[A] = 0;
             if (d.dE[A] < 0) d.vnuc[A] = 1;
@@ -1955,11 +1878,11 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
     if (K == 0) return;
     // free slots in rank order, taken before any insertion of this step
     int a = collect_slots(d.gblk, d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], d.bc_g, d.genab, 0,
-                          K, fg, sh);
+                          K, fg, sh, d.slist + d.gblk_first[m]);
     int b = collect_slots(d.eblk, d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], d.bc_e, d.benab, 0,
-                          3 * K, fe, sh);
+                          3 * K, fe, sh, d.slist + d.ngb + d.eblk_first[m]);
     int c = collect_slots(d.vblk, d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], d.bc_v, d.ven, 0,
-                          2 * K, fv, sh);
+                          2 * K, fv, sh, d.slist + d.ngb + d.neb + d.vblk_first[m]);
     if (tid == 0) { fg_n = a; fe_n = b; fv_n = c; }
     __syncthreads();
     for (int r = tid; r < K; r += nt) {

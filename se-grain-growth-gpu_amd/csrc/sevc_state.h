@@ -60,12 +60,13 @@ struct Dev {
     int nvb, neb, ngb;
     const Blk* gtile; int ntile;         // 64-grain tiles for the velocity pass 1
     double* vterm;                       // 6 per vertex: stored-energy terms (x,y) per grain slot
+    int2* slist;                         // free-slot search: (block, rank) lists, ngb+neb+nvb entries
     int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
     // work lists and scratch
     int* cnt;
     int* l_cand; int* l_two; int* l_t2; int* l_t2p; int* l_t2c; int* l_flipp; int* l_flipc;
     int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_nde; int* l_ndg; int* l_vfall;
-    int* bc_v; int* bc_e; int* bc_g;      // per table block counts (k-th selection)
+    int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
     double* partial;                      // per grain block partial sums (2 levels)
     unsigned long long* bmin;             // per edge block min t_ext bits
     int* bcnt_g; int* bcnt_v;             // per block enabled counts
