@@ -88,7 +88,7 @@ typedef struct sevc_options {
     int32_t adaptive_dt;       /* 0: fixed dt (reference)                       */
     double dt_safety;          /* dt_n = clamp(dt_safety * tau_min, dt_min, dt) */
     double dt_min;
-    double flip_threshold;     /* edges shorter than this that shrink also flip (0 = off) */
+    double flip_threshold;     /* edges of length in (0, threshold) that shrink also flip (0 = off) */
     int32_t nucleation_mode;   /* SEVC_NUCL_MC / SEVC_NUCL_RATE                 */
     double nucleation_rate;    /* RATE: p_g = nucleation_rate * SE_g per step   */
     uint64_t seed;             /* Philox key for nucleation                     */
@@ -261,7 +261,8 @@ int sevc_request_diagnostics(sevc_state* s);
  * number of entries written.  Used by the benchmark for roofline figures. */
 int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], double* ms,
                       int64_t* launches);
-/* on: 0 off, 1 every kernel, 2 only velocity / translate / text. */
+/* on: 0 off, 1 every kernel, 2 only the velocity passes ("vel_grain",
+ * "velocity"), "translate" and "text". */
 int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
 
 /* Device self-test of the shared arithmetic: evaluates op on n inputs on

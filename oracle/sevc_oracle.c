@@ -346,7 +346,9 @@ static void vertex_invert_boundary_order(orc* s, int v) {
 static int boundary_is_candidate(const orc* s, int b, double dt) {
     double te = s->bt[b];
     if (te >= 0 && te <= dt) return 1;
-    if (s->o.flip_threshold > 0 && te >= 0 && s->blen[b] < s->o.flip_threshold) return 1;
+    /* a zero-length edge (just flipped; its t_ext is the no-motion value)
+     * is not shrinking, so the length arm needs L > 0 */
+    if (s->o.flip_threshold > 0 && te >= 0 && s->blen[b] > 0 && s->blen[b] < s->o.flip_threshold) return 1;
     return 0;
 }
 

@@ -127,8 +127,8 @@ void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args
     if (grid <= 0) return;
     bool timed = s->timing;
     if (timed && s->timing_hot_only)
-        timed = std::strcmp(name, "velocity") == 0 || std::strcmp(name, "translate") == 0 ||
-                std::strcmp(name, "text") == 0;
+        timed = std::strcmp(name, "vel_grain") == 0 || std::strcmp(name, "velocity") == 0 ||
+                std::strcmp(name, "translate") == 0 || std::strcmp(name, "text") == 0;
     if (timed) {
         KTimer t{name, take_event(s), take_event(s)};
         (void)hipEventRecord(t.a, s->stream);

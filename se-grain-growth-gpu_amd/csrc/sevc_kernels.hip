@@ -115,7 +115,8 @@ __device__ __forceinline__ double boundary_compute_energy(const Dev& d, int b, d
 // geometry.h:513-517 plus the flip-threshold extension
 __device__ __forceinline__ bool is_candidate(double te, double len, double dt, double thr) {
     if (te >= 0 && te <= dt) return true;
-    if (thr > 0 && te >= 0 && len < thr) return true;
+    // a zero-length edge (just flipped, t_ext = no-motion value) is not shrinking
+    if (thr > 0 && te >= 0 && len > 0 && len < thr) return true;
     return false;
 }
 

@@ -198,3 +198,27 @@ def test_single_flip_and_single_removal_follow_the_diagrams():
         if seen_flip and seen_t2:
             break
     assert seen_flip and seen_t2
+
+
+def test_flip_threshold_spares_zero_length_edges():
+    """Flip-threshold extension: an edge shorter than the threshold that is
+    shrinking (0 <= t_ext) flips, but the zero-length edge a flip leaves
+    behind (both ends at the midpoint, t_ext = the no-motion value 100) is
+    not shrinking and must not flip straight back on the next step."""
+    thr = 3e-3
+    o, mesh = _small_mesh(n=400, seed=6, dt=2e-4, flip_threshold=thr)
+    orc = Oracle(o, mesh)
+    zero_seen = flips = 0
+    for _ in range(150):
+        pre = orc.get_mesh()
+        bv = pre["b_vtx"].reshape(-1, 2)
+        P = pre.xy()
+        d = _wrap(P[bv[:, 0]] - P[bv[:, 1]], o.domain_bound)
+        zero = (np.hypot(d[:, 0], d[:, 1]) == 0) & (pre["b_enabled"] == 1)
+        rc, st = orc.step(1)
+        assert rc == 0
+        post = orc.get_mesh()
+        assert not np.any(post["b_candidate"][zero]), np.nonzero(zero & (post["b_candidate"] == 1))
+        zero_seen += int(zero.sum())
+        flips += st.n_flips
+    assert flips > 0 and zero_seen > 0, (flips, zero_seen)
