@@ -66,6 +66,64 @@ def make_mesh(sevc, n_grains, seed):
     return o, sevc.build_mesh(o, pos, rid, ori, se, n_grains)
 
 
+def make_ensemble(sevc, members, seed0=1000, g_lo=1000, g_hi=50000):
+    """Ensemble workload: member i has G_i = round(exp(ln g_lo + U_i ln(g_hi/g_lo)))
+    grains (log-uniform), Voronoi seed seed0+i, SE = s_i * U[0,1) with
+    s_i ~ U[0,1], and the model options of the single-domain workload."""
+    opts, meshes = [], []
+    for i in range(members):
+        rng = np.random.default_rng(seed0 + i)
+        g = int(round(np.exp(np.log(g_lo) + rng.random() * np.log(g_hi / g_lo))))
+        s_i = rng.random()
+        pos, rid = sevc.voronoi(g, seed0 + i)
+        o = make_options(sevc, g)
+        o.seed = seed0 + i
+        meshes.append(sevc.build_mesh(o, pos, rid, rng.random(g), s_i * rng.random(g), g))
+        opts.append(o)
+    return opts, meshes
+
+
+def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
+    """All members stepped together in one state (one launch sequence per
+    step for the whole ensemble) against the same members stepped one state
+    at a time, back to back."""
+    t0 = time.time()
+    opts, meshes = make_ensemble(sevc, members)
+    grains = sum(o.n_grains for o in opts)
+    log(f"ensemble: {members} members, {grains} grains ({time.time() - t0:.1f} s)")
+    st = sevc.State(opts, meshes)
+    st.step(warmup)
+    st.step(0)
+    t1 = time.perf_counter()
+    s = st.step(steps)
+    t2 = time.perf_counter()
+    per = [st.member_stats(k) for k in range(members)]     # a batched call reports member 0 only
+    bad = [k for k, p in enumerate(per) if p.status != 0]
+    if bad:
+        raise SystemExit(f"ensemble members {bad[:8]} failed (status {per[bad[0]].status})")
+    batched = sum(p.n_vertex_steps for p in per) / (t2 - t1)
+    st.close()
+    vs, el = 0, 0.0
+    for o, m in zip(opts, meshes):
+        one = sevc.State(o, m)
+        one.step(warmup)
+        one.step(0)
+        a = time.perf_counter()
+        r = one.step(b2b_steps)
+        el += time.perf_counter() - a
+        vs += r.n_vertex_steps
+        one.close()
+    b2b = vs / el
+    log(f"ensemble: batched {batched:.3e} vertex-steps/s over {steps} steps; back-to-back {b2b:.3e} "
+        f"over {b2b_steps} steps per member")
+    return {"value": batched, "unit": "vertex-steps/s", "members": members, "grains": grains, "steps": steps,
+            "ms_per_step": (t2 - t1) / max(1, s.steps_done) * 1e3,
+            "back_to_back": {"value": b2b, "steps_per_member": b2b_steps, "seconds": el},
+            "speedup": batched / b2b,
+            "workload": f"{members} members, G log-uniform in [1000, 50000] (Voronoi seed 1000+i), "
+                        "SE = s_i*U[0,1) with s_i ~ U[0,1]; model options of the single-domain workload"}
+
+
 def cpu_baseline(opts, mesh, steps):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from _util import Oracle  # the oracle is the CPU comparator
@@ -107,6 +165,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the step as a HIP graph (no per-kernel timing)")
     ap.add_argument("--graph-steps", type=int, default=200, help="extra steps timed as HIP graph replays (0: skip)")
+    ap.add_argument("--members", type=int, default=256, help="ensemble members (0: skip the ensemble line)")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--b2b-steps", type=int, default=50, help="steps per member when stepped back to back")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM ceiling and PMC byte-counter scale)")
     args = ap.parse_args()
@@ -187,6 +248,8 @@ def main():
                                 "ms_per_step": (g1 - g0) / max(1, sg.steps_done) * 1e3,
                                 "steps": sg.steps_done}
     st.close()
+    if args.members > 0:
+        line["ensemble"] = bench_ensemble(sevc, args.members, args.ensemble_steps, args.b2b_steps)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -190,7 +190,9 @@ class State:
                                             float(mobility)), self.h)
 
     def enable_timing(self, on=True):
-        _check(lib().sevc_enable_kernel_timing(self.h, 1 if on else 0), self.h)
+        """0/False off, 1/True every kernel, 2 only the velocity passes,
+        translation and t_ext (HIP events on the state's stream)."""
+        _check(lib().sevc_enable_kernel_timing(self.h, int(on)), self.h)
 
     def kernel_times(self):
         n = 64
