@@ -253,6 +253,30 @@ int sevc_get_mesh(sevc_state* s, int32_t member, sevc_mesh* out);
 int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
                            const double* g_SE, double gb_scaling, double mobility);
 
+/* Gradients of one member from sevc_adjoint_window (extension). */
+typedef struct sevc_grads {
+    double* d_pos;           /* 2 per vertex slot: dL/dx at the window's start (NULL: not wanted) */
+    double* d_SE;            /* 1 per grain slot: dL/dSE_g (NULL: not wanted) */
+    double  d_gb_scaling;    /* dL/dGB_scaling */
+    double  d_mobility;      /* dL/dmobility */
+} sevc_grads;
+
+/* Forward over a topology-fixed window (extension; no counterpart in the
+ * reference, whose step is calculus.h:143-154 + :391-404 inside
+ * main.cu:124-389): `steps` steps of x <- dom_mod(x + dt*mobility*F(x)),
+ * F = -grad E the analytic force, with T1/T2/nucleation off and dt the
+ * fixed opts.dt, on every member.  Positions are checkpointed every
+ * ckpt_every steps (<= 0: every step is kept, no recomputation later).
+ * The state ends at the window's end; the next sevc_step or
+ * sevc_set_member_params invalidates the window. */
+int sevc_window_forward(sevc_state* s, int32_t steps, int32_t ckpt_every);
+
+/* Reverse pass over the last window: dL_dpos[m] = dL/dx at the window's
+ * end for member m (2 per vertex slot, host memory); fills grads[m].
+ * Segments are recomputed from their checkpoints; results are
+ * deterministic (fixed-order sums). */
+int sevc_adjoint_window(sevc_state* s, const double* const* dL_dpos, sevc_grads* grads);
+
 /* Compute the diagnostics of the next step (vertex energies,
  * dA/dt and velocity parts, calculus.h:123-133,243-253,62-69). */
 int sevc_request_diagnostics(sevc_state* s);

@@ -62,6 +62,10 @@ struct orc {
     int *voted, *nvotes, *conf;
     uint8_t *gcand, *gfix, *vnuc, *cand_prev;
     double *nucl_r, *nucl_ori, *dE, *nfac;
+    /* topology-fixed window (extension): trajectory x_0..x_N, grain areas
+     * at x_0..x_{N-1}, d gamma_b / d GB_scaling */
+    int wN;
+    double *wtraj, *warea, *webase;
     /* loop state (main.cu:30-113) */
     int n;
     double t, dt_last;
@@ -1359,6 +1363,7 @@ void orc_destroy(orc* s) {
     free(s->voted); free(s->nvotes); free(s->conf); free(s->gcand); free(s->gfix);
     free(s->vnuc); free(s->cand_prev);
     free(s->nucl_r); free(s->nucl_ori); free(s->dE); free(s->nfac);
+    free(s->wtraj); free(s->warea); free(s->webase);
     free(s);
 }
 
@@ -1423,5 +1428,156 @@ int orc_get_mesh(const orc* s, sevc_mesh* m) {
     if (m->g_dAdt) COPY(m->g_dAdt, s->gdadt, s->ng);
     COPY(m->g_orientation, s->gori, s->ng); COPY(m->g_SE, s->gse, s->ng);
     COPY(m->g_enabled, s->genab, s->ng); COPY(m->g_type, s->gtype, s->ng);
+    return SEVC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* topology-fixed window and its adjoint (extension)                    */
+/* ------------------------------------------------------------------ */
+
+/* Forward over a window of `steps` steps with the topology held fixed:
+ * the analytic model v = mobility * F(x), F = -grad E, explicit Euler with
+ * the fixed dt and the wrap of update_vertices_positions
+ * (calculus.h:391-404); no events, no nucleation.  Boundary energies are
+ * evaluated once (they depend on orientations only).  Each step is the
+ * analytic-model step of orc_step with events off: areas, velocities,
+ * translation.  The trajectory is kept for the reverse pass. */
+int orc_window_forward(orc* s, int32_t steps) {
+    if (steps < 0) return SEVC_E_INVALID_ARG;
+    free(s->wtraj); free(s->warea); free(s->webase);
+    const size_t n2 = 2 * (size_t)s->nv;
+    s->wtraj = (double*)malloc(sizeof(double) * n2 * ((size_t)steps + 1));
+    s->warea = (double*)malloc(sizeof(double) * (size_t)s->ng * ((size_t)steps + 1));
+    s->webase = (double*)calloc((size_t)s->ne + 1, sizeof(double));
+    if (!s->wtraj || !s->warea || !s->webase) return SEVC_E_NOMEM;
+    s->wN = steps;
+    compute_boundary_energies(s);
+    for (int b = 0; b < s->ne; b++)
+        if (s->benab[b]) s->webase[b] = boundary_compute_energy(s, b, s->o.GB_eps);
+    memcpy(s->wtraj, s->pos, sizeof(double) * n2);
+    const double mu = s->o.mobility, dt = s->o.dt;
+    for (int n = 0; n < steps; n++) {
+        compute_grain_areas(s);
+        memcpy(s->warea + (size_t)n * s->ng, s->garea, sizeof(double) * (size_t)s->ng);
+        for (int v = 0; v < s->nv; v++) {
+            if (!s->ven[v]) continue;
+            double fx, fy;
+            analytic_force(s, v, &fx, &fy);
+            VX(s, v) = fx * mu;
+            VY(s, v) = fy * mu;
+        }
+        update_vertices_positions(s, dt);
+        memcpy(s->wtraj + (size_t)(n + 1) * n2, s->pos, sizeof(double) * n2);
+        s->n += 1;
+        if (s->o.adaptive_dt) s->t += dt;
+        else s->t = s->n * dt;
+    }
+    return SEVC_OK;
+}
+
+/* One reverse step at vertex v: with lambda = dL/dx_{n+1} and the state
+ * x_n (positions X, grain areas A),
+ *   lambda_n = lambda - dt mu H lambda          (H = Hessian of E at x_n)
+ *   acc_mu  += dt (lambda . F)
+ *   acc_s   += dt mu (lambda . dF/dGB_scaling),  dF/ds = -sum_b e_b u_b
+ *   acc_se_j += dt mu (lambda . dF/dSE_g),       dF/dSE_g = -sign(A_g) dA_g/dp_v
+ * Edge b (other end w, d = wrap(p_v - p_w), L = |d|, u = d/L) adds
+ * (gamma_b/L)(I - u u^T)(lambda_v - lambda_w) to (H lambda)_v; grain g
+ * (list neighbours prev, next) adds
+ * SE_g sign(A_g)/2 (lambda_next.y - lambda_prev.y, lambda_prev.x - lambda_next.x). */
+static void adjoint_vertex(const orc* s, const double* X, const double* A, const double* lam, double* lam2,
+                           double* acc_mu, double* acc_s, double* acc_se, int v) {
+    const double D = s->D, mu = s->o.mobility, dt = s->o.dt, dtm = dt * mu;
+    const double x = X[2 * v], y = X[2 * v + 1];
+    const double lx = lam[2 * v], ly = lam[2 * v + 1];
+    double ex = 0.0, ey = 0.0, gx = 0.0, gy = 0.0;
+    double hex = 0.0, hey = 0.0, hgx = 0.0, hgy = 0.0, sx = 0.0, sy = 0.0;
+    for (int j = 0; j < 3; j++) {
+        int b = VB(s, v, j);
+        int w = (BI(s, b) == v) ? BE(s, b) : BI(s, b);
+        double dx = orc_wrap_dist(x, X[2 * w], D);
+        double dy = orc_wrap_dist(y, X[2 * w + 1], D);
+        double L = orc_mag(dx, dy);
+        if (L > 0) {
+            double ux = dx / L, uy = dy / L;
+            ex += s->ben[b] * ux;
+            ey += s->ben[b] * uy;
+            double wx = lx - lam[2 * w], wy = ly - lam[2 * w + 1];
+            double uw = ux * wx + uy * wy;
+            double c = s->ben[b] / L;
+            hex += c * (wx - ux * uw);
+            hey += c * (wy - uy * uw);
+            sx += s->webase[b] * ux;
+            sy += s->webase[b] * uy;
+        }
+    }
+    for (int j = 0; j < 3; j++) {
+        int g = VG(s, v, j);
+        int n = s->glen[g];
+        int k = grain_contains_vertex(s, g, v);
+        if (k < 0 || n < 3) continue;
+        int wp = GV(s, g, (k + n - 1) % n), wn = GV(s, g, (k + 1) % n);
+        double dpx = orc_wrap_dist(x, X[2 * wp], D), dpy = orc_wrap_dist(y, X[2 * wp + 1], D);
+        double dnx = orc_wrap_dist(X[2 * wn], x, D), dny = orc_wrap_dist(X[2 * wn + 1], y, D);
+        double sg = A[g] < 0 ? -1.0 : 1.0;
+        double c = s->gse[g] * sg * 0.5;
+        gx += c * (dpy + dny);
+        gy += c * (-(dpx + dnx));
+        hgx += c * (lam[2 * wn + 1] - lam[2 * wp + 1]);
+        hgy += c * (lam[2 * wp] - lam[2 * wn]);
+        double q = sg * 0.5;
+        double tse = -(lx * (q * (dpy + dny)) + ly * (q * (-(dpx + dnx))));
+        acc_se[3 * v + j] += dtm * tse;
+    }
+    double fx = -(ex + gx), fy = -(ey + gy);
+    double hx = hex + hgx, hy = hey + hgy;
+    acc_mu[v] += dt * (lx * fx + ly * fy);
+    acc_s[v] += dtm * (-(lx * sx + ly * sy));
+    lam2[2 * v] = lx - dtm * hx;
+    lam2[2 * v + 1] = ly - dtm * hy;
+}
+
+/* Reverse pass over the last window: dL_dpos = dL/dx_N (2 per vertex
+ * slot).  Outputs dL/dx_0 (d_pos, 2 per vertex slot), dL/dSE (per grain
+ * slot: the grain's list order sum of its vertices' terms), dL/dGB_scaling
+ * and dL/dmobility (orc_sum_chunked over the vertex slots). */
+int orc_adjoint_window(orc* s, const double* dL_dpos, double* d_pos, double* d_SE, double* d_gb, double* d_mu) {
+    if (!s->wtraj) return SEVC_E_STATE;
+    const int nv = s->nv;
+    double* lam = (double*)calloc(2 * (size_t)nv + 2, sizeof(double));
+    double* lam2 = (double*)calloc(2 * (size_t)nv + 2, sizeof(double));
+    double* acc_mu = (double*)calloc((size_t)nv + 1, sizeof(double));
+    double* acc_s = (double*)calloc((size_t)nv + 1, sizeof(double));
+    double* acc_se = (double*)calloc(3 * (size_t)nv + 3, sizeof(double));
+    if (!lam || !lam2 || !acc_mu || !acc_s || !acc_se) {
+        free(lam); free(lam2); free(acc_mu); free(acc_s); free(acc_se);
+        return SEVC_E_NOMEM;
+    }
+    for (int v = 0; v < nv; v++)
+        if (s->ven[v]) { lam[2 * v] = dL_dpos[2 * v]; lam[2 * v + 1] = dL_dpos[2 * v + 1]; }
+    for (int n = s->wN - 1; n >= 0; n--) {
+        const double* X = s->wtraj + (size_t)n * 2 * nv;
+        const double* A = s->warea + (size_t)n * s->ng;
+        for (int v = 0; v < nv; v++) {
+            if (!s->ven[v]) { lam2[2 * v] = 0.0; lam2[2 * v + 1] = 0.0; continue; }
+            adjoint_vertex(s, X, A, lam, lam2, acc_mu, acc_s, acc_se, v);
+        }
+        double* t = lam; lam = lam2; lam2 = t;
+    }
+    if (d_pos) memcpy(d_pos, lam, sizeof(double) * 2 * (size_t)nv);
+    if (d_mu) *d_mu = orc_sum_chunked(acc_mu, s->ven, nv);
+    if (d_gb) *d_gb = orc_sum_chunked(acc_s, s->ven, nv);
+    if (d_SE)
+        for (int g = 0; g < s->ng; g++) {
+            double r = 0.0;
+            if (s->genab[g])
+                for (int k = 0; k < s->glen[g]; k++) {
+                    int v = GV(s, g, k);
+                    for (int j = 0; j < 3; j++)
+                        if (VG(s, v, j) == g) { r += acc_se[3 * v + j]; break; }
+                }
+            d_SE[g] = r;
+        }
+    free(lam); free(lam2); free(acc_mu); free(acc_s); free(acc_se);
     return SEVC_OK;
 }

@@ -40,6 +40,11 @@ int orc_compute_velocities(orc* s);
  * on the next step. */
 void orc_request_diagnostics(orc* s);
 
+/* Topology-fixed window (extension): `steps` steps of the analytic model
+ * with events and nucleation off and dt fixed; the trajectory is kept. */
+int orc_window_forward(orc* s, int32_t steps);
+/* Reverse pass over the last window (see sevc_adjoint_window). */
+int orc_adjoint_window(orc* s, const double* dL_dpos, double* d_pos, double* d_SE, double* d_gb, double* d_mu);
 /* Loader restatement of io.h:66-264 (independent of the product loader). */
 int orc_load_files(const sevc_options* o, sevc_mesh* out);
 int orc_build_mesh(const sevc_options* o, int32_t nv, const double* unit_pos, int32_t nr,

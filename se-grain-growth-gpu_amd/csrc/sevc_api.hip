@@ -54,6 +54,17 @@ __global__ void k_rate_cand(Dev d);
 __global__ void k_block_counts(Dev d, int kind);
 __global__ void k_nucl_manager(Dev d);
 __global__ void k_nucl_refresh(Dev d);
+// kernels of sevc_adjoint.hip
+__global__ void k_win_prepare(Dev d, double* ebase);
+__global__ void k_win_area(Dev d, const double* X, double* A, int to_state);
+__global__ void k_win_step(Dev d, const double* X, const double* A, double* Xn, int write_vel);
+__global__ void k_win_end(Dev d, int steps);
+__global__ void k_win_init(Dev d, double* lam, double* acc_mu, double* acc_s, double* acc_se);
+__global__ void k_win_adjoint(Dev d, const double* X, const double* A, const double* lam, double* lam2,
+                              double* acc_mu, double* acc_s, double* acc_se, const double* ebase);
+__global__ void k_win_sum_l1(Dev d, const double* acc_mu, const double* acc_s, double* part);
+__global__ void k_win_sum_up(Dev d, double* part, double* out);
+__global__ void k_win_dse(Dev d, const double* acc_se, double* dse);
 }  // namespace sevc
 
 using namespace sevc;
@@ -85,6 +96,17 @@ struct sevc_state {
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::map<std::string, std::pair<double, long long>> ktimes;
+    // topology-fixed window: checkpoints of x every K steps (ck), the
+    // positions and areas of one segment (seg, segA; `holds` = its index),
+    // and the reverse-pass arrays
+    struct Window {
+        int steps = 0, K = 0, nck = 0, holds = -1;
+        bool ready = false;
+        double *ck = nullptr, *seg = nullptr, *segA = nullptr;
+        size_t ck_n = 0, seg_n = 0, segA_n = 0;
+        double *lamA = nullptr, *lamB = nullptr, *acc_mu = nullptr, *acc_s = nullptr, *acc_se = nullptr;
+        double *ebase = nullptr, *part = nullptr, *out = nullptr, *dse = nullptr;
+    } win;
 };
 
 namespace {
@@ -525,6 +547,7 @@ static int finish_call(sevc_state* s) {
 
 int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     if (!s || n_steps < 0) return SEVC_E_INVALID_ARG;
+    if (n_steps > 0) s->win.ready = false;       // the topology may change
     for (int k = 0; k < n_steps; ++k) {
         bool diag = s->diag_next;
         s->diag_next = false;
@@ -543,6 +566,7 @@ int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
 
 int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     if (!s || n_steps < 0) return SEVC_E_INVALID_ARG;
+    if (n_steps > 0) s->win.ready = false;
     if (!s->gexec) {
         bool timing = s->timing;
         s->timing = false;
@@ -640,6 +664,149 @@ int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const do
     HIPCHK(hipMemcpy(const_cast<Params*>(d.prm) + m, &P, sizeof(Params), hipMemcpyHostToDevice));
     if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->win.ready = false;
+    return SEVC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// topology-fixed windows and their adjoint
+// ---------------------------------------------------------------------
+
+namespace {
+
+// (Re)allocate a window buffer of at least n doubles.
+int win_buffer(sevc_state* s, double** p, size_t* have, size_t n) {
+    if (*p && *have >= n) return SEVC_OK;
+    if (*p) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(*p);
+        s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(*p)));
+        *p = nullptr;
+    }
+    *have = 0;
+    int rc = dalloc(s, p, n);
+    if (rc == SEVC_OK) *have = n;
+    return rc;
+}
+
+// Steps x_{c K} .. of segment c from the segment buffer's slot 0; areas of
+// each step into segA.  Forward of the window (state outputs on) or its
+// recomputation in the reverse pass (off).
+void win_run_segment(sevc_state* s, int len, bool forward) {
+    const Dev& d = s->d;
+    auto& w = s->win;
+    const size_t n2 = 2 * static_cast<size_t>(s->V);
+    for (int i = 0; i < len; ++i) {
+        const double* X = w.seg + i * n2;
+        double* A = w.segA + static_cast<size_t>(i) * s->G;
+        launch(s, "win_area", k_win_area, d.ngb, kBlock, d, X, A, forward ? 1 : 0);
+        launch(s, "win_step", k_win_step, d.nvb, kBlock, d, X, static_cast<const double*>(A),
+               w.seg + (i + 1) * n2, forward ? 1 : 0);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sevc_window_forward(sevc_state* s, int32_t steps, int32_t ckpt_every) {
+    if (!s || steps < 0) return SEVC_E_INVALID_ARG;
+    auto& w = s->win;
+    const Dev& d = s->d;
+    const int K = (ckpt_every <= 0 || ckpt_every > steps) ? std::max(steps, 1) : ckpt_every;
+    const int nck = (steps + K - 1) / K;
+    const size_t V = s->V, n2 = 2 * V;
+    int rc;
+    if ((rc = win_buffer(s, &w.ck, &w.ck_n, std::max<size_t>(1, nck) * n2)) != SEVC_OK ||
+        (rc = win_buffer(s, &w.seg, &w.seg_n, (static_cast<size_t>(K) + 1) * n2)) != SEVC_OK ||
+        (rc = win_buffer(s, &w.segA, &w.segA_n, static_cast<size_t>(K) * s->G)) != SEVC_OK)
+        return rc;
+    if (!w.lamA) {
+        if ((rc = dalloc(s, &w.lamA, n2)) || (rc = dalloc(s, &w.lamB, n2)) || (rc = dalloc(s, &w.acc_mu, V)) ||
+            (rc = dalloc(s, &w.acc_s, V)) || (rc = dalloc(s, &w.acc_se, 3 * V)) ||
+            (rc = dalloc(s, &w.ebase, static_cast<size_t>(s->E))) ||
+            (rc = dalloc(s, &w.part, 4 * static_cast<size_t>(d.nvb))) ||
+            (rc = dalloc(s, &w.out, 2 * static_cast<size_t>(s->B))) ||
+            (rc = dalloc(s, &w.dse, static_cast<size_t>(s->G))))
+            return rc;
+    }
+    w.ready = false;
+    launch(s, "win_prepare", k_win_prepare, d.neb, kBlock, d, w.ebase);
+    HIPCHK(hipMemcpyAsync(w.seg, d.pos, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
+    int last = 0;
+    for (int c = 0; c < nck; ++c) {
+        const int len = std::min(K, steps - c * K);
+        if (c > 0) HIPCHK(hipMemcpyAsync(w.seg, w.seg + last * n2, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(w.ck + c * n2, w.seg, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
+        win_run_segment(s, len, true);
+        last = len;
+    }
+    if (steps > 0) HIPCHK(hipMemcpyAsync(d.pos, w.seg + last * n2, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
+    launch(s, "win_end", k_win_end, (s->B + 63) / 64, 64, d, static_cast<int>(steps));
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(s, SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
+    if ((rc = finish_call(s)) != SEVC_OK) return rc;
+    w.steps = steps;
+    w.K = K;
+    w.nck = nck;
+    w.holds = nck - 1;
+    w.ready = true;
+    return SEVC_OK;
+}
+
+int sevc_adjoint_window(sevc_state* s, const double* const* dL_dpos, sevc_grads* grads) {
+    if (!s || !dL_dpos || !grads) return SEVC_E_INVALID_ARG;
+    auto& w = s->win;
+    if (!w.ready) return fail(s, SEVC_E_STATE, "no window: call sevc_window_forward first (after any step)");
+    const Dev& d = s->d;
+    const size_t n2 = 2 * static_cast<size_t>(s->V);
+    for (int m = 0; m < s->B; ++m) {
+        if (!dL_dpos[m]) return fail(s, SEVC_E_INVALID_ARG, "dL_dpos missing for a member");
+        HIPCHK(hipMemcpyAsync(w.lamA + 2 * static_cast<size_t>(s->voff[m]), dL_dpos[m],
+                              16 * static_cast<size_t>(s->nv[m]), hipMemcpyHostToDevice, s->stream));
+    }
+    launch(s, "win_init", k_win_init, d.nvb, kBlock, d, w.lamA, w.acc_mu, w.acc_s, w.acc_se);
+    double *cur = w.lamA, *nxt = w.lamB;
+    for (int c = w.nck - 1; c >= 0; --c) {
+        const int len = std::min(w.K, w.steps - c * w.K);
+        if (w.holds != c) {
+            HIPCHK(hipMemcpyAsync(w.seg, w.ck + c * n2, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
+            win_run_segment(s, len, false);
+            w.holds = c;
+        }
+        for (int i = len - 1; i >= 0; --i) {
+            launch(s, "win_adjoint", k_win_adjoint, d.nvb, kBlock, d, static_cast<const double*>(w.seg + i * n2),
+                   static_cast<const double*>(w.segA + static_cast<size_t>(i) * s->G),
+                   static_cast<const double*>(cur), nxt, w.acc_mu, w.acc_s, w.acc_se,
+                   static_cast<const double*>(w.ebase));
+            std::swap(cur, nxt);
+        }
+    }
+    launch(s, "win_sum_l1", k_win_sum_l1, (d.nvb + 255) / 256, 256, d, static_cast<const double*>(w.acc_mu),
+           static_cast<const double*>(w.acc_s), w.part);
+    if (s->B > 0)
+        hipLaunchKernelGGL(k_win_sum_up, dim3(s->B, 2), dim3(256), 0, s->stream, d, w.part, w.out);
+    launch(s, "win_dse", k_win_dse, d.ngb, kBlock, d, static_cast<const double*>(w.acc_se), w.dse);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(s, SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
+    std::vector<double> out(2 * static_cast<size_t>(s->B));
+    HIPCHK(hipMemcpyAsync(out.data(), w.out, out.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    for (int m = 0; m < s->B; ++m) {
+        if (grads[m].d_pos)
+            HIPCHK(hipMemcpyAsync(grads[m].d_pos, cur + 2 * static_cast<size_t>(s->voff[m]),
+                                  16 * static_cast<size_t>(s->nv[m]), hipMemcpyDeviceToHost, s->stream));
+        if (grads[m].d_SE)
+            HIPCHK(hipMemcpyAsync(grads[m].d_SE, w.dse + s->goff[m], 8 * static_cast<size_t>(s->ng[m]),
+                                  hipMemcpyDeviceToHost, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->timing) collect_timers(s);
+    for (int m = 0; m < s->B; ++m) {
+        grads[m].d_gb_scaling = out[2 * static_cast<size_t>(m)];
+        grads[m].d_mobility = out[2 * static_cast<size_t>(m) + 1];
+    }
     return SEVC_OK;
 }
 

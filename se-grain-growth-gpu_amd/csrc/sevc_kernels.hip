@@ -17,166 +17,9 @@
 #include "sevc_geom.h"
 #include "../../include/sevc.h"
 #include "../../include/sevc_math.h"
+#include "sevc_dev.h"
 
 namespace sevc {
-
-enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17
-enum { T_COMMON = 0, T_NUCLEATED = 1 };                                  // geometry.h:19
-constexpr unsigned kStreamMcSite = 1u, kStreamMcPick = 2u, kStreamRate = 3u;
-constexpr int kMaxStride = 128;   // largest grain-list stride supported
-
-// ---------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------
-
-// Map the hardware workgroup id so that each of the 8 XCDs (workgroups are
-// dealt round-robin over XCDs) processes one contiguous slice of the table:
-// neighbouring slots then share an L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + (x < r ? x : r) + (b >> 3);
-}
-
-__device__ __forceinline__ void set_error(const Dev& d, int code) {
-    atomicCAS(&d.cnt[C_ERR], 0, code);
-}
-
-__device__ __forceinline__ int other_end(const Dev& d, int b, int v) {
-    return d.bvx[2 * b] == v ? d.bvx[2 * b + 1] : d.bvx[2 * b];
-}
-
-// geometry.h:837-848
-__device__ __forceinline__ int grains_intersect(const Dev& d, int a, int b, int out[2]) {
-    int k = 0;
-    out[0] = out[1] = -1;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            if (d.vg[3 * a + i] == d.vg[3 * b + j]) {
-                out[k++] = d.vg[3 * a + i];
-                if (k == 2) return 2;
-            }
-    return k;
-}
-
-// geometry.h:886-898 (with grains_union, :863-875)
-__device__ __forceinline__ void grains_symdiff(const Dev& d, int a, int b, int out[2]) {
-    int gic[2], gun[4];
-    out[0] = out[1] = -1;
-    grains_intersect(d, a, b, gic);
-    gun[3] = -1;
-    for (int i = 0; i < 3; i++) gun[i] = d.vg[3 * a + i];
-    for (int i = 0; i < 3; i++) {
-        int g = d.vg[3 * b + i];
-        if (g != gun[0] && g != gun[1] && g != gun[2]) { gun[3] = g; break; }
-    }
-    int k = 0;
-    for (int i = 0; i < 4; i++)
-        if (gun[i] != gic[0] && gun[i] != gic[1]) {
-            out[k++] = gun[i];
-            if (k == 2) return;
-        }
-}
-
-__device__ __forceinline__ int* glist(const Dev& d, int g) { return d.gvx + (size_t)g * d.S; }
-
-// geometry.h:703-710
-__device__ __forceinline__ int grain_contains_vertex(const Dev& d, int g, int v) {
-    const int* L = glist(d, g);
-    int n = d.glen[g];
-    for (int i = 0; i < n; i++) if (L[i] == v) return i;
-    return -1;
-}
-
-// geometry.h:724-734
-__device__ __forceinline__ int grain_contains_consecutive(const Dev& d, int g, int a, int b) {
-    const int* L = glist(d, g);
-    int n = d.glen[g];
-    for (int i = 0; i < n; i++) {
-        int nx = L[(i + 1) % n];
-        if (L[i] == a && nx == b) return 1;
-        if (L[i] == b && nx == a) return 2;
-    }
-    return -1;
-}
-
-// geometry.h:548-550, pow(c,3) as c*c*c
-__device__ __forceinline__ double gb_energy_func(double d_alpha, double eps) {
-    double c = sevc_cos(4 * d_alpha);
-    return (1 + 0.5 * eps * (1 - c * c * c));
-}
-
-// geometry.h:559-564
-__device__ __forceinline__ double boundary_compute_energy(const Dev& d, int b, double eps) {
-    int gic[2];
-    if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) return 1.0;
-    return gb_energy_func(d.gori[gic[0]] - d.gori[gic[1]], eps);
-}
-
-// geometry.h:513-517 plus the flip-threshold extension
-__device__ __forceinline__ bool is_candidate(double te, double len, double dt, double thr) {
-    if (te >= 0 && te <= dt) return true;
-    // a zero-length edge (just flipped, t_ext = no-motion value) is not shrinking
-    if (thr > 0 && te >= 0 && len > 0 && len < thr) return true;
-    return false;
-}
-
-__device__ __forceinline__ double member_dt(const Params& P, const MState& M) {
-    if (!P.adaptive) return P.dt_max;
-    unsigned long long bits = M.tau_bits;
-    if (bits == 0x7FF0000000000000ull) return P.dt_max;
-    double tau = __longlong_as_double((long long)bits);
-    double dt = P.dt_safety * tau;
-    if (dt < P.dt_min) dt = P.dt_min;
-    if (dt > P.dt_max) dt = P.dt_max;
-    return dt;
-}
-
-__device__ __forceinline__ double grain_area(const Dev& d, int g, double D) {
-    int n = d.glen[g];
-    if (n <= 0) return 0.0;
-    const int* L = glist(d, g);
-    AreaAcc acc;
-    int w = L[0];
-    acc.start(d.pos[2 * w], d.pos[2 * w + 1]);
-    for (int k = 1; k < n; k++) {
-        w = L[k];
-        acc.push(d.pos[2 * w], d.pos[2 * w + 1], D);
-    }
-    return acc.finish();
-}
-
-// geometry.h:369-376
-__device__ __forceinline__ void reset_mods(const Dev& d, int v) {
-    for (int j = 0; j < 4; j++) {
-        d.mfix[4 * v + j] = FIX_NONE;
-        d.mgrn[4 * v + j] = -1;
-        d.mvrt[4 * v + j] = -1;
-        d.mprev[4 * v + j] = -1;
-    }
-}
-
-__device__ __forceinline__ void set_mod(const Dev& d, int v, int slot, int fix, int grn, int vrt) {
-    d.mfix[4 * v + slot] = (uint8_t)fix;
-    d.mgrn[4 * v + slot] = grn;
-    d.mvrt[4 * v + slot] = vrt;
-}
-
-// Flag grain g for the fix stage `stage` (grain.fix = true) and list it once.
-__device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
-    if (atomicExch(&d.gfix[g], 1) == 0) {
-        int i = atomicAdd(&d.cnt[C_FIX0 + stage], 1);
-        d.l_fix[stage][i] = g;
-    }
-}
-
-#define TABLE_BLOCK(tbl, nblk)                                   \
-    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);        \
-    if (lb_ >= (nblk)) return;                                   \
-    const Blk bd_ = (tbl)[lb_];                                  \
-    const Params& P = d.prm[bd_.m];                              \
-    MState& M = d.ms[bd_.m];                                     \
-    (void)P; (void)M;                                            \
-    const int i = bd_.b + (int)threadIdx.x;
 
 // ---------------------------------------------------------------------
 // step begin / end
@@ -381,37 +224,9 @@ __device__ __forceinline__ double energy_diff(const Dev& d, const Params& P, int
     return bnd_term + grn_term;
 }
 
-// Analytic -grad E (extension): see the oracle for the formula.
+// Analytic -grad E at the current positions and grain areas.
 __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, int v, double* fx, double* fy) {
-    const double D = P.D;
-    double x = d.pos[2 * v], y = d.pos[2 * v + 1];
-    double ex = 0.0, ey = 0.0, gx = 0.0, gy = 0.0;
-    for (int j = 0; j < 3; j++) {
-        int b = d.vb[3 * v + j];
-        int nb = other_end(d, b, v);
-        double dx = wrapd(x, d.pos[2 * nb], D), dy = wrapd(y, d.pos[2 * nb + 1], D);
-        double L = mag(dx, dy);
-        if (L > 0) {
-            ex += d.ben[b] * (dx / L);
-            ey += d.ben[b] * (dy / L);
-        }
-    }
-    for (int j = 0; j < 3; j++) {
-        int g = d.vg[3 * v + j];
-        int n = d.glen[g];
-        int k = grain_contains_vertex(d, g, v);
-        if (k < 0 || n < 3) continue;
-        const int* L = glist(d, g);
-        int wp = L[(k + n - 1) % n], wn = L[(k + 1) % n];
-        double dpx = wrapd(x, d.pos[2 * wp], D), dpy = wrapd(y, d.pos[2 * wp + 1], D);
-        double dnx = wrapd(d.pos[2 * wn], x, D), dny = wrapd(d.pos[2 * wn + 1], y, D);
-        double sg = d.garea[g] < 0 ? -1.0 : 1.0;
-        double c = d.gse[g] * sg * 0.5;
-        gx += c * (dpy + dny);
-        gy += c * (-(dpx + dnx));
-    }
-    *fx = -(ex + gx);
-    *fy = -(ey + gy);
+    analytic_force_at(d, d.pos, d.garea, P, v, fx, fy);
 }
 
 // Velocity, pass 1 of 2 (stored-energy part of energy_diff,

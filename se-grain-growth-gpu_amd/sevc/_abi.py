@@ -78,6 +78,11 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Grads(C.Structure):
+    _fields_ = [("d_pos", C.POINTER(C.c_double)), ("d_SE", C.POINTER(C.c_double)),
+                ("d_gb_scaling", C.c_double), ("d_mobility", C.c_double)]
+
+
 # (field, dtype, per-entity multiplicity, entity) of every Mesh array
 MESH_ARRAYS = [
     ("pos", np.float64, 2, "v"), ("vel", np.float64, 2, "v"),

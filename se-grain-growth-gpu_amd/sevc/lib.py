@@ -57,6 +57,8 @@ def lib():
         "sevc_set_member_params": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                              C.c_double, C.c_double]),
         "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
+        "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "sevc_adjoint_window": (C.c_int, [C.c_void_p, P(P(C.c_double)), P(_abi.Grads)]),
         "sevc_enable_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
         "sevc_kernel_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double), P(C.c_int64)]),
         "sevc_destroy": (None, [C.c_void_p]),
@@ -188,6 +190,34 @@ class State:
         _check(lib().sevc_set_member_params(self.h, int(m), None if pp is None else pp.ctypes.data_as(P),
                                             None if sp is None else sp.ctypes.data_as(P), float(gb_scaling),
                                             float(mobility)), self.h)
+
+    def window_forward(self, steps, ckpt_every=0):
+        """Topology-fixed window forward (analytic model, events off, fixed
+        dt) on every member; checkpoints every ckpt_every steps (0: all)."""
+        _check(lib().sevc_window_forward(self.h, int(steps), int(ckpt_every)), self.h)
+
+    def adjoint(self, dL_dpos):
+        """Reverse pass over the last window.  dL_dpos: one array per member
+        (2 per vertex slot) or a single array for a one-member state.
+        Returns, per member, (d_pos, d_SE, d_gb_scaling, d_mobility)."""
+        if isinstance(dL_dpos, np.ndarray):
+            dL_dpos = [dL_dpos]
+        P = C.POINTER(C.c_double)
+        ins = [np.ascontiguousarray(g, dtype=np.float64).reshape(-1) for g in dL_dpos]
+        outs = []
+        arr_in = (P * self.n_members)()
+        arr_g = (_abi.Grads * self.n_members)()
+        for m in range(self.n_members):
+            nv, ne, ng, S = self.sizes(m)
+            if ins[m].size != 2 * nv:
+                raise ValueError(f"member {m}: dL_dpos has {ins[m].size} values, expected {2 * nv}")
+            dp, ds = np.zeros(2 * nv), np.zeros(ng)
+            outs.append((dp, ds))
+            arr_in[m] = ins[m].ctypes.data_as(P)
+            arr_g[m].d_pos = dp.ctypes.data_as(P)
+            arr_g[m].d_SE = ds.ctypes.data_as(P)
+        _check(lib().sevc_adjoint_window(self.h, arr_in, arr_g), self.h)
+        return [(dp, ds, arr_g[m].d_gb_scaling, arr_g[m].d_mobility) for m, (dp, ds) in enumerate(outs)]
 
     def enable_timing(self, on=True):
         """0/False off, 1/True every kernel, 2 only the velocity passes,

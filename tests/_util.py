@@ -52,6 +52,9 @@ def oracle_lib():
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [C.c_double]
         L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        L.orc_window_forward.argtypes = [C.c_void_p, C.c_int32]
+        L.orc_adjoint_window.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
+                                         P(C.c_double), P(C.c_double)]
         _orc = L
     return _orc
 
@@ -76,6 +79,22 @@ class Oracle:
 
     def request_diagnostics(self):
         self.L.orc_request_diagnostics(self.h)
+
+    def window_forward(self, steps):
+        return self.L.orc_window_forward(self.h, int(steps))
+
+    def adjoint(self, dL_dpos):
+        """Reverse pass over the last window: (d_pos, d_SE, d_gb_scaling, d_mobility)."""
+        t = self.template
+        P = C.POINTER(C.c_double)
+        g = np.ascontiguousarray(dL_dpos, dtype=np.float64).reshape(-1)
+        d_pos = np.zeros(2 * t.nv)
+        d_se = np.zeros(t.ng)
+        gb, mu = C.c_double(), C.c_double()
+        rc = self.L.orc_adjoint_window(self.h, g.ctypes.data_as(P), d_pos.ctypes.data_as(P),
+                                       d_se.ctypes.data_as(P), C.byref(gb), C.byref(mu))
+        assert rc == 0, rc
+        return d_pos, d_se, gb.value, mu.value
 
     def get_mesh(self):
         t = self.template
