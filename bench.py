@@ -124,6 +124,177 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
                         "SE = s_i*U[0,1) with s_i ~ U[0,1]; model options of the single-domain workload"}
 
 
+def window_options(sevc, n_grains, dt):
+    o = sevc.default_options()
+    o.GB_eps = 0.0
+    o.GB_scaling = 1.0
+    o.mobility = 1.0
+    o.dt = dt
+    o.n_grains = n_grains
+    o.velocity_model = sevc.VEL_ANALYTIC
+    o.events = 0
+    o.do_nucleation = 0
+    o.leave_percent = -1.0
+    return o
+
+
+def bench_adjoint(sevc, grains, seed, steps, ckpt, dt, n_fd=20, reps=3):
+    """Adjoint over a topology-fixed window: a smooth 20-bin histogram of
+    grain areas after `steps` steps, against the histogram the same window
+    gives with (1.1 gamma, 0.9 mu).  Times the window forward (with its
+    checkpoints) and the reverse pass (with the segment recomputation), and
+    checks the reverse pass against differences of the forward."""
+    from sevc.calib import AreaHistogram, grain_areas
+    pos, rid = sevc.voronoi(grains, seed)
+    rng = np.random.default_rng(seed)
+    o = window_options(sevc, grains, dt)
+    mesh = sevc.build_mesh(o, pos, rid, rng.random(grains), 0.3 * rng.random(grains), grains)
+    am = float(np.abs(grain_areas(mesh)).mean())
+    hist = AreaHistogram(20, 0.0, 3.0 * am, 0.1 * am)
+    st = sevc.State(o, mesh)
+    st.set_params(0, gb_scaling=1.1, mobility=0.9)
+    st.window_forward(steps, ckpt)
+    target = hist(mesh, st.positions(0))
+    x0 = mesh["pos"].copy()
+
+    def forward(pos=x0, se=None, gb=1.0, mu=1.0):
+        st.set_params(0, pos=pos, g_SE=mesh["g_SE"] if se is None else se, gb_scaling=gb, mobility=mu)
+        st.window_forward(steps, ckpt)
+        return hist.loss_and_grad(mesh, st.positions(0), target)
+
+    L, gN = forward()
+    t_f, t_a = [], []
+    for _ in range(reps):
+        st.set_params(0, pos=x0, g_SE=mesh["g_SE"], gb_scaling=1.0, mobility=1.0)
+        a = time.perf_counter()
+        st.window_forward(steps, ckpt)
+        b = time.perf_counter()
+        (dp, dse, dgb, dmu), = st.adjoint(gN)
+        c = time.perf_counter()
+        t_f.append(b - a)
+        t_a.append(c - b)
+    # accuracy: a seeded linear functional w . x_N, whose reverse pass is
+    # compared with Richardson central differences of the window forward on
+    # n_fd - 2 seeded position coordinates (h = 1e-6), on GB_scaling and
+    # mobility (h = 1e-4), and along one seeded direction in SE (h = 1e-4;
+    # a single SE_g moves x_N by less than the rounding of a 20k-term sum)
+    pick = np.random.default_rng(seed + 1)
+    w = pick.standard_normal(2 * mesh.nv) * np.repeat(mesh["v_enabled"], 2)
+    u = pick.standard_normal(mesh.ng) * mesh["g_enabled"]
+    st.set_params(0, pos=x0, g_SE=mesh["g_SE"], gb_scaling=1.0, mobility=1.0)
+    st.window_forward(steps, ckpt)
+    xN = st.positions(0).reshape(-1).copy()
+    (lp, ls, lgb, lmu), = st.adjoint(w)
+    D = mesh.D
+
+    def lin(p, s, gbv, muv):
+        st.set_params(0, pos=p, g_SE=s, gb_scaling=gbv, mobility=muv)
+        st.window_forward(steps, ckpt)
+        d = st.positions(0).reshape(-1) - xN
+        return float(np.dot(w, d - D * np.round(d / D)))
+
+    checks = [("pos", int(c), 1e-6) for c in pick.choice(np.nonzero(np.repeat(mesh["v_enabled"], 2))[0],
+                                                          n_fd - 2, replace=False)]
+    checks += [("gb", 0, 1e-4), ("mu", 0, 1e-4), ("SE-dir", 0, 1e-4)]
+    worst, detail = 0.0, []
+    for kind, i, h in checks:
+        def f(e):
+            p, s, gbv, muv = x0.copy(), mesh["g_SE"].copy(), 1.0, 1.0
+            if kind == "pos":
+                p[i] += e
+            elif kind == "SE-dir":
+                s = s + e * u
+            elif kind == "gb":
+                gbv += e
+            else:
+                muv += e
+            return lin(p, s, gbv, muv)
+        c1 = (f(h) - f(-h)) / (2 * h)
+        c2 = (f(h / 2) - f(-h / 2)) / h
+        fd = (4 * c2 - c1) / 3
+        ad = {"pos": lambda: lp[i], "SE-dir": lambda: float(np.dot(u, ls)), "gb": lambda: lgb,
+              "mu": lambda: lmu}[kind]()
+        rel = abs(ad - fd) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        detail.append([kind, i, ad, fd, rel])
+    coords = checks
+    st.close()
+    tf, ta = min(t_f), min(t_a)
+    log(f"adjoint: {grains} grains, {steps} steps (checkpoint every {ckpt}): forward {tf * 1e3:.1f} ms, "
+        f"reverse {ta * 1e3:.1f} ms, ratio {ta / tf:.2f}; worst FD rel. error {worst:.2e}")
+    return {"ratio": ta / tf, "forward_ms": tf * 1e3, "adjoint_ms": ta * 1e3, "grains": grains, "steps": steps,
+            "ckpt_every": ckpt, "dt": dt, "loss": L, "fd_max_rel_err": worst, "fd_coords": len(coords),
+            "fd_checks": detail,
+            "workload": f"periodic Voronoi of {grains} seeds (seed {seed}), SE = 0.3*U[0,1), 20-bin smooth "
+                        "area histogram vs the window's histogram at (1.1 gamma, 0.9 mu)"}
+
+
+def bench_calibration(sevc, members, iters, steps, ckpt, dt, seed0=2000):
+    """Batched calibration: members with G ~ U[5000, 20000] grains and
+    lognormal SE; each member's target is its own 20-bin area histogram
+    after the window at (gamma, mu) = (1.1, 0.9).  Starting from (1, 1),
+    Adam on (log gamma, log mu) with the summed loss; every iteration is
+    one batched window forward, the device histogram loss (its gradient
+    seeds the reverse pass on the device) and one batched reverse pass."""
+    from sevc.calib import grain_areas
+    opts, meshes, ranges = [], [], []
+    t0 = time.time()
+    for i in range(members):
+        rng = np.random.default_rng(seed0 + i)
+        g = int(rng.integers(5000, 20001))
+        pos, rid = sevc.voronoi(g, seed0 + i)
+        o = window_options(sevc, g, dt)
+        se = np.exp(np.log(0.3) - 0.125 + 0.5 * rng.standard_normal(g))
+        m = sevc.build_mesh(o, pos, rid, rng.random(g), se, g)
+        am = float(np.abs(grain_areas(m)).mean())
+        opts.append(o)
+        meshes.append(m)
+        ranges.append((0.0, 3.0 * am, 0.1 * am))
+    grains = sum(o.n_grains for o in opts)
+    log(f"calibration: {members} members, {grains} grains ({time.time() - t0:.1f} s)")
+    st = sevc.State(opts, meshes)
+
+    def set_all(gb, mu):
+        for k, m in enumerate(meshes):
+            st.set_params(k, pos=m["pos"], gb_scaling=gb, mobility=mu)
+
+    set_all(1.1, 0.9)
+    st.window_forward(steps, ckpt)
+    targets = np.zeros((members, 20))
+    targets, _ = st.area_histogram(20, ranges, targets)
+    theta = np.zeros(2)                        # log gamma, log mu
+    m1, m2 = np.zeros(2), np.zeros(2)
+    lr, b1, b2 = 0.02, 0.9, 0.999
+    trace = []
+    t1 = time.perf_counter()
+    t_dev = 0.0
+    for it in range(iters):
+        gb, mu = float(np.exp(theta[0])), float(np.exp(theta[1]))
+        set_all(gb, mu)
+        a = time.perf_counter()
+        st.window_forward(steps, ckpt)
+        _, losses = st.area_histogram(20, ranges, targets, seed=True)    # dL/dx stays on the device
+        L = float(losses.sum())
+        res = st.adjoint(None)
+        t_dev += time.perf_counter() - a
+        g = np.array([sum(r[2] for r in res) * gb, sum(r[3] for r in res) * mu])
+        trace.append((L, gb, mu))
+        m1 = b1 * m1 + (1 - b1) * g
+        m2 = b2 * m2 + (1 - b2) * g * g
+        theta -= lr * (m1 / (1 - b1 ** (it + 1))) / (np.sqrt(m2 / (1 - b2 ** (it + 1))) + 1e-12)
+    el = time.perf_counter() - t1
+    st.close()
+    log(f"calibration: loss {trace[0][0]:.3e} -> {trace[-1][0]:.3e}, (gamma, mu) -> "
+        f"({trace[-1][1]:.4f}, {trace[-1][2]:.4f}); {el / iters:.3f} s per iteration")
+    return {"members": members, "grains": grains, "iterations": iters, "steps": steps, "ckpt_every": ckpt,
+            "dt": dt, "s_per_iteration": el / iters, "device_s_per_iteration": t_dev / iters,
+            "loss_first": trace[0][0], "loss_last": trace[-1][0],
+            "gamma_mu_last": [trace[-1][1], trace[-1][2]], "gamma_mu_true": [1.1, 0.9],
+            "gamma_times_mu_last": trace[-1][1] * trace[-1][2], "gamma_times_mu_true": 1.1 * 0.9,
+            "workload": f"{members} members, G ~ U[5000, 20000], lognormal SE (mean 0.3, sigma 0.5), "
+                        "20-bin area histograms; Adam on (log gamma, log mu) from (1, 1)"}
+
+
 def cpu_baseline(opts, mesh, steps):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from _util import Oracle  # the oracle is the CPU comparator
@@ -168,6 +339,12 @@ def main():
     ap.add_argument("--members", type=int, default=256, help="ensemble members (0: skip the ensemble line)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--b2b-steps", type=int, default=50, help="steps per member when stepped back to back")
+    ap.add_argument("--adjoint-grains", type=int, default=10000, help="adjoint window workload (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--adjoint-ckpt", type=int, default=50)
+    ap.add_argument("--calib-members", type=int, default=64, help="batched calibration workload (0: skip)")
+    ap.add_argument("--calib-iters", type=int, default=50)
+    ap.add_argument("--calib-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM ceiling and PMC byte-counter scale)")
     args = ap.parse_args()
@@ -250,6 +427,11 @@ def main():
     st.close()
     if args.members > 0:
         line["ensemble"] = bench_ensemble(sevc, args.members, args.ensemble_steps, args.b2b_steps)
+    if args.adjoint_grains > 0:
+        line["adjoint"] = bench_adjoint(sevc, args.adjoint_grains, 3, args.adjoint_steps, args.adjoint_ckpt, 1e-6)
+    if args.calib_members > 0:
+        line["calibration"] = bench_calibration(sevc, args.calib_members, args.calib_iters, args.calib_steps, 100,
+                                                5e-7)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
     print(json.dumps(line), flush=True)

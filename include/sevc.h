@@ -248,6 +248,9 @@ int32_t sevc_n_members(const sevc_state* s);
  * (device_to_host, translation.h:184-199, without freeing the device copy). */
 int sevc_get_mesh(sevc_state* s, int32_t member, sevc_mesh* out);
 
+/* Copy only the vertex positions of a member (2 per vertex slot). */
+int sevc_get_positions(sevc_state* s, int32_t member, double* pos);
+
 /* Replace the positions / stored energies of a member (adjoint driver,
  * parameter sweeps).  NULL arrays are left unchanged. */
 int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
@@ -272,10 +275,26 @@ typedef struct sevc_grads {
 int sevc_window_forward(sevc_state* s, int32_t steps, int32_t ckpt_every);
 
 /* Reverse pass over the last window: dL_dpos[m] = dL/dx at the window's
- * end for member m (2 per vertex slot, host memory); fills grads[m].
+ * end for member m (2 per vertex slot, host memory), or dL_dpos == NULL to
+ * use the device seed left by sevc_area_histogram; fills grads[m].
  * Segments are recomputed from their checkpoints; results are
  * deterministic (fixed-order sums). */
 int sevc_adjoint_window(sevc_state* s, const double* const* dL_dpos, sevc_grads* grads);
+
+/* Smooth histogram of grain areas and its loss, on the device, at the
+ * current positions (extension, for calibration with the adjoint window;
+ * no counterpart in the reference).  Member m with ranges[3m..3m+2] =
+ * (a_min, a_max, width) and n_bins bins of centres
+ * c_k = a_min + (k + 1/2)(a_max - a_min)/n_bins:
+ *   h_k = (1/G) sum_g exp(-((|A_g| - c_k)/width)^2 / 2)
+ * over its enabled grains with >= 3 vertices, and
+ *   loss[m] = sum_k (h_k - targets[m*n_bins + k])^2.
+ * hist (n_bins per member), loss and dL_dpos (per-member host arrays of
+ * 2 per vertex slot) may be NULL.  With seed_adjoint != 0, dL/dx stays on
+ * the device as the seed of the next sevc_adjoint_window call made with
+ * dL_dpos == NULL.  n_bins <= 64. */
+int sevc_area_histogram(sevc_state* s, int32_t n_bins, const double* ranges, const double* targets,
+                        double* hist, double* loss, double* const* dL_dpos, int32_t seed_adjoint);
 
 /* Compute the diagnostics of the next step (vertex energies,
  * dA/dt and velocity parts, calculus.h:123-133,243-253,62-69). */

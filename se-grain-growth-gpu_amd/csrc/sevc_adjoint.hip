@@ -226,3 +226,143 @@ __global__ void k_win_dse(Dev d, const double* acc_se, double* dse) {
 }
 
 }  // namespace sevc
+
+namespace sevc {
+
+// ---------------------------------------------------------------------
+// smooth area histogram loss (calibration), on the device
+// ---------------------------------------------------------------------
+//
+// Member m: c_k = a_min + (k + 1/2)(a_max - a_min)/nb, width w,
+//   h_k = (1/G) sum_g exp(-z_gk^2 / 2),  z_gk = (|A_g| - c_k)/w,
+//   L   = sum_k (h_k - t_k)^2,
+// over its enabled grains with at least 3 vertices (G of them).  Sums run in
+// a fixed order (grain index within 256-grain blocks, then blocks), so the
+// result does not depend on scheduling.
+
+__device__ __forceinline__ double hist_centre(const double* rg, int nb, int k) {
+    return rg[0] + (k + 0.5) * (rg[1] - rg[0]) / nb;
+}
+
+// Level 1: one workgroup per grain table block.  Stores the signed areas;
+// part[lb*(nb+1) + k] = sum over the block's grains of exp(-z^2/2) for bin
+// k, part[lb*(nb+1) + nb] = number of grains in the histogram.
+__global__ void __launch_bounds__(256) k_hist_l1(Dev d, int nb, const double* ranges, double* hA, double* part) {
+    __shared__ double a_s[kBlock];
+    __shared__ unsigned char ok_s[kBlock];
+    TABLE_BLOCK(d.gblk, d.ngb)
+    const int t = threadIdx.x;
+    const bool ok = i < bd_.e && d.genab[i] && d.glen[i] >= 3;
+    double a = 0.0;
+    if (ok) {
+        a = grain_area_at(d, d.pos, i, P.D);
+        hA[i] = a;
+    } else if (i < bd_.e) {
+        hA[i] = 0.0;
+    }
+    a_s[t] = fabs(a);
+    ok_s[t] = ok ? 1 : 0;
+    __syncthreads();
+    const int n = bd_.e - bd_.b;
+    const double* rg = ranges + 3 * bd_.m;
+    if (t < nb) {
+        const double c = hist_centre(rg, nb, t), w = rg[2];
+        double s = 0.0;
+        for (int q = 0; q < n; q++)
+            if (ok_s[q]) {
+                const double z = (a_s[q] - c) / w;
+                s += sevc_exp(-0.5 * (z * z));
+            }
+        part[(size_t)lb_ * (nb + 1) + t] = s;
+    } else if (t == nb) {
+        int cnt = 0;
+        for (int q = 0; q < n; q++) cnt += ok_s[q];
+        part[(size_t)lb_ * (nb + 1) + nb] = cnt;
+    }
+}
+
+// Level 2: one workgroup per member.  hist[m*nb+k] = h_k, res[m*nb+k] =
+// h_k - t_k, loss[m] = sum_k (h_k - t_k)^2, gcount[m] = G.
+__global__ void k_hist_member(Dev d, int nb, const double* part, const double* targets, double* hist, double* res,
+                              double* loss, double* gcount) {
+    __shared__ double r_s[kMaxHistBins];
+    __shared__ double g_s;
+    const int m = blockIdx.x;
+    if (m >= d.B) return;
+    const int first = d.gblk_first[m], cnt = d.gblk_first[m + 1] - first;
+    const int t = threadIdx.x;
+    if (t == nb) {
+        double g = 0.0;
+        for (int b = first; b < first + cnt; b++) g += part[(size_t)b * (nb + 1) + nb];
+        g_s = g;
+    }
+    __syncthreads();
+    if (t < nb) {
+        double s = 0.0;
+        for (int b = first; b < first + cnt; b++) s += part[(size_t)b * (nb + 1) + t];
+        const double G = g_s > 0 ? g_s : 1.0;
+        const double h = s / G;
+        const double r = h - targets[(size_t)m * nb + t];
+        hist[(size_t)m * nb + t] = h;
+        res[(size_t)m * nb + t] = r;
+        r_s[t] = r;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double L = 0.0;
+        for (int k = 0; k < nb; k++) L += r_s[k] * r_s[k];
+        loss[m] = L;
+        gcount[m] = g_s;
+    }
+}
+
+// Per grain: wg = dL/dA_g = sign(A_g) sum_k 2 r_k exp(-z^2/2) (-z/w) / G.
+__global__ void k_hist_weight(Dev d, int nb, const double* ranges, const double* hA, const double* res,
+                              const double* gcount, double* wg) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (i >= bd_.e) return;
+    if (!(d.genab[i] && d.glen[i] >= 3)) {
+        wg[i] = 0.0;
+        return;
+    }
+    const double* rg = ranges + 3 * bd_.m;
+    const double* r = res + (size_t)bd_.m * nb;
+    const double a = fabs(hA[i]), w = rg[2];
+    const double G = gcount[bd_.m] > 0 ? gcount[bd_.m] : 1.0;
+    double s = 0.0;
+    for (int k = 0; k < nb; k++) {
+        const double z = (a - hist_centre(rg, nb, k)) / w;
+        s += 2.0 * r[k] * sevc_exp(-0.5 * (z * z)) * (-z / w);
+    }
+    s = s / G;
+    wg[i] = hA[i] < 0 ? -s : s;
+}
+
+// Per vertex: dL/dp_v = sum over its grains of wg * dA_g/dp_v with
+// dA/dx_v = (dy_prev + dy_next)/2, dA/dy_v = -(dx_prev + dx_next)/2.
+__global__ void k_hist_seed(Dev d, const double* wg, double* seed) {
+    TABLE_BLOCK(d.vblk, d.nvb)
+    if (i >= bd_.e) return;
+    const int v = i;
+    double gx = 0.0, gy = 0.0;
+    if (d.ven[v]) {
+        const double D = P.D, x = d.pos[2 * v], y = d.pos[2 * v + 1];
+        for (int j = 0; j < 3; j++) {
+            const int g = d.vg[3 * v + j];
+            const int n = d.glen[g];
+            const int k = grain_contains_vertex(d, g, v);
+            if (k < 0 || n < 3 || !d.genab[g]) continue;
+            const int* L = glist(d, g);
+            const int wp = L[(k + n - 1) % n], wn = L[(k + 1) % n];
+            const double dpx = wrapd(x, d.pos[2 * wp], D), dpy = wrapd(y, d.pos[2 * wp + 1], D);
+            const double dnx = wrapd(d.pos[2 * wn], x, D), dny = wrapd(d.pos[2 * wn + 1], y, D);
+            const double c = wg[g] * 0.5;
+            gx += c * (dpy + dny);
+            gy += c * (-(dpx + dnx));
+        }
+    }
+    seed[2 * v] = gx;
+    seed[2 * v + 1] = gy;
+}
+
+}  // namespace sevc

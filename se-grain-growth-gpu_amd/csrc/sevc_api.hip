@@ -65,6 +65,12 @@ __global__ void k_win_adjoint(Dev d, const double* X, const double* A, const dou
 __global__ void k_win_sum_l1(Dev d, const double* acc_mu, const double* acc_s, double* part);
 __global__ void k_win_sum_up(Dev d, double* part, double* out);
 __global__ void k_win_dse(Dev d, const double* acc_se, double* dse);
+__global__ void k_hist_l1(Dev d, int nb, const double* ranges, double* hA, double* part);
+__global__ void k_hist_member(Dev d, int nb, const double* part, const double* targets, double* hist, double* res,
+                              double* loss, double* gcount);
+__global__ void k_hist_weight(Dev d, int nb, const double* ranges, const double* hA, const double* res,
+                              const double* gcount, double* wg);
+__global__ void k_hist_seed(Dev d, const double* wg, double* seed);
 }  // namespace sevc
 
 using namespace sevc;
@@ -106,6 +112,10 @@ struct sevc_state {
         size_t ck_n = 0, seg_n = 0, segA_n = 0;
         double *lamA = nullptr, *lamB = nullptr, *acc_mu = nullptr, *acc_s = nullptr, *acc_se = nullptr;
         double *ebase = nullptr, *part = nullptr, *out = nullptr, *dse = nullptr;
+        bool seeded = false;              // lamA holds dL/dx from sevc_area_histogram
+        // area histogram loss
+        double *hA = nullptr, *hwg = nullptr, *hpart = nullptr, *hrange = nullptr, *htarget = nullptr;
+        double *hhist = nullptr, *hres = nullptr, *hloss = nullptr, *hcount = nullptr;
     } win;
 };
 
@@ -649,6 +659,14 @@ int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
     return SEVC_OK;
 }
 
+int sevc_get_positions(sevc_state* s, int32_t m, double* pos) {
+    if (!s || !pos || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipMemcpyAsync(pos, s->d.pos + 2 * static_cast<size_t>(s->voff[m]), 16 * static_cast<size_t>(s->nv[m]),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return SEVC_OK;
+}
+
 int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const double* g_SE, double gb_scaling,
                            double mobility) {
     if (!s || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
@@ -691,6 +709,22 @@ int win_buffer(sevc_state* s, double** p, size_t* have, size_t n) {
     return rc;
 }
 
+// Reverse-pass arrays whose size depends only on the state.
+int win_fixed_buffers(sevc_state* s) {
+    auto& w = s->win;
+    if (w.lamA) return SEVC_OK;
+    const size_t V = s->V, n2 = 2 * V;
+    int rc;
+    if ((rc = dalloc(s, &w.lamA, n2)) || (rc = dalloc(s, &w.lamB, n2)) || (rc = dalloc(s, &w.acc_mu, V)) ||
+        (rc = dalloc(s, &w.acc_s, V)) || (rc = dalloc(s, &w.acc_se, 3 * V)) ||
+        (rc = dalloc(s, &w.ebase, static_cast<size_t>(s->E))) ||
+        (rc = dalloc(s, &w.part, 4 * static_cast<size_t>(s->d.nvb))) ||
+        (rc = dalloc(s, &w.out, 2 * static_cast<size_t>(s->B))) ||
+        (rc = dalloc(s, &w.dse, static_cast<size_t>(s->G))))
+        return rc;
+    return SEVC_OK;
+}
+
 // Steps x_{c K} .. of segment c from the segment buffer's slot 0; areas of
 // each step into segA.  Forward of the window (state outputs on) or its
 // recomputation in the reverse pass (off).
@@ -723,16 +757,9 @@ int sevc_window_forward(sevc_state* s, int32_t steps, int32_t ckpt_every) {
         (rc = win_buffer(s, &w.seg, &w.seg_n, (static_cast<size_t>(K) + 1) * n2)) != SEVC_OK ||
         (rc = win_buffer(s, &w.segA, &w.segA_n, static_cast<size_t>(K) * s->G)) != SEVC_OK)
         return rc;
-    if (!w.lamA) {
-        if ((rc = dalloc(s, &w.lamA, n2)) || (rc = dalloc(s, &w.lamB, n2)) || (rc = dalloc(s, &w.acc_mu, V)) ||
-            (rc = dalloc(s, &w.acc_s, V)) || (rc = dalloc(s, &w.acc_se, 3 * V)) ||
-            (rc = dalloc(s, &w.ebase, static_cast<size_t>(s->E))) ||
-            (rc = dalloc(s, &w.part, 4 * static_cast<size_t>(d.nvb))) ||
-            (rc = dalloc(s, &w.out, 2 * static_cast<size_t>(s->B))) ||
-            (rc = dalloc(s, &w.dse, static_cast<size_t>(s->G))))
-            return rc;
-    }
+    if ((rc = win_fixed_buffers(s)) != SEVC_OK) return rc;
     w.ready = false;
+    w.seeded = false;
     launch(s, "win_prepare", k_win_prepare, d.neb, kBlock, d, w.ebase);
     HIPCHK(hipMemcpyAsync(w.seg, d.pos, n2 * 8, hipMemcpyDeviceToDevice, s->stream));
     int last = 0;
@@ -757,16 +784,21 @@ int sevc_window_forward(sevc_state* s, int32_t steps, int32_t ckpt_every) {
 }
 
 int sevc_adjoint_window(sevc_state* s, const double* const* dL_dpos, sevc_grads* grads) {
-    if (!s || !dL_dpos || !grads) return SEVC_E_INVALID_ARG;
+    if (!s || !grads) return SEVC_E_INVALID_ARG;
     auto& w = s->win;
     if (!w.ready) return fail(s, SEVC_E_STATE, "no window: call sevc_window_forward first (after any step)");
     const Dev& d = s->d;
     const size_t n2 = 2 * static_cast<size_t>(s->V);
-    for (int m = 0; m < s->B; ++m) {
-        if (!dL_dpos[m]) return fail(s, SEVC_E_INVALID_ARG, "dL_dpos missing for a member");
-        HIPCHK(hipMemcpyAsync(w.lamA + 2 * static_cast<size_t>(s->voff[m]), dL_dpos[m],
-                              16 * static_cast<size_t>(s->nv[m]), hipMemcpyHostToDevice, s->stream));
+    if (dL_dpos) {
+        for (int m = 0; m < s->B; ++m) {
+            if (!dL_dpos[m]) return fail(s, SEVC_E_INVALID_ARG, "dL_dpos missing for a member");
+            HIPCHK(hipMemcpyAsync(w.lamA + 2 * static_cast<size_t>(s->voff[m]), dL_dpos[m],
+                                  16 * static_cast<size_t>(s->nv[m]), hipMemcpyHostToDevice, s->stream));
+        }
+    } else if (!w.seeded) {
+        return fail(s, SEVC_E_STATE, "dL_dpos is NULL and no sevc_area_histogram seed is held");
     }
+    w.seeded = false;
     launch(s, "win_init", k_win_init, d.nvb, kBlock, d, w.lamA, w.acc_mu, w.acc_s, w.acc_se);
     double *cur = w.lamA, *nxt = w.lamB;
     for (int c = w.nck - 1; c >= 0; --c) {
@@ -807,6 +839,56 @@ int sevc_adjoint_window(sevc_state* s, const double* const* dL_dpos, sevc_grads*
         grads[m].d_gb_scaling = out[2 * static_cast<size_t>(m)];
         grads[m].d_mobility = out[2 * static_cast<size_t>(m) + 1];
     }
+    return SEVC_OK;
+}
+
+int sevc_area_histogram(sevc_state* s, int32_t n_bins, const double* ranges, const double* targets, double* hist,
+                        double* loss, double* const* dL_dpos, int32_t seed_adjoint) {
+    if (!s || !ranges || !targets || n_bins < 1 || n_bins > kMaxHistBins) return SEVC_E_INVALID_ARG;
+    for (int m = 0; m < s->B; ++m)
+        if (!(ranges[3 * m + 2] > 0) || !(ranges[3 * m + 1] > ranges[3 * m]))
+            return fail(s, SEVC_E_INVALID_ARG, "histogram range needs a_max > a_min and width > 0");
+    auto& w = s->win;
+    const Dev& d = s->d;
+    int rc;
+    if ((rc = win_fixed_buffers(s)) != SEVC_OK) return rc;
+    const size_t B = s->B, G = s->G, nb = static_cast<size_t>(n_bins);
+    if (!w.hA) {
+        if ((rc = dalloc(s, &w.hA, G)) || (rc = dalloc(s, &w.hwg, G)) ||
+            (rc = dalloc(s, &w.hpart, static_cast<size_t>(d.ngb) * (kMaxHistBins + 1))) ||
+            (rc = dalloc(s, &w.hrange, 3 * B)) || (rc = dalloc(s, &w.htarget, B * kMaxHistBins)) ||
+            (rc = dalloc(s, &w.hhist, B * kMaxHistBins)) || (rc = dalloc(s, &w.hres, B * kMaxHistBins)) ||
+            (rc = dalloc(s, &w.hloss, B)) || (rc = dalloc(s, &w.hcount, B)))
+            return rc;
+    }
+    HIPCHK(hipMemcpyAsync(w.hrange, ranges, 3 * B * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(w.htarget, targets, B * nb * 8, hipMemcpyHostToDevice, s->stream));
+    launch(s, "hist_l1", k_hist_l1, d.ngb, kBlock, d, n_bins, static_cast<const double*>(w.hrange), w.hA, w.hpart);
+    launch(s, "hist_member", k_hist_member, s->B, kMaxHistBins + 64, d, n_bins,
+           static_cast<const double*>(w.hpart), static_cast<const double*>(w.htarget), w.hhist, w.hres, w.hloss,
+           w.hcount);
+    const bool need_grad = seed_adjoint || dL_dpos;
+    if (need_grad) {
+        launch(s, "hist_weight", k_hist_weight, d.ngb, kBlock, d, n_bins, static_cast<const double*>(w.hrange),
+               static_cast<const double*>(w.hA), static_cast<const double*>(w.hres),
+               static_cast<const double*>(w.hcount), w.hwg);
+        launch(s, "hist_seed", k_hist_seed, d.nvb, kBlock, d, static_cast<const double*>(w.hwg), w.lamA);
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(s, SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
+    std::vector<double> hh(B * nb);
+    // the kernels store member m's bins at m * n_bins
+    if (hist) HIPCHK(hipMemcpyAsync(hh.data(), w.hhist, B * nb * 8, hipMemcpyDeviceToHost, s->stream));
+    if (loss) HIPCHK(hipMemcpyAsync(loss, w.hloss, B * 8, hipMemcpyDeviceToHost, s->stream));
+    if (dL_dpos)
+        for (int m = 0; m < s->B; ++m)
+            if (dL_dpos[m])
+                HIPCHK(hipMemcpyAsync(dL_dpos[m], w.lamA + 2 * static_cast<size_t>(s->voff[m]),
+                                      16 * static_cast<size_t>(s->nv[m]), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (hist) std::memcpy(hist, hh.data(), hh.size() * 8);
+    if (s->timing) collect_timers(s);
+    w.seeded = need_grad && seed_adjoint;
     return SEVC_OK;
 }
 

@@ -57,6 +57,9 @@ def lib():
         "sevc_set_member_params": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                              C.c_double, C.c_double]),
         "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
+        "sevc_get_positions": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double)]),
+        "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
+                                          P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "sevc_adjoint_window": (C.c_int, [C.c_void_p, P(P(C.c_double)), P(_abi.Grads)]),
         "sevc_enable_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -180,6 +183,33 @@ class State:
         h.D = h.struct.domain_bound
         return h
 
+    def area_histogram(self, n_bins, ranges, targets, seed=False, want_grad=False):
+        """Device area-histogram loss of every member at the current
+        positions.  ranges: (n_members, 3) of (a_min, a_max, width);
+        targets: (n_members, n_bins).  Returns (hist, loss[, dL_dpos list]);
+        seed=True keeps dL/dx on the device for adjoint(None)."""
+        P = C.POINTER(C.c_double)
+        B = self.n_members
+        rg = np.ascontiguousarray(ranges, dtype=np.float64).reshape(B, 3)
+        tg = np.ascontiguousarray(targets, dtype=np.float64).reshape(B, n_bins)
+        hist = np.zeros((B, n_bins))
+        loss = np.zeros(B)
+        grads, arr = None, None
+        if want_grad:
+            grads = [np.zeros(2 * self.sizes(m)[0]) for m in range(B)]
+            arr = (P * B)(*[g.ctypes.data_as(P) for g in grads])
+        _check(lib().sevc_area_histogram(self.h, int(n_bins), rg.ctypes.data_as(P), tg.ctypes.data_as(P),
+                                         hist.ctypes.data_as(P), loss.ctypes.data_as(P), arr, int(bool(seed))),
+               self.h)
+        return (hist, loss, grads) if want_grad else (hist, loss)
+
+    def positions(self, m=0):
+        """Vertex positions of member m, shape (n_vertex_slots, 2)."""
+        nv = self.sizes(m)[0]
+        out = np.empty(2 * nv)
+        _check(lib().sevc_get_positions(self.h, int(m), out.ctypes.data_as(C.POINTER(C.c_double))), self.h)
+        return out.reshape(-1, 2)
+
     def request_diagnostics(self):
         _check(lib().sevc_request_diagnostics(self.h), self.h)
 
@@ -196,24 +226,28 @@ class State:
         dt) on every member; checkpoints every ckpt_every steps (0: all)."""
         _check(lib().sevc_window_forward(self.h, int(steps), int(ckpt_every)), self.h)
 
-    def adjoint(self, dL_dpos):
+    def adjoint(self, dL_dpos=None):
         """Reverse pass over the last window.  dL_dpos: one array per member
-        (2 per vertex slot) or a single array for a one-member state.
+        (2 per vertex slot), a single array for a one-member state, or None
+        for the device seed of area_histogram(seed=True).
         Returns, per member, (d_pos, d_SE, d_gb_scaling, d_mobility)."""
         if isinstance(dL_dpos, np.ndarray):
             dL_dpos = [dL_dpos]
         P = C.POINTER(C.c_double)
-        ins = [np.ascontiguousarray(g, dtype=np.float64).reshape(-1) for g in dL_dpos]
         outs = []
-        arr_in = (P * self.n_members)()
         arr_g = (_abi.Grads * self.n_members)()
+        arr_in = None
+        if dL_dpos is not None:
+            ins = [np.ascontiguousarray(g, dtype=np.float64).reshape(-1) for g in dL_dpos]
+            arr_in = (P * self.n_members)()
         for m in range(self.n_members):
             nv, ne, ng, S = self.sizes(m)
-            if ins[m].size != 2 * nv:
-                raise ValueError(f"member {m}: dL_dpos has {ins[m].size} values, expected {2 * nv}")
+            if arr_in is not None:
+                if ins[m].size != 2 * nv:
+                    raise ValueError(f"member {m}: dL_dpos has {ins[m].size} values, expected {2 * nv}")
+                arr_in[m] = ins[m].ctypes.data_as(P)
             dp, ds = np.zeros(2 * nv), np.zeros(ng)
             outs.append((dp, ds))
-            arr_in[m] = ins[m].ctypes.data_as(P)
             arr_g[m].d_pos = dp.ctypes.data_as(P)
             arr_g[m].d_SE = ds.ctypes.data_as(P)
         _check(lib().sevc_adjoint_window(self.h, arr_in, arr_g), self.h)

@@ -189,3 +189,33 @@ def test_torch_function_gradients_match_differences():
     for name, ad, f in checks:
         assert abs(ad - f) <= 1e-6 * abs(f) + 1e-12, (name, ad, f)
     st.close()
+
+
+def test_device_area_histogram_matches_numpy():
+    """sevc_area_histogram (device) against sevc.calib (numpy): histogram,
+    loss and dL/dx at the window's end; the device seed then drives the
+    reverse pass exactly as the same gradient passed from the host."""
+    from sevc.calib import AreaHistogram, grain_areas
+    o1, m1 = _mesh(n=400, seed=11)
+    o2, m2 = _mesh(n=250, seed=12, mobility=1.1)
+    st = sevc.State([o1, o2], [m1, m2])
+    st.window_forward(20, 6)
+    ranges, hs = [], []
+    for m in (m1, m2):
+        am = float(np.abs(grain_areas(m)).mean())
+        ranges.append((0.0, 3.0 * am, 0.1 * am))
+        hs.append(AreaHistogram(20, 0.0, 3.0 * am, 0.1 * am))
+    targets = np.random.default_rng(0).random((2, 20)) * 0.5
+    hist, loss, grads = st.area_histogram(20, ranges, targets, seed=True, want_grad=True)
+    for k, (h, m) in enumerate(zip(hs, (m1, m2))):
+        P = st.positions(k)
+        Lr, gr = h.loss_and_grad(m, P, targets[k])
+        assert np.allclose(hist[k], h(m, P), rtol=1e-12, atol=1e-15), k
+        assert abs(loss[k] - Lr) <= 1e-12 * abs(Lr), (k, loss[k], Lr)
+        assert np.allclose(grads[k], gr, rtol=1e-10, atol=1e-12 * np.abs(gr).max()), k
+    seeded = st.adjoint(None)
+    host = st.adjoint(grads)
+    for a, b in zip(seeded, host):
+        for x, y in zip(a, b):
+            assert _same(x, y)
+    st.close()
