@@ -28,6 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # each datum once): velocity 98 B, translation 48 B.
 B_VELOCITY = 98.0
 B_TRANSLATE = 48.0
+# HBM bytes per enabled vertex of the two velocity passes at the 1M-grain
+# workload, from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy of known
+# size (profiles/step_1M_hbm_traffic.md): 551.4 MB for 2.0M vertices.
+TRAFFIC_VELOCITY_B_PER_VERTEX = 276.0
 
 
 def log(*a):
@@ -406,7 +410,10 @@ def main():
                                "adaptive dt, flip threshold 1e-4",
                    "grains": args.grains, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_VELOCITY_B_PER_VERTEX * verts_per_step
+                                 if args.grains == 1_000_000 and args.seed == 1 else None),
+                     "traffic_unit": "bytes per launch (both passes)",
                      "kernel": "FD velocity (k_vel_grain + k_vel_vertex)", "bytes_per_vertex": B_VELOCITY,
                      "avg_launch_us": vel_avg * 1e6,
                      "pass_us": {"k_vel_grain": g_avg * 1e6, "k_vel_vertex": v_avg * 1e6}},

@@ -166,3 +166,44 @@ def test_batch_members_match_separate_oracles():
             assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
             for key in STAT_KEYS:
                 assert getattr(s1, key) == getattr(s2, key), (k, key)
+
+
+def _config1(seed, dt=2e-4):
+    """2,000-grain fixture with the model options of the small-scale runs:
+    gamma = mu = 1, SE = U[0,1) * 0.5, rate nucleation 1e-3 * SE per grain
+    and step with Philox seed `seed`, adaptive dt (upper bound dt), flip
+    threshold 1e-4, stop at 50 % of the grains."""
+    return ref2k_options(dt=dt, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                         nucleation_mode=sevc.NUCL_RATE, nucleation_rate=1e-3, nucleation_scaling=0.9,
+                         seed=seed, extra_capacity=0.05, leave_percent=0.5)
+
+
+def test_seed_ensemble_curves_match_oracle():
+    """32 Philox seeds of the 2,000-grain workload in one batched state: every
+    member's state, grain count and mean grain area equal its own oracle
+    run after each chunk (so the ensemble curves agree exactly)."""
+    seeds = list(range(32))
+    opts = [_config1(s) for s in seeds]
+    meshes = [sevc.load_files(o) for o in opts]
+    st = sevc.State(opts, meshes)
+    orcs = [Oracle(o, m) for o, m in zip(opts, meshes)]
+    curves = {"gpu": [], "oracle": []}
+    events = 0
+    for chunk in (40, 40, 40):
+        print(f"  chunk of {chunk} steps x {len(seeds)} members", flush=True)
+        st.step(chunk)
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            s1 = st.member_stats(k)
+            a, b = st.get_mesh(k), orc.get_mesh()
+            assert diff_fields(a, b) == {}, seeds[k]
+            for key in STAT_KEYS:
+                assert getattr(s1, key) == getattr(s2, key), (seeds[k], key)
+            events += s1.n_nucleations + s1.n_flips + s1.n_removals
+            for name, h, s in (("gpu", a, s1), ("oracle", b, s2)):
+                en = h["g_enabled"] == 1
+                curves[name].append((s.n_grains, float(np.abs(h["g_area"][en]).mean())))
+    assert curves["gpu"] == curves["oracle"]
+    assert events > 0
+    st.close()
