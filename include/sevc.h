@@ -166,7 +166,8 @@ typedef struct sevc_state sevc_state;
 void sevc_options_default(sevc_options* o);
 
 /* Parse an .ini file with the semantics of load_config_file
- * (options.h:69-206): '#'/'[' lines skipped, exact "key=value" match,
+ * (options.h:69-206): the defaults first, then the file's keys;
+ * '#'/'[' lines skipped, exact "key=value" match,
  * do_nucleation true only for "true", nucleation_gap = (int)(dtau/dt),
  * tmax/nmax defaulting (options.h:173-181).  Unknown keys are ignored;
  * the extension keys are accepted too. */

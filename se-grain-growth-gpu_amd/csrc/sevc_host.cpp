@@ -80,6 +80,7 @@ int sevc_options_from_ini(const char* path, sevc_options* o) {
     if (!path || !o) return SEVC_E_INVALID_ARG;
     FILE* f = std::fopen(path, "r");
     if (!f) return SEVC_E_IO;
+    sevc_options_default(o);                      // load_config_file starts from its defaults (options.h:71-85)
     char line[1000];
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '[') continue;

@@ -13,6 +13,7 @@ from ._abi import HostMesh, Mesh, Options, StepStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SEVC_LIB_PATH") or os.path.join(os.path.dirname(_HERE), "build", "libsevc.so")
+CLI_PATH = os.path.join(os.path.dirname(LIB_PATH), "sevc")       # command-line driver (sevc -c config.ini)
 
 _lib = None
 
@@ -93,8 +94,9 @@ def default_options():
     return o
 
 
-def options_from_ini(path, base=None):
-    o = base.copy() if base is not None else default_options()
+def options_from_ini(path):
+    """Options of an .ini file: the defaults, then every key the file sets."""
+    o = Options()
     _check(lib().sevc_options_from_ini(path.encode(), C.byref(o)))
     return o
 

@@ -15,8 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_cli_snapshots_match_oracle(tmp_path):
-    from sevc import lib as L
-    cli = os.path.join(os.path.dirname(L.LIB_PATH), "sevc")
+    cli = sevc.CLI_PATH
     assert os.path.exists(cli), "CLI not built"
     out = tmp_path / "run"
     ini = tmp_path / "c.ini"

@@ -168,3 +168,21 @@ def test_snapshot_format(tmp_path):
     hg = lines.index("#GRAIN ID ENABLED NUCLEATED ORIENT SE AREA dAdt VLEN VERTICES")
     g0 = lines[hg + 1].split()
     assert len(g0) == 8 + 100 and int(g0[7]) == m["g_vlen"][0]
+
+
+def test_cli_usage_and_clean_failure(tmp_path):
+    """The command-line driver prints its usage for bad arguments and, on a
+    machine without a GPU, loads the inputs and stops with a message."""
+    import subprocess
+    cli = sevc.CLI_PATH
+    assert os.path.exists(cli)
+    r = subprocess.run([cli], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    if sevc.lib().sevc_device_count() == 0:
+        ini = tmp_path / "c.ini"
+        ini.write_text("\n".join([
+            f"vertex_fname={REF2K}/vertices.txt", f"boundary_fname={REF2K}/ridges.txt",
+            f"orientation_fname={REF2K}/oris.txt", f"SE_fname={REF2K}/SE/uniform.txt",
+            "n_grains=2000", f"folder={tmp_path / 'out'}", ""]))
+        r = subprocess.run([cli, "-c", str(ini)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "no HIP device" in r.stderr, r.stderr
