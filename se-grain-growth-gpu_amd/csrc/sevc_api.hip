@@ -16,8 +16,7 @@
 namespace sevc {
 // kernels of sevc_kernels.hip
 __global__ void k_begin(Dev d);
-__global__ void k_count_grains(Dev d);
-__global__ void k_count_vertices(Dev d);
+__global__ void k_count_all(Dev d);
 __global__ void k_tau_partial(Dev d);
 __global__ void k_member_reduce(Dev d, int what);
 __global__ void k_end(Dev d);
@@ -51,7 +50,6 @@ __global__ void k_nucl_area_l1(Dev d);
 __global__ void k_nucl_area_up(Dev d);
 __global__ void k_nucl_factor(Dev d);
 __global__ void k_rate_cand(Dev d);
-__global__ void k_block_counts(Dev d, int kind);
 __global__ void k_nucl_manager(Dev d);
 __global__ void k_nucl_refresh(Dev d);
 // kernels of sevc_adjoint.hip
@@ -198,9 +196,6 @@ void launch_step(sevc_state* s, bool diag) {
     if (s->any_nucl) {
         launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
         launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
-        launch(s, "block_counts", k_block_counts, d.nvb, kBlock, d, 0);
-        launch(s, "block_counts", k_block_counts, d.neb, kBlock, d, 1);
-        launch(s, "block_counts", k_block_counts, d.ngb, kBlock, d, 2);
         launch(s, "nucl_manager", k_nucl_manager, s->B, 256, d);
         launch(s, "nucl_refresh", k_nucl_refresh, kListGrid, 256, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
@@ -228,7 +223,7 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "t2_split", k_t2_split, kListGrid, 256, d);
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
-        launch(s, "fix", k_fix, kListGrid, 256, d, 1);
+        launch(s, "fix", k_fix, kListGrid, 64, d, 1);
         launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 1, 0);
         launch(s, "area_list", k_area_list, kListGrid, 256, d, 0);
         launch(s, "area_list", k_area_list, kListGrid, 256, d, 1);
@@ -236,12 +231,11 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "flip_split", k_flip_split, kListGrid, 256, d);
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
-        launch(s, "fix", k_fix, kListGrid, 256, d, 2);
+        launch(s, "fix", k_fix, kListGrid, 64, d, 2);
         launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 2, 0);
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
-    launch(s, "count_grains", k_count_grains, d.ngb, kBlock, d);
-    launch(s, "count_vertices", k_count_vertices, d.nvb, kBlock, d);
+    launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);
     launch(s, "member_reduce", k_member_reduce, s->B, 256, d, 1);
     launch(s, "end", k_end, nb, 64, d);
 }
@@ -366,7 +360,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.vblk_first, vf.size()); DA(d.eblk_first, ef.size()); DA(d.gblk_first, gf.size());
     DA(d.bc_v, vt.size()); DA(d.bc_e, et.size()); DA(d.bc_g, gt.size());
     DA(d.partial, 2 * gt.size() + 2);
-    DA(d.bmin, et.size()); DA(d.bcnt_g, gt.size()); DA(d.bcnt_v, vt.size());
+    DA(d.bmin, et.size());
     d.vblk = dvt; d.eblk = det; d.gblk = dgt;
     {
         std::vector<Blk> tt;
@@ -491,6 +485,12 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         hipMemset(d.vnuc, 0, V) != hipSuccess || hipMemset(d.cnt, 0, C_COUNT * 4) != hipSuccess ||
         hipMemset(d.nucl_r, 0, V * 8) != hipSuccess || hipMemset(d.nucl_ori, 0, V * 8) != hipSuccess ||
         hipMemset(d.dE, 0, V * 8) != hipSuccess || hipMemset(d.nfac, 0, V * 8) != hipSuccess) {
+        sevc_destroy(s);
+        return SEVC_E_HIP;
+    }
+    // per-block (enabled, free) counts, which the first step's nucleation reads
+    hipLaunchKernelGGL(k_count_all, dim3(d.ngb + d.neb + d.nvb), dim3(kBlock), 0, s->stream, d);
+    if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
         sevc_destroy(s);
         return SEVC_E_HIP;
     }

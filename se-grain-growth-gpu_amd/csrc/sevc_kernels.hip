@@ -66,24 +66,29 @@ __device__ __forceinline__ int block_count(int c) {
     return r;
 }
 
-__global__ void k_count_grains(Dev d) {
-    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    if (lb_ >= d.ngb) return;
-    const Blk bd_ = d.gblk[lb_];
-    const int i = bd_.b + (int)threadIdx.x;
-    int c = (d.ms[bd_.m].active && i < bd_.e && d.genab[i]) ? 1 : 0;
-    int r = block_count(c);
-    if (threadIdx.x == 0) d.bcnt_g[lb_] = r;
-}
-
-__global__ void k_count_vertices(Dev d) {
-    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    if (lb_ >= d.nvb) return;
-    const Blk bd_ = d.vblk[lb_];
-    const int i = bd_.b + (int)threadIdx.x;
-    int c = (d.ms[bd_.m].active && i < bd_.e && d.ven[i]) ? 1 : 0;
-    int r = block_count(c);
-    if (threadIdx.x == 0) d.bcnt_v[lb_] = r;
+// Per table block, for grains, edges and vertices in one launch: the
+// numbers of enabled and of free slots.  The enabled counts feed the
+// grain / vertex totals of main.cu:317; the pairs also serve the next
+// step's nucleation (free-slot search, alive-vertex draw), since nothing
+// enables or disables a slot between the end of a step and nucleation.
+__global__ void k_count_all(Dev d) {
+    int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb >= d.ngb + d.neb + d.nvb) return;
+    const Blk* tbl;
+    const uint8_t* en;
+    int2* out;
+    if (lb < d.ngb) {
+        tbl = d.gblk; en = d.genab; out = d.bc_g;
+    } else if (lb < d.ngb + d.neb) {
+        lb -= d.ngb; tbl = d.eblk; en = d.benab; out = d.bc_e;
+    } else {
+        lb -= d.ngb + d.neb; tbl = d.vblk; en = d.ven; out = d.bc_v;
+    }
+    const Blk bd = tbl[lb];
+    const int i = bd.b + (int)threadIdx.x;
+    const int c = (i < bd.e && en[i]) ? 1 : 0;
+    const int r = block_count(c);
+    if (threadIdx.x == 0) out[lb] = make_int2(r, (bd.e - bd.b) - r);
 }
 
 // One workgroup per member: sums / minimises the per-block partials.
@@ -108,8 +113,8 @@ __global__ void __launch_bounds__(256) k_member_reduce(Dev d, int what) {
         if (tid == 0) M.tau_bits = sm[0];
     } else {
         long long a = 0, b2 = 0;
-        for (int b = d.gblk_first[m] + tid; b < d.gblk_first[m + 1]; b += blockDim.x) a += d.bcnt_g[b];
-        for (int b = d.vblk_first[m] + tid; b < d.vblk_first[m + 1]; b += blockDim.x) b2 += d.bcnt_v[b];
+        for (int b = d.gblk_first[m] + tid; b < d.gblk_first[m + 1]; b += blockDim.x) a += d.bc_g[b].x;
+        for (int b = d.vblk_first[m] + tid; b < d.vblk_first[m + 1]; b += blockDim.x) b2 += d.bc_v[b].x;
         sa[tid] = a;
         sb[tid] = b2;
         __syncthreads();
@@ -725,10 +730,102 @@ __device__ void fix_one_grain(const Dev& d, int g) {
     }
 }
 
-__global__ void k_fix(Dev d, int stage) {
-    const int n = d.cnt[C_FIX0 + stage];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        fix_one_grain(d, d.l_fix[stage][k]);
+// fix_grain_lists for the flagged grains of a stage, one wave (workgroup of
+// 64) per grain.  The wave stages the grain's list and the four deferred-edit
+// records of every listed vertex in LDS with one round of independent loads;
+// lane 0 then replays perform_ADD_ops, perform_REMOVE_ops and
+// perform_REPLACE_ops (topological.h:1035-1136) on the staged copy in exactly
+// the order fix_one_grain uses on global memory (an entry inserted or
+// replaced during the replay reads its vertex's records from global memory,
+// as fix_one_grain does), and the wave writes the list back, including the
+// -1 padding fix_one_grain leaves after the removals.  Records do not change
+// during this kernel, and each grain edits only its own list.
+__global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
+    __shared__ int Ls[kMaxStride];
+    __shared__ short src[kMaxStride];            // index of the staged records, -1: read globally
+    __shared__ unsigned char mf[4 * kMaxStride];
+    __shared__ int mg[4 * kMaxStride], mv[4 * kMaxStride], mp[4 * kMaxStride];
+    __shared__ int n_add_s, n_fin_s;
+    const int n_items = d.cnt[C_FIX0 + stage];
+    const int lane = threadIdx.x;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int g = d.l_fix[stage][item];
+        if (!d.genab[g]) continue;
+        int* L = glist(d, g);
+        const int n0 = d.glen[g];
+        for (int i = lane; i < n0; i += 64) {
+            const int v = L[i];
+            Ls[i] = v;
+            src[i] = (short)i;
+            for (int j = 0; j < 4; j++) {
+                mf[4 * i + j] = d.mfix[4 * v + j];
+                mg[4 * i + j] = d.mgrn[4 * v + j];
+                mv[4 * i + j] = d.mvrt[4 * v + j];
+                mp[4 * i + j] = d.mprev[4 * v + j];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            auto rec = [&](int es, int v, int j, int& f, int& gg, int& vv, int& pp) {
+                if (es >= 0) {
+                    const int q = 4 * es + j;
+                    f = mf[q]; gg = mg[q]; vv = mv[q]; pp = mp[q];
+                } else {
+                    f = d.mfix[4 * v + j]; gg = d.mgrn[4 * v + j]; vv = d.mvrt[4 * v + j]; pp = d.mprev[4 * v + j];
+                }
+            };
+            int n = n0;
+            for (int i = 0; i < n; i++) {                            // ADD, :1040-1067
+                const int es = src[i], v = Ls[i];
+                for (int j = 0; j < 4; j++) {
+                    int f, gg, vv, pp;
+                    rec(es, v, j, f, gg, vv, pp);
+                    if (f != FIX_ADD || gg != g) continue;
+                    int pos = -1;                                    // grain_add_vertex, geometry.h:629-645
+                    for (int q = 0; q < n; q++) if (Ls[q] == pp) { pos = q + 1; break; }
+                    if (pos < 0) { set_error(d, SEVC_E_STATE); continue; }
+                    if (n >= d.S) { set_error(d, SEVC_E_GRAIN_FULL); continue; }
+                    for (int q = n; q > pos; q--) { Ls[q] = Ls[q - 1]; src[q] = src[q - 1]; }
+                    Ls[pos] = vv;
+                    src[pos] = -1;
+                    n++;
+                }
+            }
+            const int n_add = n;
+            uint64_t doomed[kMaxStride / 64] = {0, 0};               // REMOVE, :1086-1106
+            for (int i = 0; i < n; i++) {
+                const int es = src[i], v = Ls[i];
+                for (int j = 0; j < 4; j++) {
+                    int f, gg, vv, pp;
+                    rec(es, v, j, f, gg, vv, pp);
+                    if (f != FIX_REMOVE || gg != g) continue;
+                    for (int q = 0; q < n; q++)
+                        if (Ls[q] == vv) { doomed[q >> 6] |= 1ull << (q & 63); break; }
+                }
+            }
+            int w = 0;
+            for (int i = 0; i < n; i++)
+                if (!((doomed[i >> 6] >> (i & 63)) & 1ull)) { Ls[w] = Ls[i]; src[w] = src[i]; w++; }
+            n = w;
+            for (int i = 0; i < n; i++) {                            // REPLACE, :1124-1136
+                const int es = src[i], v = Ls[i];
+                for (int j = 0; j < 4; j++) {
+                    int f, gg, vv, pp;
+                    rec(es, v, j, f, gg, vv, pp);
+                    if (f != FIX_REPLACE || gg != g) continue;
+                    for (int q = 0; q < n; q++)                      // grain_replace_vertex(pp -> vv)
+                        if (Ls[q] == pp) { Ls[q] = vv; src[q] = -1; break; }
+                }
+            }
+            n_add_s = n_add;
+            n_fin_s = n;
+        }
+        __syncthreads();
+        const int n_add = n_add_s, n_fin = n_fin_s;
+        for (int i = lane; i < n_add; i += 64) L[i] = i < n_fin ? Ls[i] : -1;
+        if (lane == 0) d.glen[g] = n_fin;
+        __syncthreads();
+    }
 }
 
 // clear_grains_fix_state (topological.h:1150-1163) for the flagged grains,
@@ -1331,28 +1428,6 @@ __global__ void k_rate_cand(Dev d) {
     d.l_nucl[s] = i;
 }
 
-// Per table block: numbers of enabled and of free slots (for the slot
-// searches of the nucleation manager).
-__global__ void k_block_counts(Dev d, int kind) {
-    const Blk* tbl = kind == 0 ? d.vblk : (kind == 1 ? d.eblk : d.gblk);
-    const int nblk = kind == 0 ? d.nvb : (kind == 1 ? d.neb : d.ngb);
-    const uint8_t* en = kind == 0 ? d.ven : (kind == 1 ? d.benab : d.genab);
-    int2* out = kind == 0 ? d.bc_v : (kind == 1 ? d.bc_e : d.bc_g);
-    int lb = blockIdx.x;
-    if (lb >= nblk) return;
-    Blk bd = tbl[lb];
-    if (!d.ms[bd.m].nucl_now) return;
-    int i = bd.b + (int)threadIdx.x;
-    int c = (i < bd.e && en[i]) ? 1 : 0;
-    __shared__ int acc;
-    if (threadIdx.x == 0) acc = 0;
-    __syncthreads();
-    unsigned long long bal = __ballot(c);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&acc, (int)__popcll(bal));
-    __syncthreads();
-    if (threadIdx.x == 0) out[lb] = make_int2(acc, (bd.e - bd.b) - acc);
-}
-
 // Exclusive prefix sum of one int per thread over the workgroup (LDS).
 // Returns the prefix; *total receives the sum.  All threads must call it.
 __device__ int wg_exclusive_scan(int x, int* sh, int* total) {
@@ -1421,12 +1496,16 @@ __device__ int collect_slots(const Blk* tbl, int first, int nblk, const int2* bc
     __syncthreads();
     if (rs.cnt > 0 && rs.pre < R) {
         int rank = rs.pre;
-        for (int b = rs.lo; b < rs.hi && rank < R; b++) {
-            const int c = block_matches(bc, b, want);
-            if (c > 0) {
-                blist[atomicAdd(&nl, 1)] = make_int2(b, rank);
-                rank += c;
-            }
+        for (int b0 = rs.lo; b0 < rs.hi && rank < R; b0 += 8) {
+            int t[8];                              // 8 independent loads, then in order
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = (b0 + q < rs.hi) ? block_matches(bc, b0 + q, want) : 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (t[q] > 0 && rank < R) {
+                    blist[atomicAdd(&nl, 1)] = make_int2(b0 + q, rank);
+                    rank += t[q];
+                }
         }
     }
     __syncthreads();
@@ -1434,16 +1513,23 @@ __device__ int collect_slots(const Blk* tbl, int first, int nblk, const int2* bc
     const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int q = wave; q < n; q += nw) {
+        // one wave per listed block: the block's flags are loaded at once
+        // (kBlock/64 per lane), then ranked 64 slots at a time
         const int2 e = blist[q];
-        const int hi = tbl[e.x].e;
+        const int lo = tbl[e.x].b, hi = tbl[e.x].e;
+        bool hit[kBlock / 64];
+#pragma unroll
+        for (int k = 0; k < kBlock / 64; k++) {
+            const int idx = lo + 64 * k + lane;
+            hit[k] = idx < hi && ((en[idx] != 0) == (want != 0));
+        }
         int rank = e.y;
-        for (int i = tbl[e.x].b; i < hi && rank < R; i += 64) {
-            const int idx = i + lane;
-            const bool hit = idx < hi && ((en[idx] != 0) == (want != 0));
-            const unsigned long long hb = __ballot(hit);
-            if (hit) {
+#pragma unroll
+        for (int k = 0; k < kBlock / 64; k++) {
+            const unsigned long long hb = __ballot(hit[k]);
+            if (hit[k]) {
                 const int r = rank + (int)__popcll(hb & below);
-                if (r < R) out[r] = idx;
+                if (r < R) out[r] = lo + 64 * k + lane;
             }
             rank += (int)__popcll(hb);
         }
@@ -1542,10 +1628,17 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
     __syncthreads();
     if (k >= rs.pre && k < rs.pre + rs.cnt) {
         int rank = rs.pre;
-        for (int b = rs.lo; b < rs.hi; b++) {
-            const int c = block_matches(bc, b, want);
-            if (k < rank + c) { fb = b; fr = k - rank; break; }
-            rank += c;
+        bool found_b = false;
+        for (int b0 = rs.lo; b0 < rs.hi && !found_b; b0 += 8) {
+            int t[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = (b0 + q < rs.hi) ? block_matches(bc, b0 + q, want) : 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (!found_b) {
+                    if (k < rank + t[q]) { fb = b0 + q; fr = k - rank; found_b = true; }
+                    else rank += t[q];
+                }
         }
     }
     __syncthreads();
@@ -1581,9 +1674,12 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
 //  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
 //  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free
 //  vertices, [11ng,12ng) candidate sites.
+constexpr int kNuclLds = 1024;   // rate-mode candidates held in LDS
+
 __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
     __shared__ int sh[257];
     __shared__ int nkeep, nmade, fg_n, fe_n, fv_n, ncand;
+    __shared__ int c_s[kNuclLds], site_s[kNuclLds], sg_s[3 * kNuclLds], k_s[kNuclLds];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
@@ -1621,7 +1717,9 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             if (d.vnuc[A]) { kept[0] = A; nkeep = 1; }
         }
     } else {
-        // this member's candidates, then grain order
+        // this member's candidates, then grain order.  Up to kNuclLds of
+        // them are handled in LDS (c_s, site_s, sg_s, k_s); beyond that the
+        // same code runs on the global scratch.
         const int total = d.cnt[C_NUCL];
         for (int q = tid; q < total; q += nt) {
             int g = d.l_nucl[q];
@@ -1629,18 +1727,23 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
         }
         __syncthreads();
         const int nc = ncand;
-        // rank sort (distinct grain ids), through the kept region
+        const bool small = nc <= kNuclLds;
+        int* uns = small ? k_s : kept;            // unsorted copy, later the keep flags
+        int* srt = small ? c_s : cand;            // candidates in grain order
+        int* ss = small ? site_s : site;
+        int* g3 = small ? sg_s : sgr;
+        for (int e = tid; e < nc; e += nt) uns[e] = cand[e];
+        __syncthreads();
+        // rank sort (distinct grain ids)
         for (int e = tid; e < nc; e += nt) {
-            const int g = cand[e];
+            const int g = uns[e];
             int r = 0;
-            for (int f = 0; f < nc; f++) r += cand[f] < g ? 1 : 0;
-            kept[r] = g;
+            for (int f = 0; f < nc; f++) r += uns[f] < g ? 1 : 0;
+            srt[r] = g;
         }
         __syncthreads();
-        for (int e = tid; e < nc; e += nt) cand[e] = kept[e];
-        __syncthreads();
         for (int e = tid; e < nc; e += nt) {
-            const int g = cand[e];
+            const int g = srt[e];
             double u0, u1;
             sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
             const int len = d.glen[g];
@@ -1650,38 +1753,44 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             double minAL = d.blen[d.vb[3 * A]];
             for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
             // a site on a zero-length boundary is not a candidate
-            site[e] = (minAL > 0) ? A : -1;
-            for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
+            ss[e] = (minAL > 0) ? A : -1;
+            for (int a = 0; a < 3; a++) g3[3 * e + a] = d.vg[3 * A + a];
         }
         __syncthreads();
         // kept iff no earlier candidate touches one of its grains
         for (int e = tid; e < nc; e += nt) {
-            bool clash = site[e] < 0;
+            bool clash = ss[e] < 0;
             for (int f = 0; f < e && !clash; f++) {
-                if (site[f] < 0) continue;
+                if (ss[f] < 0) continue;
                 for (int a = 0; a < 3 && !clash; a++)
                     for (int b = 0; b < 3; b++)
-                        if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
+                        if (g3[3 * e + a] == g3[3 * f + b]) { clash = true; break; }
             }
-            kept[e] = clash ? 0 : 1;
+            uns[e] = clash ? 0 : 1;
         }
         __syncthreads();
-        if (tid == 0) {
-            int w = 0;
-            for (int e = 0; e < nc; e++) {
-                if (!kept[e]) continue;
-                cand[w] = cand[e];
-                site[w] = site[e];
-                w++;
+        // stable compaction of the kept candidates, one workgroup scan per
+        // nt candidates (values are read before the scan, written after)
+        int off = 0;
+        for (int b0 = 0; b0 < nc; b0 += nt) {
+            const int e = b0 + tid;
+            const int flag = e < nc ? uns[e] : 0;
+            const int g = flag ? srt[e] : 0, A = flag ? ss[e] : 0;
+            int tot;
+            const int pre = wg_exclusive_scan(flag, sh, &tot);
+            if (flag) {
+                srt[off + pre] = g;
+                ss[off + pre] = A;
             }
-            nkeep = w;
+            off += tot;
+            __syncthreads();
         }
+        if (tid == 0) nkeep = off;
         __syncthreads();
         for (int r = tid; r < nkeep; r += nt) {
-            const int g = cand[r], A = site[r];
-            double u0, u1, u2, u3;
+            const int g = srt[r], A = ss[r];
+            double u2, u3;
             sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
-            (void)u0; (void)u1;
             double minAL = d.blen[d.vb[3 * A]];
             for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
             d.nucl_r[A] = P.nucl_scaling * minAL * u2;

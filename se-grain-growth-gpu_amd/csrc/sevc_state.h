@@ -70,7 +70,6 @@ struct Dev {
     int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
     double* partial;                      // per grain block partial sums (2 levels)
     unsigned long long* bmin;             // per edge block min t_ext bits
-    int* bcnt_g; int* bcnt_v;             // per block enabled counts
     int* nscr;                            // nucleation scratch, 12 ints per grain slot
     int diag;                             // compute export diagnostics this step
 };
