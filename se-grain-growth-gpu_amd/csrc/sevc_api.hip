@@ -212,7 +212,7 @@ void launch_step(sevc_state* s, bool diag) {
     launch(s, "text", k_text, d.neb, kBlock, d);
     if (s->any_adaptive) {
         launch(s, "tau_partial", k_tau_partial, d.neb, kBlock, d);
-        launch(s, "member_reduce", k_member_reduce, s->B, 256, d, 0);
+        launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 0);
     }
     if (s->any_events) {
         launch(s, "label", k_label, d.neb, kBlock, d);
@@ -236,7 +236,7 @@ void launch_step(sevc_state* s, bool diag) {
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);
-    launch(s, "member_reduce", k_member_reduce, s->B, 256, d, 1);
+    launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 1);
     launch(s, "end", k_end, nb, 64, d);
 }
 

@@ -93,9 +93,9 @@ __global__ void k_count_all(Dev d) {
 
 // One workgroup per member: sums / minimises the per-block partials.
 // what = 0: tau (adaptive dt); 1: grain and vertex counts.
-__global__ void __launch_bounds__(256) k_member_reduce(Dev d, int what) {
-    __shared__ unsigned long long sm[256];
-    __shared__ long long sa[256], sb[256];
+__global__ void __launch_bounds__(1024) k_member_reduce(Dev d, int what) {
+    __shared__ unsigned long long sm[1024];
+    __shared__ long long sa[1024], sb[1024];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     MState& M = d.ms[m];
@@ -103,22 +103,25 @@ __global__ void __launch_bounds__(256) k_member_reduce(Dev d, int what) {
     const int tid = threadIdx.x;
     if (what == 0) {
         unsigned long long v = 0x7FF0000000000000ull;
+#pragma unroll 4
         for (int b = d.eblk_first[m] + tid; b < d.eblk_first[m + 1]; b += blockDim.x) v = d.bmin[b] < v ? d.bmin[b] : v;
         sm[tid] = v;
         __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
             if (tid < st && sm[tid + st] < sm[tid]) sm[tid] = sm[tid + st];
             __syncthreads();
         }
         if (tid == 0) M.tau_bits = sm[0];
     } else {
         long long a = 0, b2 = 0;
+#pragma unroll 4
         for (int b = d.gblk_first[m] + tid; b < d.gblk_first[m + 1]; b += blockDim.x) a += d.bc_g[b].x;
+#pragma unroll 4
         for (int b = d.vblk_first[m] + tid; b < d.vblk_first[m + 1]; b += blockDim.x) b2 += d.bc_v[b].x;
         sa[tid] = a;
         sb[tid] = b2;
         __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
             if (tid < st) { sa[tid] += sa[tid + st]; sb[tid] += sb[tid + st]; }
             __syncthreads();
         }
@@ -859,23 +862,47 @@ __global__ void k_area_list(Dev d, int which) {
 // ---------------------------------------------------------------------
 
 // Edges whose two grains are both 3-sided, found from the 3-sided grains.
+// For edge (a, c) of 3-sided grain i the two grains are vg(a) ∩ vg(c), the
+// set grains_intersect forms from the edge's ends; the edge is listed once,
+// by the lower of the two grains.  Loads are issued in rounds (list, then
+// the vertices' edges and grains, then the edges' ends and flags, then the
+// neighbours' list lengths).
 __global__ void k_two_scan(Dev d) {
     TABLE_BLOCK(d.gblk, d.ngb)
     if (!M.active || !P.events || i >= bd_.e) return;
     if (!d.genab[i] || d.glen[i] != 3) return;
     const int* L = glist(d, i);
+    const int vv[3] = {L[0], L[1], L[2]};
+    int bb[9], gv[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        bb[e] = d.vb[3 * vv[e / 3] + e % 3];
+        gv[e] = d.vg[3 * vv[e / 3] + e % 3];
+    }
+    int oe[9];
+    uint8_t en[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        const int b0 = d.bvx[2 * bb[e]], b1 = d.bvx[2 * bb[e] + 1];
+        oe[e] = (b0 == vv[e / 3]) ? b1 : b0;
+        en[e] = d.benab[bb[e]];
+    }
     for (int k = 0; k < 3; k++) {
-        int a = L[k], c = L[(k + 1) % 3];
+        const int kc = (k + 1) % 3;
+        const int c = vv[kc];
         for (int q = 0; q < 3; q++) {
-            int e = d.vb[3 * a + q];
-            if (other_end(d, e, a) != c || !d.benab[e]) continue;
-            int gic[2];
-            if (grains_intersect(d, d.bvx[2 * e], d.bvx[2 * e + 1], gic) < 2) continue;
+            const int e = 3 * k + q;
+            if (oe[e] != c || !en[e]) continue;
+            int gic[2] = {-1, -1}, cnt = 0;
+            for (int x = 0; x < 3 && cnt < 2; x++)
+                for (int y = 0; y < 3; y++)
+                    if (gv[3 * k + x] == gv[3 * kc + y]) { gic[cnt++] = gv[3 * k + x]; break; }
+            if (cnt < 2) continue;
             if (d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3) {
                 int lo = gic[0] < gic[1] ? gic[0] : gic[1];
                 if (lo == i) {
                     int s = atomicAdd(&d.cnt[C_TWO], 1);
-                    d.l_two[s] = e;
+                    d.l_two[s] = bb[e];
                 }
             }
         }
@@ -1022,21 +1049,44 @@ __global__ void k_label3(Dev d) {
     const int g = i;
     uint8_t cand = 0;
     if (d.glen[g] == 3) {
+        // every datum of the 9 (vertex, edge) visits is loaded up front (three
+        // rounds of independent loads), then the visits run in the order of
+        // label_3sided_grains; "in g" is membership in the 3-entry list
+        // (grain_contains_vertex)
         const double dt = member_dt(P, M);
         const int* L = glist(d, g);
+        const int v0 = L[0], v1 = L[1], v2 = L[2];
+        const int vv[3] = {v0, v1, v2};
+        int bb[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) bb[e] = d.vb[3 * vv[e / 3] + e % 3];
+        int ini[9], end[9];
+        uint8_t bc[9], be[9];
+        double te[9], bl[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const int b = bb[e];
+            ini[e] = d.bvx[2 * b];
+            end[e] = d.bvx[2 * b + 1];
+            bc[e] = d.bcand[b];
+            be[e] = d.benab[b];
+            te[e] = d.bt[b];
+            bl[e] = d.blen[b];
+        }
+        auto in_g = [&](int x) { return x == v0 || x == v1 || x == v2; };
+        // label_3sided_grains stops counting a vertex's edges at the first
+        // labelled one, but then the grain is a candidate whatever the count,
+        // so counting every visit gives the same label
+        bool hit = false;
         int ans = 0;
-        for (int q = 0; q < 3; q++) {
-            int v = L[q];
-            for (int k = 0; k < 3; k++) {
-                int b = d.vb[3 * v + k];
-                int ip = grain_contains_vertex(d, g, d.bvx[2 * b]);
-                int ep = grain_contains_vertex(d, g, d.bvx[2 * b + 1]);
-                if (ip >= 0 && ep >= 0 && d.bcand[b] && d.benab[b]) { cand = 1; break; }
-                if (ip >= 0 && ep >= 0 && is_candidate(d.bt[b], d.blen[b], dt, P.flip_thr) && d.benab[b]) ans++;
-            }
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const bool inn = in_g(ini[e]) && in_g(end[e]);
+            hit = hit || (inn && bc[e] && be[e]);
+            ans += (inn && is_candidate(te[e], bl[e], dt, P.flip_thr) && be[e]) ? 1 : 0;
         }
         ans /= 2;
-        if (ans == 3) cand = 1;
+        if (hit || ans == 3) cand = 1;
     }
     d.gcand[g] = cand;
     if (cand) {
@@ -1289,8 +1339,19 @@ __global__ void k_nucl_area_l1(Dev d) {
     if (lb >= d.ngb) return;
     Blk bd = d.gblk[lb];
     double s = 0.0;
-    for (int g = bd.b; g < bd.e; g++)
-        if (d.genab[g] && d.gtype[g] == T_NUCLEATED) s += d.garea[g];
+    for (int g0 = bd.b; g0 < bd.e; g0 += 8) {       // 8 grains' loads at a time, summed in order
+        double a[8];
+        bool ok[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int g = g0 + q;
+            ok[q] = g < bd.e && d.genab[g] && d.gtype[g] == T_NUCLEATED;
+            a[q] = ok[q] ? d.garea[g] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (ok[q]) s += a[q];
+    }
     d.partial[lb] = s;
 }
 
