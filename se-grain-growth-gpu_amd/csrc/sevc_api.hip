@@ -17,7 +17,6 @@ namespace sevc {
 // kernels of sevc_kernels.hip
 __global__ void k_begin(Dev d);
 __global__ void k_count_all(Dev d);
-__global__ void k_tau_partial(Dev d);
 __global__ void k_member_reduce(Dev d, int what);
 __global__ void k_end(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
@@ -211,7 +210,6 @@ void launch_step(sevc_state* s, bool diag) {
     if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
     launch(s, "text", k_text, d.neb, kBlock, d);
     if (s->any_adaptive) {
-        launch(s, "tau_partial", k_tau_partial, d.neb, kBlock, d);
         launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 0);
     }
     if (s->any_events) {

@@ -393,7 +393,10 @@ __global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
         const double nbx = d.pos[2 * nb], nby = d.pos[2 * nb + 1];
         const double lx = arclen(px, pyy, nbx, nby, D);
         const double ly = arclen(qx, qy, nbx, nby, D);
-        const double L0 = d.blen[b];
+        // the stored arclength of b (compute_boundary_arclengths at the step's
+        // positions, refreshed for edges moved by nucleation): wrap_dist is odd
+        // and vector2_mag sign-symmetric, so |wrap(p_v - p_nb)| has its bits
+        const double L0 = arclen(x, y, nbx, nby, D);
         const double gam = uni ? P.gb_scaling : d.ben[b];
         bx += (gam * (lx - L0));
         by += (gam * (ly - L0));
@@ -485,48 +488,42 @@ __global__ void k_translate(Dev d) {
 // topological.h: extinction times, labels, votes
 // ---------------------------------------------------------------------
 
-// compute_t_ext, topological.h:303-334, plus the adaptive-dt reduction
-// (smallest positive t_ext of edges not shorter than the flip threshold).
+// compute_t_ext, topological.h:303-334, fused with the adaptive-dt partial:
+// the smallest positive t_ext of the block's enabled edges not shorter than
+// the flip threshold, one value per table block (no contended atomics),
+// reduced per member by k_member_reduce.  The minimum does not depend on
+// the order, so the result equals a serial scan.
 __global__ void k_text(Dev d) {
-    TABLE_BLOCK(d.eblk, d.neb)
-    if (!M.active || i >= bd_.e || !d.benab[i]) return;
-    const double D = P.D;
-    d.bcand[i] = 0;
-    int a = d.bvx[2 * i], b = d.bvx[2 * i + 1];
-    // adjust_origin_for_points on (ini, end): U0 = 0, U1 = 0 + wrap(end - ini)
-    double x1 = 0 + wrapd(d.pos[2 * b], d.pos[2 * a], D);
-    double y1 = 0 + wrapd(d.pos[2 * b + 1], d.pos[2 * a + 1], D);
-    double dx = x1 - 0, dy = y1 - 0;
-    double norm = mag(dx, dy);
-    double tx, ty;
-    if (norm == 0) { tx = 0; ty = 0; }
-    else {
-        double ang = sevc_atan2(dy, dx);
-        tx = sevc_cos(ang);
-        ty = sevc_sin(ang);
-    }
-    double px = d.vel[2 * b] - d.vel[2 * a];
-    double py = d.vel[2 * b + 1] - d.vel[2 * a + 1];
-    double aux = tx * px + ty * py;
-    double te = (aux == 0) ? 100.0 : -norm / aux;
-    d.bt[i] = te;
-}
-
-// Adaptive dt: smallest positive t_ext over enabled edges not shorter than
-// the flip threshold; one partial per table block (no contended atomics),
-// reduced per member by k_member_reduce.
-__global__ void k_tau_partial(Dev d) {
     __shared__ unsigned long long wmin[kBlock / 64];
-    int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (lb_ >= d.neb) return;
     const Blk bd_ = d.eblk[lb_];
     const Params& P = d.prm[bd_.m];
     const MState& M = d.ms[bd_.m];
     const int i = bd_.b + (int)threadIdx.x;
     unsigned long long v = 0x7FF0000000000000ull;
-    if (M.active && P.adaptive && i < bd_.e && d.benab[i]) {
-        double te = d.bt[i];
-        if (te > 0 && !(d.blen[i] < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
+    if (M.active && i < bd_.e && d.benab[i]) {
+        const double D = P.D;
+        d.bcand[i] = 0;
+        int a = d.bvx[2 * i], b = d.bvx[2 * i + 1];
+        // adjust_origin_for_points on (ini, end): U0 = 0, U1 = 0 + wrap(end - ini)
+        double x1 = 0 + wrapd(d.pos[2 * b], d.pos[2 * a], D);
+        double y1 = 0 + wrapd(d.pos[2 * b + 1], d.pos[2 * a + 1], D);
+        double dx = x1 - 0, dy = y1 - 0;
+        double norm = mag(dx, dy);
+        double tx, ty;
+        if (norm == 0) { tx = 0; ty = 0; }
+        else {
+            double ang = sevc_atan2(dy, dx);
+            tx = sevc_cos(ang);
+            ty = sevc_sin(ang);
+        }
+        double px = d.vel[2 * b] - d.vel[2 * a];
+        double py = d.vel[2 * b + 1] - d.vel[2 * a + 1];
+        double aux = tx * px + ty * py;
+        double te = (aux == 0) ? 100.0 : -norm / aux;
+        d.bt[i] = te;
+        if (P.adaptive && te > 0 && !(d.blen[i] < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
     }
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(v, off);
