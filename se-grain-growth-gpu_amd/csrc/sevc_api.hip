@@ -195,7 +195,7 @@ void launch_step(sevc_state* s, bool diag) {
     if (s->any_nucl) {
         launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
         launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
-        launch(s, "nucl_manager", k_nucl_manager, s->B, 256, d);
+        launch(s, "nucl_manager", k_nucl_manager, s->B, kNuclThreads, d);
         launch(s, "nucl_refresh", k_nucl_refresh, kListGrid, 256, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     }

@@ -1486,22 +1486,34 @@ __global__ void k_rate_cand(Dev d) {
     d.l_nucl[s] = i;
 }
 
-// Exclusive prefix sum of one int per thread over the workgroup (LDS).
-// Returns the prefix; *total receives the sum.  All threads must call it.
+// Exclusive prefix sum of one int per thread over the workgroup: an
+// in-wave shuffle scan, the wave totals scanned by wave 0, two barriers.
+// Returns the prefix; *total receives the sum.  All threads must call it;
+// blockDim.x is a multiple of 64 (at most 1024) and `sh` holds >= 65 ints.
 __device__ int wg_exclusive_scan(int x, int* sh, int* total) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    sh[tid] = x;
-    __syncthreads();
-    for (int off = 1; off < nt; off <<= 1) {
-        int y = tid >= off ? sh[tid - off] : 0;
-        __syncthreads();
-        sh[tid] += y;
-        __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
+    int inc = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
     }
-    int incl = sh[tid];
-    *total = sh[nt - 1];
+    if (lane == 63) sh[wave] = inc;
     __syncthreads();
-    return incl - x;
+    if (wave == 0) {
+        const int v = lane < nw ? sh[lane] : 0;
+        int vi = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(vi, o, 64);
+            if (lane >= o) vi += t;
+        }
+        if (lane < nw) sh[lane] = vi - v;
+        if (lane == nw - 1) sh[64] = vi;
+    }
+    __syncthreads();
+    const int r = sh[wave] + inc - x;
+    *total = sh[64];
+    __syncthreads();
+    return r;
 }
 
 __device__ __forceinline__ int block_matches(const int2* bc, int b, int want) {
@@ -1732,12 +1744,13 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
 //  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
 //  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free
 //  vertices, [11ng,12ng) candidate sites.
-constexpr int kNuclLds = 1024;   // rate-mode candidates held in LDS
+constexpr int kNuclLds = 1024;       // rate-mode candidates held in LDS
 
-__global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
-    __shared__ int sh[257];
+__global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
+    __shared__ int sh[kNuclThreads + 1];
     __shared__ int nkeep, nmade, fg_n, fe_n, fv_n, ncand;
     __shared__ int c_s[kNuclLds], site_s[kNuclLds], sg_s[3 * kNuclLds], k_s[kNuclLds];
+    __shared__ double al_s[kNuclLds];         // shortest edge at each candidate's site
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
@@ -1781,7 +1794,11 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
         const int total = d.cnt[C_NUCL];
         for (int q = tid; q < total; q += nt) {
             int g = d.l_nucl[q];
-            if (g >= P.goff && g < P.goff + ng) cand[atomicAdd(&ncand, 1)] = g;
+            if (g >= P.goff && g < P.goff + ng) {
+                const int e = atomicAdd(&ncand, 1);
+                cand[e] = g;
+                if (e < kNuclLds) k_s[e] = g;
+            }
         }
         __syncthreads();
         const int nc = ncand;
@@ -1790,8 +1807,11 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
         int* srt = small ? c_s : cand;            // candidates in grain order
         int* ss = small ? site_s : site;
         int* g3 = small ? sg_s : sgr;
-        for (int e = tid; e < nc; e += nt) uns[e] = cand[e];
-        __syncthreads();
+        double* al = small ? al_s : nullptr;      // beyond kNuclLds: recomputed when kept
+        if (!small) {
+            for (int e = tid; e < nc; e += nt) uns[e] = cand[e];
+            __syncthreads();
+        }
         // rank sort (distinct grain ids)
         for (int e = tid; e < nc; e += nt) {
             const int g = uns[e];
@@ -1812,6 +1832,7 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
             // a site on a zero-length boundary is not a candidate
             ss[e] = (minAL > 0) ? A : -1;
+            if (al) al[e] = minAL;
             for (int a = 0; a < 3; a++) g3[3 * e + a] = d.vg[3 * A + a];
         }
         __syncthreads();
@@ -1834,11 +1855,13 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             const int e = b0 + tid;
             const int flag = e < nc ? uns[e] : 0;
             const int g = flag ? srt[e] : 0, A = flag ? ss[e] : 0;
+            const double la = (flag && al) ? al[e] : 0.0;
             int tot;
             const int pre = wg_exclusive_scan(flag, sh, &tot);
             if (flag) {
                 srt[off + pre] = g;
                 ss[off + pre] = A;
+                if (al) al[off + pre] = la;
             }
             off += tot;
             __syncthreads();
@@ -1849,8 +1872,13 @@ __global__ void __launch_bounds__(256) k_nucl_manager(Dev d) {
             const int g = srt[r], A = ss[r];
             double u2, u3;
             sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
-            double minAL = d.blen[d.vb[3 * A]];
-            for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+            double minAL;
+            if (al) {
+                minAL = al[r];
+            } else {
+                minAL = d.blen[d.vb[3 * A]];
+                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+            }
             d.nucl_r[A] = P.nucl_scaling * minAL * u2;
             d.nucl_ori[A] = 2 * SEVC_PI * u3;
             kept[r] = A;

@@ -15,6 +15,7 @@ constexpr int kBlock = 256;          // elements per table block (one workgroup)
 constexpr int kFixStages = 3;        // 0: two-grain, 1: T2 removal, 2: flips
 constexpr int kTile = 64;            // grains per velocity tile (pass 1)
 constexpr int kMaxHistBins = 64;     // bins of the device area histogram
+constexpr int kNuclThreads = 1024;   // workgroup of the nucleation manager (one per member)
 
 // Per-member run parameters (from sevc_options).
 struct Params {
