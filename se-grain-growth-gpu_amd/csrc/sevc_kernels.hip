@@ -258,10 +258,11 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
 // area the reference's left-to-right shoelace sum, so the results are
 // bit-identical.  Pass 2 sends vertices with dom_mod(x) != x or
 // dom_mod(y) != y to the two-pass path, which this form does not cover.
-constexpr int kTilePts = 1024;
+constexpr int kTilePts = 512;
 static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
 __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
     __shared__ double2 Ps[kTilePts];
+    __shared__ double2 Ds[kTilePts];        // wrap(P_k - P_{k-1}) of list slot k > 0
     __shared__ int Ls[kTilePts];
     __shared__ unsigned char pgl[kTilePts];
     __shared__ int off[kTile + 1];
@@ -307,15 +308,30 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
             Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
         }
         __syncthreads();
+        for (int p = tid; p < npts; p += nt) {
+            // the unwrap deltas of the unperturbed polygon, shared by its area
+            // and by every shifted area of the grain (the same wrap_dist calls
+            // the sequential unwrap makes)
+            const int b0 = off[pgl[p]];
+            Ds[p] = (p > b0) ? make_double2(wrapd(Ps[p].x, Ps[p - 1].x, D), wrapd(Ps[p].y, Ps[p - 1].y, D))
+                             : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
         for (int g = tid; g < ngr; g += nt) {
             const int b0 = off[g], n = off[g + 1] - b0;
-            double a = 0.0;
-            if (n > 0) {
-                AreaAcc acc;
-                acc.start(Ps[b0].x, Ps[b0].y);
-                for (int k = 1; k < n; k++) acc.push(Ps[b0 + k].x, Ps[b0 + k].y, D);
-                a = acc.finish();
+            // compute_area from the deltas: U_k = U_{k-1} + D_k, left to right
+            double ux = 0, uy = 0, a = 0;
+            for (int k = 1; k < n; k++) {
+                const double2 du = Ds[b0 + k];
+                const double nx = ux + du.x, ny = uy + du.y;
+                a += ux * ny - uy * nx;
+                ux = nx;
+                uy = ny;
             }
+            const double z = 0;
+            a += ux * z - uy * z;
+            a *= 0.5;
+            if (n <= 0) a = 0.0;
             if (d.genab[gs + g]) d.garea[gs + g] = a;
             area_s[g] = a;
         }
@@ -334,13 +350,10 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
                 exk1 = wrapd(Ps[p + 1].x, px, D);
                 eyk1 = wrapd(Ps[p + 1].y, qy, D);
             }
-            double prx = Ps[b0].x, pry = Ps[b0].y;
             double ux = 0, uy = 0, xx = 0, yy = 0, sx = 0, sy = 0;
             for (int i = 1; i < n; i++) {
-                const double cx = Ps[b0 + i].x, cy = Ps[b0 + i].y;
-                const double dux = wrapd(cx, prx, D), duy = wrapd(cy, pry, D);
-                prx = cx;
-                pry = cy;
+                const double2 du = Ds[b0 + i];
+                const double dux = du.x, duy = du.y;
                 const double dxx = (i == k) ? exk : ((i == k + 1) ? exk1 : dux);
                 const double dyy = (i == k) ? eyk : ((i == k + 1) ? eyk1 : duy);
                 const double nux = ux + dux, nuy = uy + duy;
