@@ -96,6 +96,7 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
     grains = sum(o.n_grains for o in opts)
     log(f"ensemble: {members} members, {grains} grains ({time.time() - t0:.1f} s)")
     st = sevc.State(opts, meshes)
+    tile = st.autotune()
     st.step(warmup)
     st.step(0)
     t1 = time.perf_counter()
@@ -121,6 +122,7 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
     log(f"ensemble: batched {batched:.3e} vertex-steps/s over {steps} steps; back-to-back {b2b:.3e} "
         f"over {b2b_steps} steps per member")
     return {"value": batched, "unit": "vertex-steps/s", "members": members, "grains": grains, "steps": steps,
+            "tile_grains": tile,
             "ms_per_step": (t2 - t1) / max(1, s.steps_done) * 1e3,
             "back_to_back": {"value": b2b, "steps_per_member": b2b_steps, "seconds": el},
             "speedup": batched / b2b,
@@ -369,6 +371,7 @@ def main():
     opts, mesh = make_mesh(sevc, args.grains, args.seed)
     log(f"mesh: {mesh.nv} vertex slots, {mesh.ne} boundary slots, {mesh.ng} grain slots ({time.time() - t0:.1f} s)")
     st = sevc.State(opts, mesh)
+    tile = st.autotune()
     st.step(args.warmup, graph=bool(args.graph))
     st.enable_timing(2)           # HIP events around velocity / translate / text only
     sync = st.step(0)
@@ -408,7 +411,7 @@ def main():
         "config": {"workload": f"plain step, periodic Voronoi of {args.grains} seeds (seed {args.seed}), "
                                "gamma=1, mu=1, lognormal SE (mean 0.3, sigma 0.5), nucleation 1e-4/grain/step, "
                                "adaptive dt, flip threshold 1e-4",
-                   "grains": args.grains, "graph": bool(args.graph)},
+                   "grains": args.grains, "graph": bool(args.graph), "tile_grains": tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_VELOCITY_B_PER_VERTEX * verts_per_step

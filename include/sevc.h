@@ -215,6 +215,15 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area);
 int sevc_export_snapshot(const sevc_mesh* m, const sevc_options* o, const char* folder,
                          int32_t index, double time);
 
+/* Read a snapshot written by sevc_export_snapshot back into a newly
+ * allocated mesh (grain list stride max_vrt_per_grn), for resuming a run;
+ * *time receives its TIME.  Values are the snapshot's %.16f text, so
+ * positions come back rounded to 16 decimals.  The format lists the other
+ * end of each vertex boundary, not its id, so vertex.boundaries of enabled
+ * vertices is rebuilt from the enabled boundaries in that neighbour order;
+ * disabled vertices get none. */
+int sevc_load_snapshot(const char* path, int32_t max_vrt_per_grn, sevc_mesh* out, double* time);
+
 /* ---------------------------------------------------------------- */
 /* Device state                                                      */
 /* ---------------------------------------------------------------- */
@@ -248,6 +257,12 @@ int32_t sevc_n_members(const sevc_state* s);
 /* Copy a member back to host arrays sized by sevc_member_sizes
  * (device_to_host, translation.h:184-199, without freeing the device copy). */
 int sevc_get_mesh(sevc_state* s, int32_t member, sevc_mesh* out);
+
+/* Pick the tile size of the velocity's grain pass (grains per workgroup,
+ * 16..64) for this state by timing that pass on the current positions
+ * (extension; no counterpart in the reference).  Results do not depend on
+ * the choice; the state is left unchanged.  tile_grains may be NULL. */
+int sevc_autotune(sevc_state* s, int32_t* tile_grains);
 
 /* Copy only the vertex positions of a member (2 per vertex slot). */
 int sevc_get_positions(sevc_state* s, int32_t member, double* pos);

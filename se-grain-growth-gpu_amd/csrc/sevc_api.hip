@@ -18,11 +18,10 @@ namespace sevc {
 __global__ void k_begin(Dev d);
 __global__ void k_count_all(Dev d);
 __global__ void k_member_reduce(Dev d, int what);
-__global__ void k_end(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
-__global__ void k_vel_grain(Dev d);
+__global__ void k_vel_grain(Dev d, int tuning);
 __global__ void k_vel_vertex(Dev d);
 __global__ void k_velocity_generic(Dev d);
 __global__ void k_velocity_fallback(Dev d);
@@ -34,15 +33,12 @@ __global__ void k_label(Dev d);
 __global__ void k_poll(Dev d);
 __global__ void k_fix(Dev d, int stage);
 __global__ void k_fix_clear(Dev d, int stage, int with_area);
-__global__ void k_area_list(Dev d, int which);
 __global__ void k_two_scan(Dev d);
 __global__ void k_two_grain(Dev d);
 __global__ void k_label3(Dev d);
-__global__ void k_t2_split(Dev d);
 __global__ void k_t2_par(Dev d);
 __global__ void k_t2_serial(Dev d);
 __global__ void k_st2(Dev d);
-__global__ void k_flip_split(Dev d);
 __global__ void k_flip_par(Dev d);
 __global__ void k_flip_serial(Dev d);
 __global__ void k_nucl_area_l1(Dev d);
@@ -99,6 +95,9 @@ struct sevc_state {
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::map<std::string, std::pair<double, long long>> ktimes;
+    // grains per tile of velocity pass 1 (sevc_autotune), table capacity
+    int tile_grains = kTile;
+    size_t tile_cap = 0;
     // topology-fixed window: checkpoints of x every K steps (ck), the
     // positions and areas of one segment (seg, segA; `holds` = its index),
     // and the reverse-pass arrays
@@ -182,6 +181,27 @@ void collect_timers(sevc_state* s) {
 }
 
 constexpr int kListGrid = 512;
+constexpr int kTileMin = 16;             // smallest tile the autotuner tries
+
+// Tile table of velocity pass 1: T consecutive grains of one member per
+// tile.  Any T <= kTile gives the same results (each grain's terms are
+// computed the same way whatever the tiling); it only changes the speed.
+int set_tiles(sevc_state* s, int T) {
+    std::vector<Blk> tt;
+    for (int m = 0; m < s->B; ++m)
+        for (int b = 0; b < s->ng[m]; b += T)
+            tt.push_back(Blk{m, s->goff[m] + b, s->goff[m] + std::min(b + T, s->ng[m])});
+    if (tt.size() > s->tile_cap || T < kTileMin || T > kTile) return SEVC_E_INVALID_ARG;
+    if (hipMemcpy(const_cast<Blk*>(s->d.gtile), tt.data(), tt.size() * sizeof(Blk), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return SEVC_E_HIP;
+    s->d.ntile = static_cast<int>(tt.size());
+    s->tile_grains = T;
+    // a captured step holds the old table size
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return SEVC_OK;
+}
 
 // One time step of every active member (main.cu:124-389).
 void launch_step(sevc_state* s, bool diag) {
@@ -202,7 +222,7 @@ void launch_step(sevc_state* s, bool diag) {
     if (diag) {
         launch(s, "velocity_generic", k_velocity_generic, d.nvb, kBlock, d);
     } else {
-        launch(s, "vel_grain", k_vel_grain, d.ntile, 256, d);
+        launch(s, "vel_grain", k_vel_grain, d.ntile, 256, d, 0);
         launch(s, "velocity", k_vel_vertex, d.nvb, kBlock, d);
         launch(s, "velocity_fallback", k_velocity_fallback, 64, 256, d);
     }
@@ -218,15 +238,11 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "two_scan", k_two_scan, d.ngb, kBlock, d);
         launch(s, "two_grain", k_two_grain, 1, 64, d);
         launch(s, "label3", k_label3, d.ngb, kBlock, d);
-        launch(s, "t2_split", k_t2_split, kListGrid, 256, d);
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
         launch(s, "fix", k_fix, kListGrid, 64, d, 1);
-        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 1, 0);
-        launch(s, "area_list", k_area_list, kListGrid, 256, d, 0);
-        launch(s, "area_list", k_area_list, kListGrid, 256, d, 1);
+        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 1, 1);
         launch(s, "st2", k_st2, kListGrid, 256, d);
-        launch(s, "flip_split", k_flip_split, kListGrid, 256, d);
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
         launch(s, "fix", k_fix, kListGrid, 64, d, 2);
@@ -235,7 +251,6 @@ void launch_step(sevc_state* s, bool diag) {
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);
     launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 1);
-    launch(s, "end", k_end, nb, 64, d);
 }
 
 void build_table(std::vector<Blk>& tbl, std::vector<int>& first, const std::vector<int>& off,
@@ -339,8 +354,8 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
     DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
     DA(d.cnt, C_COUNT);
-    DA(d.l_cand, E); DA(d.l_two, E); DA(d.l_t2, G); DA(d.l_t2p, G); DA(d.l_t2c, G);
-    DA(d.l_flipp, E); DA(d.l_flipc, E);
+    DA(d.l_cand, E); DA(d.l_two, E); DA(d.l_t2, G); DA(d.l_t2c, G);
+    DA(d.l_flipc, E);
     for (int k = 0; k < kFixStages; ++k) DA(d.l_fix[k], G);
     DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
     DA(d.nscr, 12 * G + 16);
@@ -361,18 +376,17 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.bmin, et.size());
     d.vblk = dvt; d.eblk = det; d.gblk = dgt;
     {
-        std::vector<Blk> tt;
-        for (int m = 0; m < B; ++m)
-            for (int b = 0; b < s->ng[m]; b += kTile)
-                tt.push_back(Blk{m, s->goff[m] + b, s->goff[m] + std::min(b + kTile, s->ng[m])});
+        // room for the smallest tile size the autotuner tries
+        size_t cap = 0;
+        for (int m = 0; m < B; ++m) cap += (s->ng[m] + kTileMin - 1) / kTileMin;
         Blk* dtt;
-        DA(dtt, tt.size());
-        if (hipMemcpy(dtt, tt.data(), tt.size() * sizeof(Blk), hipMemcpyHostToDevice) != hipSuccess) {
-            sevc_destroy(s);
-            return SEVC_E_HIP;
-        }
+        DA(dtt, cap);
         d.gtile = dtt;
-        d.ntile = static_cast<int>(tt.size());
+        s->tile_cap = cap;
+        if ((rc = set_tiles(s, kTile)) != SEVC_OK) {
+            sevc_destroy(s);
+            return rc;
+        }
     }
     DA(d.vterm, 6 * V);
     DA(d.slist, static_cast<size_t>(d.ngb) + d.neb + d.nvb);
@@ -654,6 +668,36 @@ int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
     loc(out->v_grn, 3 * static_cast<size_t>(nv), s->goff[m]);
     loc(out->b_vtx, 2 * static_cast<size_t>(ne), s->voff[m]);
     loc(out->g_vtx, static_cast<size_t>(ng) * S, s->voff[m]);
+    return SEVC_OK;
+}
+
+int sevc_autotune(sevc_state* s, int32_t* tile_grains) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    static const int cand[] = {16, 24, 32, 48, 64};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = s->tile_grains;
+    float best_ms = 1e30f;
+    int rc = SEVC_OK;
+    for (int T : cand) {
+        if ((rc = set_tiles(s, T)) != SEVC_OK) break;
+        const Dev d = s->d;
+        hipLaunchKernelGGL(k_vel_grain, dim3(d.ntile), dim3(256), 0, s->stream, d, 1);   // warm-up
+        (void)hipEventRecord(e0, s->stream);
+        for (int r = 0; r < 3; ++r) hipLaunchKernelGGL(k_vel_grain, dim3(d.ntile), dim3(256), 0, s->stream, d, 1);
+        (void)hipEventRecord(e1, s->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = SEVC_E_HIP; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) { best_ms = ms; best = T; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SEVC_OK) return fail(s, rc, "autotune failed");
+    if ((rc = set_tiles(s, best)) != SEVC_OK) return rc;
+    if (tile_grains) *tile_grains = best;
     return SEVC_OK;
 }
 

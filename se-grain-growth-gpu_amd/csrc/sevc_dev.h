@@ -69,6 +69,19 @@ __device__ __forceinline__ void grains_symdiff(const Dev& d, int a, int b, int o
 
 __device__ __forceinline__ int* glist(const Dev& d, int g) { return d.gvx + (size_t)g * d.S; }
 
+// Member owning a grain / edge slot (members own consecutive slot ranges).
+__device__ __forceinline__ int member_of_grain(const Dev& d, int g) {
+    int m = 0;
+    while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
+    return m;
+}
+
+__device__ __forceinline__ int member_of_edge(const Dev& d, int e) {
+    int m = 0;
+    while (m + 1 < d.B && e >= d.prm[m + 1].eoff) m++;
+    return m;
+}
+
 // geometry.h:703-710
 __device__ __forceinline__ int grain_contains_vertex(const Dev& d, int g, int v) {
     const int* L = glist(d, g);

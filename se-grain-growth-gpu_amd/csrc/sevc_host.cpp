@@ -480,6 +480,114 @@ int sevc_export_snapshot(const sevc_mesh* m, const sevc_options* o, const char* 
     return SEVC_OK;
 }
 
+int sevc_load_snapshot(const char* path, int32_t max_vrt_per_grn, sevc_mesh* out, double* time) {
+    if (!path || !out) return SEVC_E_INVALID_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return SEVC_E_IO;
+    char line[4096];
+    double t = 0, D = 1;
+    int nv = -1, ne = -1, ng = -1;
+    // header (io.h:300-317): "KEY value" lines up to the vertex section
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') break;
+        if (std::sscanf(line, "TIME %lf", &t) == 1) continue;
+        if (std::sscanf(line, "DOMAIN BOUND %lf", &D) == 1) continue;
+        if (std::sscanf(line, "VERTICES %d", &nv) == 1) continue;
+        if (std::sscanf(line, "BOUNDARIES %d", &ne) == 1) continue;
+        std::sscanf(line, "GRAINS %d", &ng);
+    }
+    if (nv < 0 || ne < 0 || ng < 0) { std::fclose(f); return SEVC_E_IO; }
+    const int S = max_vrt_per_grn > 0 ? max_vrt_per_grn : 32;
+    int rc = sevc_mesh_alloc(out, nv, ne, ng, S);
+    if (rc != SEVC_OK) { std::fclose(f); return rc; }
+    out->domain_bound = D;
+    std::vector<int> nbr(3 * static_cast<size_t>(nv), -1);
+    auto fail_io = [&](int code) { std::fclose(f); sevc_mesh_free(out); return code; };
+    for (int i = 0; i < nv; ++i) {                       // #VERTEX section
+        if (!std::fgets(line, sizeof line, f)) return fail_io(SEVC_E_IO);
+        int id, en, g[3], n[3];
+        double e, x, y, vx, vy, p0, p1, p2, p3;
+        if (std::sscanf(line, "%d %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %d %d %d %d %d %d", &id, &en, &e, &x, &y,
+                        &vx, &vy, &p0, &p1, &p2, &p3, &g[0], &g[1], &g[2], &n[0], &n[1], &n[2]) != 17 || id != i)
+            return fail_io(SEVC_E_IO);
+        out->v_enabled[i] = static_cast<uint8_t>(en);
+        out->pos[2 * i] = x; out->pos[2 * i + 1] = y;
+        out->vel[2 * i] = vx; out->vel[2 * i + 1] = vy;
+        if (out->v_energy) out->v_energy[i] = e;
+        if (out->v_vel_parts) {
+            out->v_vel_parts[4 * i] = p0; out->v_vel_parts[4 * i + 1] = p1;
+            out->v_vel_parts[4 * i + 2] = p2; out->v_vel_parts[4 * i + 3] = p3;
+        }
+        for (int j = 0; j < 3; ++j) { out->v_grn[3 * i + j] = g[j]; nbr[3 * static_cast<size_t>(i) + j] = n[j]; }
+    }
+    if (!std::fgets(line, sizeof line, f) || line[0] != '#') return fail_io(SEVC_E_IO);
+    for (int b = 0; b < ne; ++b) {                       // #BOUNDARY section
+        if (!std::fgets(line, sizeof line, f)) return fail_io(SEVC_E_IO);
+        int id, en, ini, end, cand;
+        double e, L, te;
+        if (std::sscanf(line, "%d %d %d %d %lf %lf %lf %d", &id, &en, &ini, &end, &e, &L, &te, &cand) != 8 || id != b)
+            return fail_io(SEVC_E_IO);
+        out->b_enabled[b] = static_cast<uint8_t>(en);
+        out->b_vtx[2 * b] = ini; out->b_vtx[2 * b + 1] = end;
+        if (out->b_energy) out->b_energy[b] = e;
+        if (out->b_arclength) out->b_arclength[b] = L;
+        if (out->b_t_ext) out->b_t_ext[b] = te;
+        if (out->b_candidate) out->b_candidate[b] = static_cast<uint8_t>(cand);
+    }
+    if (!std::fgets(line, sizeof line, f) || line[0] != '#') return fail_io(SEVC_E_IO);
+    for (int g = 0; g < ng; ++g) {                       // #GRAIN section
+        if (!std::fgets(line, sizeof line, f)) return fail_io(SEVC_E_IO);
+        char* q = line;
+        auto next_long = [&q]() { char* e; long v = std::strtol(q, &e, 10); q = e; return v; };
+        auto next_double = [&q]() { char* e; double v = std::strtod(q, &e); q = e; return v; };
+        if (next_long() != g) return fail_io(SEVC_E_IO);
+        out->g_enabled[g] = static_cast<uint8_t>(next_long());
+        const long typ = next_long();
+        if (out->g_type) out->g_type[g] = static_cast<uint8_t>(typ);
+        out->g_orientation[g] = next_double();
+        out->g_SE[g] = next_double();
+        const double a = next_double(), dadt = next_double();
+        if (out->g_area) out->g_area[g] = a;
+        if (out->g_dAdt) out->g_dAdt[g] = dadt;
+        const long n = next_long();
+        if (n < 0 || n > SEVC_REF_MAX_VRT_PER_GRN) return fail_io(SEVC_E_IO);
+        if (n > S) return fail_io(SEVC_E_GRAIN_FULL);
+        out->g_vlen[g] = static_cast<int32_t>(n);
+        for (long k = 0; k < n; ++k) out->g_vtx[static_cast<size_t>(g) * S + k] = static_cast<int32_t>(next_long());
+    }
+    std::fclose(f);
+    // v_bnd from the neighbour columns: for an enabled vertex, its enabled
+    // boundaries ordered as its listed neighbours (export_data writes the
+    // other end of vertex.boundaries[j] as N_j)
+    std::vector<std::vector<int>> inc(static_cast<size_t>(nv));
+    for (int b = 0; b < ne; ++b)
+        if (out->b_enabled[b])
+            for (int e = 0; e < 2; ++e) {
+                const int v = out->b_vtx[2 * b + e];
+                if (v >= 0 && v < nv) inc[static_cast<size_t>(v)].push_back(b);
+            }
+    for (int i = 0; i < nv; ++i) {
+        if (!out->v_enabled[i]) continue;
+        std::vector<int>& cand = inc[static_cast<size_t>(i)];
+        std::vector<char> used(cand.size(), 0);
+        for (int j = 0; j < 3; ++j) {
+            const int want = nbr[3 * static_cast<size_t>(i) + j];
+            for (size_t c = 0; c < cand.size(); ++c) {
+                const int b = cand[c];
+                const int other = out->b_vtx[2 * b] == i ? out->b_vtx[2 * b + 1] : out->b_vtx[2 * b];
+                if (!used[c] && other == want) {
+                    out->v_bnd[3 * i + j] = b;
+                    used[c] = 1;
+                    break;
+                }
+            }
+            if (out->v_bnd[3 * i + j] < 0) { sevc_mesh_free(out); return SEVC_E_TOPOLOGY; }
+        }
+    }
+    if (time) *time = t;
+    return SEVC_OK;
+}
+
 const char* sevc_strerror(int code) {
     switch (code) {
         case SEVC_OK: return "ok";

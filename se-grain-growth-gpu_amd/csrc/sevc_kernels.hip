@@ -125,26 +125,23 @@ __global__ void __launch_bounds__(1024) k_member_reduce(Dev d, int what) {
             if (tid < st) { sa[tid] += sa[tid + st]; sb[tid] += sb[tid + st]; }
             __syncthreads();
         }
-        if (tid == 0) { M.curr_grains = (int)sa[0]; M.n_venabled = (int)sb[0]; }
+        if (tid == 0) {
+            M.curr_grains = (int)sa[0];
+            M.n_venabled = (int)sb[0];
+            // main.cu:388-393: step counter, time and stop rules
+            const Params& P = d.prm[m];
+            M.dt = member_dt(P, M);
+            M.n += 1;
+            M.steps += 1;
+            if (P.adaptive) M.t += M.dt;
+            else M.t = M.n * P.dt_max;
+            if (d.cnt[C_ERR] != 0 && M.status == 0) M.status = d.cnt[C_ERR];
+            if (M.status != 0) M.done = 1;
+            if (M.curr_grains <= P.min_grains && P.min_grains > 0) M.done = 1;
+        }
     }
 }
 
-// main.cu:388-393
-__global__ void k_end(Dev d) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= d.B) return;
-    const Params& P = d.prm[m];
-    MState& M = d.ms[m];
-    if (!M.active) return;
-    M.dt = member_dt(P, M);
-    M.n += 1;
-    M.steps += 1;
-    if (P.adaptive) M.t += M.dt;
-    else M.t = M.n * P.dt_max;
-    if (d.cnt[C_ERR] != 0 && M.status == 0) M.status = d.cnt[C_ERR];
-    if (M.status != 0) M.done = 1;
-    if (M.curr_grains <= P.min_grains && P.min_grains > 0) M.done = 1;
-}
 
 // ---------------------------------------------------------------------
 // calculus.h: lengths, energies, areas, velocities, translation
@@ -260,12 +257,13 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
 // dom_mod(y) != y to the two-pass path, which this form does not cover.
 constexpr int kTilePts = 512;
 static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
-__global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
+__global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     __shared__ double2 Ps[kTilePts];
     __shared__ double2 Ds[kTilePts];        // wrap(P_k - P_{k-1}) of list slot k > 0
     __shared__ int Ls[kTilePts];
     __shared__ unsigned char pgl[kTilePts];
-    __shared__ int off[kTile + 1];
+    __shared__ int off[kTile + 1];          // by slot: grains ordered by list length
+    __shared__ int gid_s[kTile];            // slot -> grain index in the tile
     __shared__ double area_s[kTile];
     __shared__ int gend_s;
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -273,7 +271,9 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
     const Blk bd = d.gtile[lb];
     const Params& P = d.prm[bd.m];
     const MState& M = d.ms[bd.m];
-    if (!M.active || P.vel_model != SEVC_VEL_FD) return;
+    // tuning: timed by sevc_autotune on any member, without writing areas
+    // (the terms it writes are scratch of the next step's pass 2)
+    if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD) return;
     const double D = P.D;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int gs = bd.b; gs < bd.e;) {
@@ -289,7 +289,22 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
             }
             const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
             const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
-            if (tid < cnt) off[tid] = inc - n;
+            // slots in order of (list length, index): a wave's points then
+            // come from grains of similar length, so its lanes run loops of
+            // similar trip count
+            int r = 0, o = 0;
+            for (int j = 0; j < cnt; j++) {
+                const int nj = __shfl(n, j, kTile);
+                r += (nj < n || (nj == n && j < tid)) ? 1 : 0;
+            }
+            for (int j = 0; j < cnt; j++) {
+                const int rj = __shfl(r, j, kTile), nj = __shfl(n, j, kTile);
+                o += (rj < r) ? nj : 0;
+            }
+            if (tid < cnt) {
+                off[r] = o;
+                gid_s[r] = tid;
+            }
             if (tid == cnt - 1) off[cnt] = inc;
             if (tid == 0) {
                 gend_s = gs + cnt;
@@ -303,7 +318,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p];
-            const int w = glist(d, gs + g)[p - off[g]];
+            const int w = glist(d, gs + gid_s[g])[p - off[g]];
             Ls[p] = w;
             Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
         }
@@ -332,13 +347,14 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d) {
             a += ux * z - uy * z;
             a *= 0.5;
             if (n <= 0) a = 0.0;
-            if (d.genab[gs + g]) d.garea[gs + g] = a;
+            const int G = gs + gid_s[g];
+            if (d.genab[G] && !tuning) d.garea[G] = a;
             area_s[g] = a;
         }
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p], b0 = off[g], n = off[g + 1] - b0, k = p - b0;
-            const int v = Ls[p], gg = gs + g;
+            const int v = Ls[p], gg = gs + gid_s[g];
             const double x = Ps[p].x, y = Ps[p].y;
             const double px = dom_mod(x + SEVC_GRAD_EPS, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
             double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
@@ -584,7 +600,67 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         }
         return d.vb[3 * v + arg];
     };
-    for (;;) {
+    constexpr int kPer = 2;
+    if (nc <= kPer * nt) {
+        // Up to kPer candidates per lane: their ends' boundaries and those
+        // boundaries' t_ext do not change during the rounds, so they are
+        // loaded once; a round then reads only the candidate flags.
+        int cc[kPer], eb[kPer][6];
+        double et[kPer][6];
+        bool two[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int k = tid + u * nt;
+            cc[u] = k < nc ? d.l_cand[k] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            // listed candidates are enabled edges, whose ends and boundaries are valid
+            const int c = cc[u];
+            const int a0 = c >= 0 ? d.bvx[2 * c] : 0, a1 = c >= 0 ? d.bvx[2 * c + 1] : 0;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                eb[u][q] = c >= 0 ? d.vb[3 * a0 + q] : 0;
+                eb[u][3 + q] = c >= 0 ? d.vb[3 * a1 + q] : 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) et[u][j] = cc[u] >= 0 ? d.bt[eb[u][j]] : 0.0;
+        // vertices_vote (topological.h:362-386): the end's candidate with the
+        // smallest t_ext, later slots winning ties (<=); slot 0 if none
+        auto vote = [&](int u, int e) {
+            double min_t = 99999999;
+            int arg = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (et[u][3 * e + k] <= min_t && d.bcand[eb[u][3 * e + k]]) { min_t = et[u][3 * e + k]; arg = k; }
+            return eb[u][3 * e + arg];
+        };
+        for (;;) {
+            if (tid == 0) changed = 0;
+#pragma unroll
+            for (int u = 0; u < kPer; u++) {
+                two[u] = false;
+                if (cc[u] >= 0 && d.bcand[cc[u]])
+                    two[u] = (vote(u, 0) == cc[u]) && (vote(u, 1) == cc[u]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kPer; u++)
+                if (two[u])
+#pragma unroll
+                    for (int j = 0; j < 6; j++) {
+                        const int x = eb[u][j];
+                        if (x != cc[u] && d.bcand[x]) { d.bcand[x] = 0; changed = 1; }
+                    }
+            __syncthreads();
+            int ch = changed;
+            __syncthreads();
+            if (!ch) break;
+        }
+    } else for (;;) {
         if (tid == 0) changed = 0;
         // phase A: votes on the flags at the start of the round
         for (int k = tid; k < nc; k += nt) {
@@ -841,29 +917,26 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
     }
 }
 
-// clear_grains_fix_state (topological.h:1150-1163) for the flagged grains,
-// optionally followed by the area update of main.cu:266 for them.
+// clear_grains_fix_state (topological.h:1150-1163) for the flagged grains;
+// with with_area also the area update of main.cu:266 for them and for the
+// grains edited by the two-triangle stage (its dirty list).
 __global__ void k_fix_clear(Dev d, int stage, int with_area) {
     const int n = d.cnt[C_FIX0 + stage];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = d.l_fix[stage][k];
-        d.gfix[g] = 0;
-        if (!d.genab[g]) continue;
-        const int* L = glist(d, g);
-        for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
-    }
-    (void)with_area;
-}
-
-__global__ void k_area_list(Dev d, int which) {
-    const int n = which == 0 ? min(d.cnt[C_DIRTY], d.G) : d.cnt[C_FIX1];
-    const int* lst = which == 0 ? d.l_dirty : d.l_fix[1];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = lst[k];
-        if (!d.genab[g]) continue;
-        int m = 0;
-        while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
-        d.garea[g] = grain_area(d, g, d.prm[m].D);
+    const int nd = with_area ? min(d.cnt[C_DIRTY], d.G) : 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n + nd; k += gridDim.x * blockDim.x) {
+        int g;
+        if (k < n) {
+            g = d.l_fix[stage][k];
+            d.gfix[g] = 0;
+            if (!d.genab[g]) continue;
+            const int* L = glist(d, g);
+            for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
+            if (!with_area) continue;
+        } else {
+            g = d.l_dirty[k - n];
+            if (!d.genab[g]) continue;
+        }
+        d.garea[g] = grain_area(d, g, d.prm[member_of_grain(d, g)].D);
     }
 }
 
@@ -970,11 +1043,6 @@ __device__ void sort_ints(int* a, int n) {
     }
 }
 
-__device__ int member_of_edge(const Dev& d, int e) {
-    int m = 0;
-    while (m + 1 < d.B && e >= d.prm[m + 1].eoff) m++;
-    return m;
-}
 
 // The loop of main.cu:227-250 on one lane: serial passes in edge order over
 // the qualifying edges (re-checked when reached, as the <<<1,1>>> scan of
@@ -1105,26 +1173,6 @@ __global__ void k_label3(Dev d) {
     }
 }
 
-// Split the T2 candidates: a candidate that shares a vertex with another
-// candidate goes to the serial list.
-__global__ void k_t2_split(Dev d) {
-    const int n = d.cnt[C_T2];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = d.l_t2[k];
-        const int* L = glist(d, g);
-        bool clash = false;
-        for (int q = 0; q < 3 && !clash; q++) {
-            int v = L[q];
-            for (int r = 0; r < 3; r++) {
-                int h = d.vg[3 * v + r];
-                if (h != g && h >= 0 && d.genab[h] && d.gcand[h]) { clash = true; break; }
-            }
-        }
-        if (clash) { int s = atomicAdd(&d.cnt[C_T2C], 1); d.l_t2c[s] = g; }
-        else { int s = atomicAdd(&d.cnt[C_T2P], 1); d.l_t2p[s] = g; }
-    }
-}
-
 // remove_grain, topological.h:687-822
 __device__ void remove_grain(const Dev& d, int g, double D) {
     const int* Lg = glist(d, g);
@@ -1188,19 +1236,35 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     for (int i = 0; i < 3; i++) if (d.vg[3 * A + i] == g) d.vg[3 * A + i] = G3;
 }
 
-__device__ int member_of_grain(const Dev& d, int g) {
-    int m = 0;
-    while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
-    return m;
-}
 
+// T2 candidates in parallel: a candidate with no other enabled candidate
+// among the grains of its vertices is removed at once; the others go to the
+// serial list (k_t2_serial, index order).  A removed grain never has a
+// candidate neighbour (it would have seen that neighbour, which is enabled
+// until its own check passes), and a removal writes only data that no
+// non-adjacent candidate's check reads, so the outcome does not depend on
+// the schedule.
 __global__ void k_t2_par(Dev d) {
-    const int n = d.cnt[C_T2P];
+    const int n = d.cnt[C_T2];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = d.l_t2p[k];
-        int m = member_of_grain(d, g);
-        remove_grain(d, g, d.prm[m].D);
-        atomicAdd((unsigned long long*)&d.ms[m].removals, 1ull);
+        int g = d.l_t2[k];
+        const int* L = glist(d, g);
+        bool clash = false;
+        for (int q = 0; q < 3 && !clash; q++) {
+            int v = L[q];
+            for (int r = 0; r < 3; r++) {
+                int h = d.vg[3 * v + r];
+                if (h != g && h >= 0 && d.genab[h] && d.gcand[h]) { clash = true; break; }
+            }
+        }
+        if (clash) {
+            int s = atomicAdd(&d.cnt[C_T2C], 1);
+            d.l_t2c[s] = g;
+        } else {
+            int m = member_of_grain(d, g);
+            remove_grain(d, g, d.prm[m].D);
+            atomicAdd((unsigned long long*)&d.ms[m].removals, 1ull);
+        }
     }
 }
 
@@ -1233,24 +1297,6 @@ __global__ void k_st2(Dev d) {
 // ---------------------------------------------------------------------
 // T1: flips (topological.h:174-291, 1016-1026)
 // ---------------------------------------------------------------------
-
-__global__ void k_flip_split(Dev d) {
-    const int n = d.cnt[C_CAND];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int c = d.l_cand[k];
-        if (!(d.benab[c] && d.bcand[c])) continue;
-        bool clash = false;
-        for (int e = 0; e < 2 && !clash; e++) {
-            int v = d.bvx[2 * c + e];
-            for (int q = 0; q < 3; q++) {
-                int x = d.vb[3 * v + q];
-                if (x != c && d.benab[x] && d.bcand[x]) { clash = true; break; }
-            }
-        }
-        if (clash) { int s = atomicAdd(&d.cnt[C_FLIPC], 1); d.l_flipc[s] = c; }
-        else { int s = atomicAdd(&d.cnt[C_FLIPP], 1); d.l_flipp[s] = c; }
-    }
-}
 
 // flip, topological.h:174-291
 __device__ void flip(const Dev& d, int bnd, const Params& P) {
@@ -1314,13 +1360,33 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
     d.vel[2 * B] = 0; d.vel[2 * B + 1] = 0;
 }
 
+// Flips in parallel: a candidate with no other enabled candidate at either
+// end flips at once, the others go to the serial list (k_flip_serial, index
+// order).  A flip changes the boundary lists of its own two vertices and the
+// ends of two edges; a candidate reads those only if it shares a vertex, and
+// then both are in the serial list, so the outcome does not depend on the
+// schedule.
 __global__ void k_flip_par(Dev d) {
-    const int n = d.cnt[C_FLIPP];
+    const int n = d.cnt[C_CAND];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int c = d.l_flipp[k];
-        int m = member_of_edge(d, c);
-        flip(d, c, d.prm[m]);
-        atomicAdd((unsigned long long*)&d.ms[m].flips, 1ull);
+        int c = d.l_cand[k];
+        if (!(d.benab[c] && d.bcand[c])) continue;
+        bool clash = false;
+        for (int e = 0; e < 2 && !clash; e++) {
+            int v = d.bvx[2 * c + e];
+            for (int q = 0; q < 3; q++) {
+                int x = d.vb[3 * v + q];
+                if (x != c && d.benab[x] && d.bcand[x]) { clash = true; break; }
+            }
+        }
+        if (clash) {
+            int s = atomicAdd(&d.cnt[C_FLIPC], 1);
+            d.l_flipc[s] = c;
+        } else {
+            int m = member_of_edge(d, c);
+            flip(d, c, d.prm[m]);
+            atomicAdd((unsigned long long*)&d.ms[m].flips, 1ull);
+        }
     }
 }
 

@@ -39,7 +39,7 @@ struct MState {
 struct Blk { int m, b, e; };
 
 enum Counter {
-    C_CAND = 0, C_TWO, C_T2, C_T2P, C_T2C, C_FLIPP, C_FLIPC,
+    C_CAND = 0, C_TWO, C_T2, C_T2C, C_FLIPC,
     C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NDE, C_NDG, C_VFALL, C_ERR, C_COUNT
 };
 
@@ -66,7 +66,7 @@ struct Dev {
     int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
     // work lists and scratch
     int* cnt;
-    int* l_cand; int* l_two; int* l_t2; int* l_t2p; int* l_t2c; int* l_flipp; int* l_flipc;
+    int* l_cand; int* l_two; int* l_t2; int* l_t2c; int* l_flipc;
     int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_nde; int* l_ndg; int* l_vfall;
     int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
     double* partial;                      // per grain block partial sums (2 levels)

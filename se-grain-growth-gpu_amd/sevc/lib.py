@@ -46,6 +46,7 @@ def lib():
         "sevc_free": (None, [C.c_void_p]),
         "sevc_check_mesh": (C.c_int, [P(Mesh), P(C.c_double)]),
         "sevc_export_snapshot": (C.c_int, [P(Mesh), P(Options), C.c_char_p, C.c_int32, C.c_double]),
+        "sevc_load_snapshot": (C.c_int, [C.c_char_p, C.c_int32, P(Mesh), P(C.c_double)]),
         "sevc_create": (C.c_int, [P(Options), P(Mesh), P(C.c_void_p)]),
         "sevc_batch_create": (C.c_int, [P(Options), P(Mesh), C.c_int32, P(C.c_void_p)]),
         "sevc_step": (C.c_int, [C.c_void_p, C.c_int32, P(StepStats)]),
@@ -59,6 +60,7 @@ def lib():
                                              C.c_double, C.c_double]),
         "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
         "sevc_get_positions": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double)]),
+        "sevc_autotune": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -125,6 +127,14 @@ def build_mesh(opts, unit_pos, ridges, orientations01, se_raw, n_grains_expected
                                  ori.ctypes.data_as(P(C.c_double)), se.ctypes.data_as(P(C.c_double)),
                                  C.byref(m)))
     return _take_mesh(m)
+
+
+def load_snapshot(path, max_vrt_per_grn=32):
+    """(mesh, time) of a snapshot written by sevc_export_snapshot."""
+    m = Mesh()
+    t = C.c_double()
+    _check(lib().sevc_load_snapshot(str(path).encode(), int(max_vrt_per_grn), C.byref(m), C.byref(t)))
+    return _take_mesh(m), t.value
 
 
 def voronoi(n_seeds, seed):
@@ -204,6 +214,13 @@ class State:
                                          hist.ctypes.data_as(P), loss.ctypes.data_as(P), arr, int(bool(seed))),
                self.h)
         return (hist, loss, grads) if want_grad else (hist, loss)
+
+    def autotune(self):
+        """Time the velocity grain pass for several tile sizes and keep the
+        fastest; returns the chosen grains per tile."""
+        t = C.c_int32()
+        _check(lib().sevc_autotune(self.h, C.byref(t)), self.h)
+        return t.value
 
     def positions(self, m=0):
         """Vertex positions of member m, shape (n_vertex_slots, 2)."""

@@ -207,3 +207,30 @@ def test_seed_ensemble_curves_match_oracle():
     assert curves["gpu"] == curves["oracle"]
     assert events > 0
     st.close()
+
+
+def test_autotuned_tiles_keep_results():
+    """sevc_autotune changes only the tiling of the velocity's grain pass:
+    the state it leaves and every later step equal the oracle's."""
+    o, mesh = _generated(2500, 8, dt=1e-4, adaptive_dt=1, flip_threshold=2e-3, do_nucleation=1,
+                         nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.01, seed=3, extra_capacity=0.1)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    before = st.get_mesh()
+    tile = st.autotune()
+    assert 16 <= tile <= 64
+    assert diff_fields(st.get_mesh(), before) == {}
+    for chunk in (30, 60):
+        st.step(chunk)
+        rc, _ = orc.step(chunk)
+        assert rc == 0
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+    st.close()
+
+
+def test_anisotropic_boundary_energy():
+    """GB_eps != 0 (geometry.h:548-564): per-edge gamma from the grains'
+    orientations in the velocity, with T1/T2 events."""
+    o = ref2k_options(dt=2e-4, GB_eps=0.3)
+    mesh = sevc.load_files(o)
+    tot = _run_pair(o, mesh, [40, 80])
+    assert tot["flips"] > 0, tot

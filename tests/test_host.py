@@ -186,3 +186,32 @@ def test_cli_usage_and_clean_failure(tmp_path):
             "n_grains=2000", f"folder={tmp_path / 'out'}", ""]))
         r = subprocess.run([cli, "-c", str(ini)], capture_output=True, text=True, timeout=60)
         assert r.returncode == 1 and "no HIP device" in r.stderr, r.stderr
+
+
+def test_snapshot_round_trip_for_resume(tmp_path):
+    """A snapshot of a state with T1/T2 history reads back with the same
+    topology (vertex boundary order rebuilt from the neighbour columns) and
+    positions rounded to the format's 16 decimals."""
+    from _util import Oracle
+    o = ref2k_options(dt=2e-4)
+    orc = Oracle(o, sevc.load_files(o))
+    rc, st = orc.step(300)
+    assert rc == 0 and st.n_flips > 0 and st.n_removals > 0
+    h = orc.get_mesh()
+    L = sevc.lib()
+    assert L.sevc_export_snapshot(C.byref(h.struct), C.byref(o), str(tmp_path).encode(), 7, C.c_double(st.t)) == 0
+    r, t = sevc.load_snapshot(tmp_path / "000007.txt", h.S)
+    assert abs(t - st.t) < 1e-15
+    for f in ("v_enabled", "v_grn", "b_vtx", "b_enabled", "g_vlen", "g_enabled", "g_type"):
+        assert np.array_equal(r[f], h[f]), f
+    gl_r, gl_h = r.grain_lists(), h.grain_lists()
+    for g in np.nonzero(h["g_enabled"])[0]:
+        n = h["g_vlen"][g]
+        assert np.array_equal(gl_r[g, :n], gl_h[g, :n]), g
+    en = h["v_enabled"] == 1
+    assert np.array_equal(r["v_bnd"].reshape(-1, 3)[en], h["v_bnd"].reshape(-1, 3)[en])
+    assert np.max(np.abs(r["pos"] - h["pos"])) <= 1e-15
+    assert np.allclose(r["g_SE"], h["g_SE"], rtol=0, atol=1e-15)
+    # the restored mesh is a valid state
+    bad, total = sevc.check_mesh(r)
+    assert bad == 0 and abs(total - 1.0) < 1e-12
