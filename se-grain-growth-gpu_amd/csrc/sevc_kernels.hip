@@ -262,8 +262,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     __shared__ double2 Ds[kTilePts];        // wrap(P_k - P_{k-1}) of list slot k > 0
     __shared__ int Ls[kTilePts];
     __shared__ unsigned char pgl[kTilePts];
-    __shared__ int off[kTile + 1];          // by slot: grains ordered by list length
-    __shared__ int gid_s[kTile];            // slot -> grain index in the tile
+    __shared__ int off[kTile + 1];
     __shared__ double area_s[kTile];
     __shared__ int gend_s;
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -289,22 +288,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             }
             const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
             const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
-            // slots in order of (list length, index): a wave's points then
-            // come from grains of similar length, so its lanes run loops of
-            // similar trip count
-            int r = 0, o = 0;
-            for (int j = 0; j < cnt; j++) {
-                const int nj = __shfl(n, j, kTile);
-                r += (nj < n || (nj == n && j < tid)) ? 1 : 0;
-            }
-            for (int j = 0; j < cnt; j++) {
-                const int rj = __shfl(r, j, kTile), nj = __shfl(n, j, kTile);
-                o += (rj < r) ? nj : 0;
-            }
-            if (tid < cnt) {
-                off[r] = o;
-                gid_s[r] = tid;
-            }
+            if (tid < cnt) off[tid] = inc - n;
             if (tid == cnt - 1) off[cnt] = inc;
             if (tid == 0) {
                 gend_s = gs + cnt;
@@ -318,7 +302,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p];
-            const int w = glist(d, gs + gid_s[g])[p - off[g]];
+            const int w = glist(d, gs + g)[p - off[g]];
             Ls[p] = w;
             Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
         }
@@ -347,14 +331,14 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             a += ux * z - uy * z;
             a *= 0.5;
             if (n <= 0) a = 0.0;
-            const int G = gs + gid_s[g];
+            const int G = gs + g;
             if (d.genab[G] && !tuning) d.garea[G] = a;
             area_s[g] = a;
         }
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p], b0 = off[g], n = off[g + 1] - b0, k = p - b0;
-            const int v = Ls[p], gg = gs + gid_s[g];
+            const int v = Ls[p], gg = gs + g;
             const double x = Ps[p].x, y = Ps[p].y;
             const double px = dom_mod(x + SEVC_GRAD_EPS, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
             double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;

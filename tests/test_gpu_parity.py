@@ -234,3 +234,69 @@ def test_anisotropic_boundary_energy():
     mesh = sevc.load_files(o)
     tot = _run_pair(o, mesh, [40, 80])
     assert tot["flips"] > 0, tot
+
+
+@pytest.mark.parametrize("n,gen_seed", [(12, 3), (16, 11), (20, 5), (30, 13), (40, 7)])
+def test_tiny_meshes(n, gen_seed):
+    """Periodic tessellations of a dozen to a few dozen grains with a large
+    dt: T1 and T2 events crowd and must still follow the oracle bit for bit
+    until the stop rule at half the grains (below that these meshes
+    degenerate, which the oracle reports as an inconsistent topology)."""
+    o, mesh = _generated(n, gen_seed, dt=1e-2, leave_percent=0.5)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    for chunk in (5, 20, 100):
+        print(f"  {n} grains: chunk of {chunk} steps", flush=True)
+        s1 = st.step(chunk)
+        rc, s2 = orc.step(chunk)
+        assert rc == 0 and s1.status == 0
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+        for k in STAT_KEYS:
+            assert getattr(s1, k) == getattr(s2, k), k
+    st.close()
+
+
+def test_member_stopping_early_leaves_others_running():
+    """Per-member stop rules in a batch: a member that reaches its grain rule
+    stops while the other keeps stepping; both equal their own oracle run."""
+    o1 = ref2k_options(dt=2e-4, leave_percent=0.9)
+    m1 = sevc.load_files(o1)
+    o2, m2 = _generated(1500, 3, dt=1e-4)
+    st = sevc.State([o1, o2], [m1, m2])
+    orcs = [Oracle(o1, m1), Oracle(o2, m2)]
+    for chunk in (100, 200):
+        st.step(chunk)
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            s1 = st.member_stats(k)
+            assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
+            for key in STAT_KEYS:
+                assert getattr(s1, key) == getattr(s2, key), (k, key)
+    assert st.member_stats(0).done and not st.member_stats(1).done
+    st.close()
+
+
+def test_grain_list_overflow_is_reported():
+    """A grain list capacity equal to the initial largest grain: the first
+    grain to grow past it stops the run with SEVC_E_GRAIN_FULL, in the same
+    chunk of steps as the oracle."""
+    o = ref2k_options(dt=2e-4)
+    o.max_vrt_per_grn = int(sevc.load_files(o)["g_vlen"].max())
+    mesh = sevc.load_files(o)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    gpu_fail = orc_fail = None
+    for chunk in range(40):
+        if gpu_fail is None:
+            try:
+                st.step(10)
+            except sevc.SevcError as e:
+                gpu_fail = (chunk, e.code)
+        if orc_fail is None:
+            rc, _ = orc.step(10)
+            if rc != 0:
+                orc_fail = (chunk, rc)
+        if gpu_fail and orc_fail:
+            break
+    assert gpu_fail is not None and gpu_fail == orc_fail, (gpu_fail, orc_fail)
+    assert gpu_fail[1] == sevc.E_GRAIN_FULL
+    st.close()
