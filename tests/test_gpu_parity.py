@@ -300,3 +300,29 @@ def test_grain_list_overflow_is_reported():
     assert gpu_fail is not None and gpu_fail == orc_fail, (gpu_fail, orc_fail)
     assert gpu_fail[1] == sevc.E_GRAIN_FULL
     st.close()
+
+
+@pytest.mark.parametrize("n,gen_seed", [(12, 3), (30, 13)])
+def test_degenerate_mesh_error_matches_oracle(n, gen_seed):
+    """Run a tiny tessellation down to a handful of grains, where a flip
+    meets an edge whose ends no longer share two grains
+    (topological.h:174-291 has no case for it): both sides match bit for bit
+    until then and report SEVC_E_STATE in the same chunk of steps.  (The
+    oracle, built with AddressSanitizer, stays in bounds on these runs.)"""
+    o, mesh = _generated(n, gen_seed, dt=2e-2, leave_percent=0.1)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    gpu_fail = orc_fail = None
+    for chunk in range(30):
+        print(f"  chunk {chunk} of 10 steps", flush=True)
+        try:
+            st.step(10)
+        except sevc.SevcError as e:
+            gpu_fail = (chunk, e.code)
+        rc, _ = orc.step(10)
+        if rc != 0:
+            orc_fail = (chunk, rc)
+        if gpu_fail or orc_fail:
+            break
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}, chunk
+    assert gpu_fail == orc_fail and gpu_fail[1] == sevc.E_STATE, (gpu_fail, orc_fail)
+    st.close()
