@@ -181,6 +181,9 @@ void collect_timers(sevc_state* s) {
 }
 
 constexpr int kListGrid = 512;
+// one wave per flagged grain in the fix stages: enough workgroups that
+// every grain of a busy step has its own
+constexpr int kFixGrid = 2048;
 constexpr int kTileMin = 16;             // smallest tile the autotuner tries
 
 // Tile table of velocity pass 1: T consecutive grains of one member per
@@ -240,13 +243,13 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "label3", k_label3, d.ngb, kBlock, d);
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
-        launch(s, "fix", k_fix, kListGrid, 64, d, 1);
-        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 1, 1);
+        launch(s, "fix", k_fix, kFixGrid, 64, d, 1);
+        launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 1, 1);
         launch(s, "st2", k_st2, kListGrid, 256, d);
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
-        launch(s, "fix", k_fix, kListGrid, 64, d, 2);
-        launch(s, "fix_clear", k_fix_clear, kListGrid, 256, d, 2, 0);
+        launch(s, "fix", k_fix, kFixGrid, 64, d, 2);
+        launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 2, 0);
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);

@@ -59,6 +59,10 @@ struct AreaAcc {
         double dx = wrapd(cx, px, D);
         double dy = wrapd(cy, py, D);
         px = cx; py = cy;
+        push_delta(dx, dy);
+    }
+    // the same step from a precomputed wrapped delta (raw point not tracked)
+    __device__ __forceinline__ void push_delta(double dx, double dy) {
         double nx = ux + dx, ny = uy + dy;
         area += ux * ny - uy * nx;
         ux = nx; uy = ny;

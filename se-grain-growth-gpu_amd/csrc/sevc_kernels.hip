@@ -588,20 +588,28 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
     if (nc <= kPer * nt) {
         // Up to kPer candidates per lane: their ends' boundaries and those
         // boundaries' t_ext do not change during the rounds, so they are
-        // loaded once; a round then reads only the candidate flags.
-        int cc[kPer], eb[kPer][6];
+        // loaded once.  Only listed candidates carry a flag that can be set
+        // (the ends of an enabled edge touch enabled edges only, and k_label
+        // lists every enabled candidate), so each neighbour is mapped once
+        // to its list position through conf, and the rounds then read and
+        // clear flags in LDS only.
+        __shared__ int cf[kPer * 1024];
+        int cc[kPer], eb[kPer][6], ep[kPer][6], ends[kPer][2];
         double et[kPer][6];
         bool two[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             const int k = tid + u * nt;
             cc[u] = k < nc ? d.l_cand[k] : -1;
+            if (cc[u] >= 0) { d.conf[cc[u]] = k; cf[k] = 1; }
         }
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             // listed candidates are enabled edges, whose ends and boundaries are valid
             const int c = cc[u];
             const int a0 = c >= 0 ? d.bvx[2 * c] : 0, a1 = c >= 0 ? d.bvx[2 * c + 1] : 0;
+            ends[u][0] = a0;
+            ends[u][1] = a1;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 eb[u][q] = c >= 0 ? d.vb[3 * a0 + q] : 0;
@@ -612,14 +620,21 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         for (int u = 0; u < kPer; u++)
 #pragma unroll
             for (int j = 0; j < 6; j++) et[u][j] = cc[u] >= 0 ? d.bt[eb[u][j]] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; u++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) ep[u][j] = cc[u] >= 0 ? d.conf[eb[u][j]] : -1;
         // vertices_vote (topological.h:362-386): the end's candidate with the
         // smallest t_ext, later slots winning ties (<=); slot 0 if none
         auto vote = [&](int u, int e) {
             double min_t = 99999999;
             int arg = 0;
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (et[u][3 * e + k] <= min_t && d.bcand[eb[u][3 * e + k]]) { min_t = et[u][3 * e + k]; arg = k; }
+            for (int k = 0; k < 3; k++) {
+                const int p = ep[u][3 * e + k];
+                if (et[u][3 * e + k] <= min_t && p >= 0 && cf[p]) { min_t = et[u][3 * e + k]; arg = k; }
+            }
             return eb[u][3 * e + arg];
         };
         for (;;) {
@@ -627,7 +642,7 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
                 two[u] = false;
-                if (cc[u] >= 0 && d.bcand[cc[u]])
+                if (cc[u] >= 0 && cf[tid + u * nt])
                     two[u] = (vote(u, 0) == cc[u]) && (vote(u, 1) == cc[u]);
             }
             __syncthreads();
@@ -636,14 +651,93 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
                 if (two[u])
 #pragma unroll
                     for (int j = 0; j < 6; j++) {
-                        const int x = eb[u][j];
-                        if (x != cc[u] && d.bcand[x]) { d.bcand[x] = 0; changed = 1; }
+                        const int p = ep[u][j];
+                        if (eb[u][j] != cc[u] && p >= 0 && cf[p]) { cf[p] = 0; changed = 1; }
                     }
             __syncthreads();
             int ch = changed;
             __syncthreads();
             if (!ch) break;
         }
+        // check_conflictive_boundaries and unlabel_conflictive_boundaries
+        // on the final flags, as one decision per flagged candidate c: c is
+        // dropped when one of its non-candidate enabled neighbours x has a
+        // flagged candidate at each end (the first in slot order, bi at the
+        // end shared with c, be at the other) and c is not the winner of the
+        // two (smaller t_ext, lower index on a tie; symmetric in bi, be).
+        // The reference writes that winner into x and lets c read it back;
+        // here c forms it itself.  Loads go out in rounds (the neighbours'
+        // flags and ends, then the far ends' boundaries, then their list
+        // positions, then t_ext of be), with index 0 standing in for slots
+        // that need none.
+        auto flagp = [&](int p) { return p >= 0 && cf[p] != 0; };
+        bool keep[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int c = cc[u];
+            keep[u] = c >= 0 && cf[tid + u * nt] != 0;
+            bool need[6];
+            int xi[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                need[j] = keep[u] && eb[u][j] != c && !flagp(ep[u][j]);
+                xi[j] = need[j] ? eb[u][j] : 0;
+            }
+            int xe[6], x0[6], x1[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                xe[j] = d.benab[xi[j]];
+                x0[j] = d.bvx[2 * xi[j]];
+                x1[j] = d.bvx[2 * xi[j] + 1];
+            }
+            int yw[6][3];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                need[j] = need[j] && xe[j];
+                const int V = ends[u][j / 3];
+                const int W = need[j] ? (x0[j] == V ? x1[j] : x0[j]) : 0;
+#pragma unroll
+                for (int r = 0; r < 3; r++) yw[j][r] = need[j] ? d.vb[3 * W + r] : 0;
+            }
+            int pw[6][3];
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) pw[j][r] = need[j] ? d.conf[yw[j][r]] : -1;
+            int bi[6], be[6];
+            double ti[6], tb[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const int x = eb[u][j], e3 = 3 * (j / 3);
+                bi[j] = -1; ti[j] = 0.0;
+#pragma unroll
+                for (int r = 2; r >= 0; r--)          // first in slot order
+                    if (eb[u][e3 + r] != x && flagp(ep[u][e3 + r])) { bi[j] = eb[u][e3 + r]; ti[j] = et[u][e3 + r]; }
+                be[j] = -1;
+#pragma unroll
+                for (int r = 2; r >= 0; r--)
+                    if (yw[j][r] != x && flagp(pw[j][r])) be[j] = yw[j][r];
+                if (!need[j] || bi[j] < 0) be[j] = -1;
+            }
+#pragma unroll
+            for (int j = 0; j < 6; j++) tb[j] = be[j] >= 0 ? d.bt[be[j]] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                if (be[j] < 0) continue;
+                int w;
+                if (ti[j] == tb[j]) w = bi[j] < be[j] ? bi[j] : be[j];
+                else w = ti[j] < tb[j] ? bi[j] : be[j];
+                if (w != c) keep[u] = false;
+            }
+        }
+        __syncthreads();                 // every list position read
+#pragma unroll
+        for (int u = 0; u < kPer; u++)
+            if (cc[u] >= 0) {
+                d.bcand[cc[u]] = keep[u] ? 1 : 0;
+                d.conf[cc[u]] = -1;
+            }
+        return;
     } else for (;;) {
         if (tid == 0) changed = 0;
         // phase A: votes on the flags at the start of the round
@@ -903,24 +997,50 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
 
 // clear_grains_fix_state (topological.h:1150-1163) for the flagged grains;
 // with with_area also the area update of main.cu:266 for them and for the
-// grains edited by the two-triangle stage (its dirty list).
-__global__ void k_fix_clear(Dev d, int stage, int with_area) {
+// grains edited by the two-triangle stage (its dirty list).  One wave
+// (workgroup of 64) per grain: the lanes reset the listed vertices' records
+// and load the list's points in parallel, each forming the wrapped delta to
+// its predecessor; lane 0 then accumulates the shoelace sum over the deltas
+// in list order, the same arithmetic as grain_area.
+__global__ void __launch_bounds__(64) k_fix_clear(Dev d, int stage, int with_area) {
+    __shared__ double sdx[kMaxStride], sdy[kMaxStride];
     const int n = d.cnt[C_FIX0 + stage];
     const int nd = with_area ? min(d.cnt[C_DIRTY], d.G) : 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n + nd; k += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x;
+    for (int item = blockIdx.x; item < n + nd; item += gridDim.x) {
         int g;
-        if (k < n) {
-            g = d.l_fix[stage][k];
-            d.gfix[g] = 0;
+        if (item < n) {
+            g = d.l_fix[stage][item];
+            if (lane == 0) d.gfix[g] = 0;
             if (!d.genab[g]) continue;
             const int* L = glist(d, g);
-            for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
+            const int m = d.glen[g];
+            for (int i = lane; i < m; i += 64) reset_mods(d, L[i]);
             if (!with_area) continue;
         } else {
-            g = d.l_dirty[k - n];
+            g = d.l_dirty[item - n];
             if (!d.genab[g]) continue;
         }
-        d.garea[g] = grain_area(d, g, d.prm[member_of_grain(d, g)].D);
+        const int m = d.glen[g];
+        const int* L = glist(d, g);
+        const double D = d.prm[member_of_grain(d, g)].D;
+        for (int i = 1 + lane; i < m; i += 64) {
+            const int w = L[i], wp = L[i - 1];
+            sdx[i] = wrapd(d.pos[2 * w], d.pos[2 * wp], D);
+            sdy[i] = wrapd(d.pos[2 * w + 1], d.pos[2 * wp + 1], D);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double a = 0.0;
+            if (m > 0) {
+                AreaAcc acc;
+                acc.start(0.0, 0.0);
+                for (int i = 1; i < m; i++) acc.push_delta(sdx[i], sdy[i]);
+                a = acc.finish();
+            }
+            d.garea[g] = a;
+        }
+        __syncthreads();
     }
 }
 
