@@ -182,8 +182,12 @@ def bench_adjoint(sevc, grains, seed, steps, ckpt, dt, n_fd=20, reps=3):
     # accuracy: a seeded linear functional w . x_N, whose reverse pass is
     # compared with Richardson central differences of the window forward on
     # n_fd - 2 seeded position coordinates (h = 1e-6), on GB_scaling and
-    # mobility (h = 1e-4), and along one seeded direction in SE (h = 1e-4;
-    # a single SE_g moves x_N by less than the rounding of a 20k-term sum)
+    # mobility (h = 1e-3), and along one seeded direction in SE (h = 3e-3;
+    # a single SE_g moves x_N by less than the rounding of a 20k-term sum).
+    # The rounding of w . x_N over a 500-step window is ~2e-13, so a
+    # difference quotient carries ~3e-13 / h of noise: parameter steps are
+    # taken large enough to keep that below 1e-6 of the derivative, which
+    # the O(h^4) Richardson truncation allows.
     pick = np.random.default_rng(seed + 1)
     w = pick.standard_normal(2 * mesh.nv) * np.repeat(mesh["v_enabled"], 2)
     u = pick.standard_normal(mesh.ng) * mesh["g_enabled"]
@@ -201,7 +205,7 @@ def bench_adjoint(sevc, grains, seed, steps, ckpt, dt, n_fd=20, reps=3):
 
     checks = [("pos", int(c), 1e-6) for c in pick.choice(np.nonzero(np.repeat(mesh["v_enabled"], 2))[0],
                                                           n_fd - 2, replace=False)]
-    checks += [("gb", 0, 1e-4), ("mu", 0, 1e-4), ("SE-dir", 0, 1e-4)]
+    checks += [("gb", 0, 1e-3), ("mu", 0, 1e-3), ("SE-dir", 0, 3e-3)]
     worst, detail = 0.0, []
     for kind, i, h in checks:
         def f(e):

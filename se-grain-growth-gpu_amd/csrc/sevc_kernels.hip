@@ -562,6 +562,55 @@ __global__ void k_label(Dev d) {
     if (c && d.benab[i]) {
         int k = atomicAdd(&d.cnt[C_CAND], 1);
         d.l_cand[k] = i;
+        d.conf[i] = k;              // list position, for k_poll_gather
+    }
+}
+
+// The boundaries polling and the conflict check look at, for each listed
+// candidate c at position k (nc <= kPollFast), mapped to list positions
+// through conf (k_label wrote conf[c] = k; -1 elsewhere), as columns of
+// d.pgat (column f at f * kPollFast):
+//   0..5    the boundaries of c's ends a0 (slots 0-2) and a1 (3-5), vb order;
+//   6..23   for slot j = 0..5 and r = 0..2 (column 6 + 3j + r): boundary r
+//           of the far end of the slot's edge x, -1 when x is c, x is
+//           disabled, or that boundary is x itself;
+//   24      bits 0-1: slot 0 of end a0 / a1 is c itself; bits 2-7: slot j
+//           holds an enabled edge other than c.
+// d.pgt[k] = t_ext of c.  Many workgroups issue these gathers, so that the
+// one-workgroup polling kernel reads contiguous columns only.
+__global__ void k_poll_gather(Dev d) {
+    const int nc = d.cnt[C_CAND];
+    if (nc > kPollFast) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const int c = d.l_cand[k];
+        const int a[2] = {d.bvx[2 * c], d.bvx[2 * c + 1]};
+        d.pgt[k] = d.bt[c];
+        int eb[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) eb[j] = d.vb[3 * a[j / 3] + j % 3];
+        int bits = (eb[0] == c ? 1 : 0) | (eb[3] == c ? 2 : 0);
+        int en[6], x0[6], x1[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            d.pgat[j * kPollFast + k] = d.conf[eb[j]];
+            en[j] = eb[j] != c && d.benab[eb[j]];
+            x0[j] = d.bvx[2 * eb[j]];
+            x1[j] = d.bvx[2 * eb[j] + 1];
+        }
+        int yw[6][3];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            if (en[j]) bits |= 4 << j;
+            const int W = en[j] ? (x0[j] == a[j / 3] ? x1[j] : x0[j]) : 0;
+#pragma unroll
+            for (int r = 0; r < 3; r++) yw[j][r] = en[j] ? d.vb[3 * W + r] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                d.pgat[(6 + 3 * j + r) * kPollFast + k] = (en[j] && yw[j][r] != eb[j]) ? d.conf[yw[j][r]] : -1;
+        d.pgat[24 * kPollFast + k] = bits;
     }
 }
 
@@ -584,66 +633,53 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         }
         return d.vb[3 * v + arg];
     };
-    constexpr int kPer = 2;
-    if (nc <= kPer * nt) {
-        // Up to kPer candidates per lane: their ends' boundaries and those
-        // boundaries' t_ext do not change during the rounds, so they are
-        // loaded once.  Only listed candidates carry a flag that can be set
-        // (the ends of an enabled edge touch enabled edges only, and k_label
-        // lists every enabled candidate), so each neighbour is mapped once
-        // to its list position through conf, and the rounds then read and
-        // clear flags in LDS only.
-        __shared__ int cf[kPer * 1024];
-        int cc[kPer], eb[kPer][6], ep[kPer][6], ends[kPer][2];
-        double et[kPer][6];
+    if (nc <= kPollFast && 2 * nt == kPollFast) {
+        // Up to two candidates per lane (kPollFast = 2 x 1024).  Their list
+        // positions, t_ext and flags sit in LDS; k_poll_gather has mapped
+        // every boundary the rounds and the conflict check look at to its
+        // list position (-1: not a listed candidate, whose flag cannot be
+        // set since the ends of an enabled edge touch enabled edges only
+        // and k_label lists every enabled candidate).  The rounds and the
+        // check then read LDS only.
+        constexpr int kPer = kPollFast / 1024;
+        __shared__ int cf[kPollFast], cid[kPollFast];
+        __shared__ double ct[kPollFast];
+        int ep[kPer][6], bits[kPer];
         bool two[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             const int k = tid + u * nt;
-            cc[u] = k < nc ? d.l_cand[k] : -1;
-            if (cc[u] >= 0) { d.conf[cc[u]] = k; cf[k] = 1; }
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            // listed candidates are enabled edges, whose ends and boundaries are valid
-            const int c = cc[u];
-            const int a0 = c >= 0 ? d.bvx[2 * c] : 0, a1 = c >= 0 ? d.bvx[2 * c + 1] : 0;
-            ends[u][0] = a0;
-            ends[u][1] = a1;
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                eb[u][q] = c >= 0 ? d.vb[3 * a0 + q] : 0;
-                eb[u][3 + q] = c >= 0 ? d.vb[3 * a1 + q] : 0;
+            const bool on = k < nc;
+            if (on) {
+                cid[k] = d.l_cand[k];
+                ct[k] = d.pgt[k];
+                cf[k] = 1;
             }
+#pragma unroll
+            for (int j = 0; j < 6; j++) ep[u][j] = on ? d.pgat[j * kPollFast + k] : -1;
+            bits[u] = on ? d.pgat[24 * kPollFast + k] : 0;
         }
-#pragma unroll
-        for (int u = 0; u < kPer; u++)
-#pragma unroll
-            for (int j = 0; j < 6; j++) et[u][j] = cc[u] >= 0 ? d.bt[eb[u][j]] : 0.0;
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < kPer; u++)
-#pragma unroll
-            for (int j = 0; j < 6; j++) ep[u][j] = cc[u] >= 0 ? d.conf[eb[u][j]] : -1;
-        // vertices_vote (topological.h:362-386): the end's candidate with the
-        // smallest t_ext, later slots winning ties (<=); slot 0 if none
-        auto vote = [&](int u, int e) {
+        // vertices_vote (topological.h:362-386) at end e of the candidate at
+        // position k: the end's flagged candidate with the smallest t_ext,
+        // later slots winning ties (<=), or slot 0 if none; true when the
+        // vote goes to the candidate itself
+        auto votes_self = [&](int u, int e, int k) {
             double min_t = 99999999;
-            int arg = 0;
+            int arg = -1;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int p = ep[u][3 * e + k];
-                if (et[u][3 * e + k] <= min_t && p >= 0 && cf[p]) { min_t = et[u][3 * e + k]; arg = k; }
+            for (int r = 0; r < 3; r++) {
+                const int p = ep[u][3 * e + r];
+                if (p >= 0 && cf[p] && ct[p] <= min_t) { min_t = ct[p]; arg = p; }
             }
-            return eb[u][3 * e + arg];
+            return arg >= 0 ? arg == k : ((bits[u] >> e) & 1) != 0;
         };
         for (;;) {
             if (tid == 0) changed = 0;
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
-                two[u] = false;
-                if (cc[u] >= 0 && cf[tid + u * nt])
-                    two[u] = (vote(u, 0) == cc[u]) && (vote(u, 1) == cc[u]);
+                const int k = tid + u * nt;
+                two[u] = k < nc && cf[k] && votes_self(u, 0, k) && votes_self(u, 1, k);
             }
             __syncthreads();
 #pragma unroll
@@ -652,7 +688,7 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
 #pragma unroll
                     for (int j = 0; j < 6; j++) {
                         const int p = ep[u][j];
-                        if (eb[u][j] != cc[u] && p >= 0 && cf[p]) { cf[p] = 0; changed = 1; }
+                        if (p >= 0 && p != tid + u * nt && cf[p]) { cf[p] = 0; changed = 1; }
                     }
             __syncthreads();
             int ch = changed;
@@ -661,82 +697,52 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         }
         // check_conflictive_boundaries and unlabel_conflictive_boundaries
         // on the final flags, as one decision per flagged candidate c: c is
-        // dropped when one of its non-candidate enabled neighbours x has a
+        // dropped when one of its enabled neighbours x without a flag has a
         // flagged candidate at each end (the first in slot order, bi at the
-        // end shared with c, be at the other) and c is not the winner of the
-        // two (smaller t_ext, lower index on a tie; symmetric in bi, be).
-        // The reference writes that winner into x and lets c read it back;
-        // here c forms it itself.  Loads go out in rounds (the neighbours'
-        // flags and ends, then the far ends' boundaries, then their list
-        // positions, then t_ext of be), with index 0 standing in for slots
-        // that need none.
+        // end shared with c, be at the far end) and c is not the winner of
+        // the two (smaller t_ext, lower index on a tie; symmetric in bi,
+        // be).  The reference writes that winner into x and lets c read it
+        // back; here c forms it itself.
         auto flagp = [&](int p) { return p >= 0 && cf[p] != 0; };
         bool keep[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
-            const int c = cc[u];
-            keep[u] = c >= 0 && cf[tid + u * nt] != 0;
-            bool need[6];
-            int xi[6];
-#pragma unroll
-            for (int j = 0; j < 6; j++) {
-                need[j] = keep[u] && eb[u][j] != c && !flagp(ep[u][j]);
-                xi[j] = need[j] ? eb[u][j] : 0;
-            }
-            int xe[6], x0[6], x1[6];
-#pragma unroll
-            for (int j = 0; j < 6; j++) {
-                xe[j] = d.benab[xi[j]];
-                x0[j] = d.bvx[2 * xi[j]];
-                x1[j] = d.bvx[2 * xi[j] + 1];
-            }
-            int yw[6][3];
-#pragma unroll
-            for (int j = 0; j < 6; j++) {
-                need[j] = need[j] && xe[j];
-                const int V = ends[u][j / 3];
-                const int W = need[j] ? (x0[j] == V ? x1[j] : x0[j]) : 0;
-#pragma unroll
-                for (int r = 0; r < 3; r++) yw[j][r] = need[j] ? d.vb[3 * W + r] : 0;
-            }
+            const int k = tid + u * nt;
+            keep[u] = k < nc && cf[k] != 0;
             int pw[6][3];
 #pragma unroll
             for (int j = 0; j < 6; j++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) pw[j][r] = need[j] ? d.conf[yw[j][r]] : -1;
-            int bi[6], be[6];
-            double ti[6], tb[6];
+                for (int r = 0; r < 3; r++) pw[j][r] = keep[u] ? d.pgat[(6 + 3 * j + r) * kPollFast + k] : -1;
 #pragma unroll
             for (int j = 0; j < 6; j++) {
-                const int x = eb[u][j], e3 = 3 * (j / 3);
-                bi[j] = -1; ti[j] = 0.0;
+                if (!keep[u] || !((bits[u] >> (2 + j)) & 1) || flagp(ep[u][j])) continue;
+                int pbi = -1, pbe = -1;
+                const int e3 = 3 * (j / 3);
 #pragma unroll
-                for (int r = 2; r >= 0; r--)          // first in slot order
-                    if (eb[u][e3 + r] != x && flagp(ep[u][e3 + r])) { bi[j] = eb[u][e3 + r]; ti[j] = et[u][e3 + r]; }
-                be[j] = -1;
-#pragma unroll
-                for (int r = 2; r >= 0; r--)
-                    if (yw[j][r] != x && flagp(pw[j][r])) be[j] = yw[j][r];
-                if (!need[j] || bi[j] < 0) be[j] = -1;
-            }
-#pragma unroll
-            for (int j = 0; j < 6; j++) tb[j] = be[j] >= 0 ? d.bt[be[j]] : 0.0;
-#pragma unroll
-            for (int j = 0; j < 6; j++) {
-                if (be[j] < 0) continue;
+                for (int r = 2; r >= 0; r--) {           // first in slot order
+                    if (flagp(ep[u][e3 + r])) pbi = ep[u][e3 + r];
+                    if (flagp(pw[j][r])) pbe = pw[j][r];
+                }
+                if (pbi < 0 || pbe < 0) continue;
+                const double ti = ct[pbi], tb = ct[pbe];
+                const int bi = cid[pbi], be = cid[pbe];
                 int w;
-                if (ti[j] == tb[j]) w = bi[j] < be[j] ? bi[j] : be[j];
-                else w = ti[j] < tb[j] ? bi[j] : be[j];
-                if (w != c) keep[u] = false;
+                if (ti == tb) w = bi < be ? bi : be;
+                else w = ti < tb ? bi : be;
+                if (w != cid[k]) keep[u] = false;
             }
         }
-        __syncthreads();                 // every list position read
+        __syncthreads();                 // every flag read
 #pragma unroll
-        for (int u = 0; u < kPer; u++)
-            if (cc[u] >= 0) {
-                d.bcand[cc[u]] = keep[u] ? 1 : 0;
-                d.conf[cc[u]] = -1;
+        for (int u = 0; u < kPer; u++) {
+            const int k = tid + u * nt;
+            if (k < nc) {
+                const int c = cid[k];
+                d.bcand[c] = keep[u] ? 1 : 0;
+                d.conf[c] = -1;
             }
+        }
         return;
     } else for (;;) {
         if (tid == 0) changed = 0;

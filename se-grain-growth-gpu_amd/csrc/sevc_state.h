@@ -16,6 +16,7 @@ constexpr int kFixStages = 3;        // 0: two-grain, 1: T2 removal, 2: flips
 constexpr int kTile = 64;            // grains per velocity tile (pass 1)
 constexpr int kMaxHistBins = 64;     // bins of the device area histogram
 constexpr int kNuclThreads = 1024;   // workgroup of the nucleation manager (one per member)
+constexpr int kPollFast = 2048;      // candidate lists up to this length poll from LDS (two per lane)
 
 // Per-member run parameters (from sevc_options).
 struct Params {
@@ -51,6 +52,7 @@ struct Dev {
     double* nucl_r; double* nucl_ori; double* dE; double* nfac; uint8_t* vnuc;
     // boundaries
     int* bvx; double* ben; double* blen; double* bt; uint8_t* benab; uint8_t* bcand; int* conf;
+    int* pgat; double* pgt;          // polling gathers: 25 columns of kPollFast, t_ext
     // grains
     int* gvx; int* glen; double* garea; double* gdadt; double* gori; double* gse;
     uint8_t* genab; uint8_t* gtype; uint8_t* gcand; int* gfix;

@@ -326,3 +326,30 @@ def test_degenerate_mesh_error_matches_oracle(n, gen_seed):
         assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}, chunk
     assert gpu_fail == orc_fail and gpu_fail[1] == sevc.E_STATE, (gpu_fail, orc_fail)
     st.close()
+
+
+@pytest.mark.parametrize("n,factor", [(8000, 1.0), (5000, 1.5)])
+def test_long_candidate_lists(n, factor):
+    """A flip threshold of the order of the mean edge length makes every
+    short shrinking edge a candidate: about 2,000 listed candidates per step
+    for (8000, 1.0), the size where polling switches from the in-LDS lists
+    (at most 2,048) to the general rounds, and 3,000-4,000 for (5000, 1.5).
+    Flips, removals and the whole state match the oracle bit for bit."""
+    o, mesh = _generated(n, 21, dt=1e-5, flip_threshold=factor * 0.6 * np.sqrt(1.0 / n))
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    for c in (1, 1, 3, 5):
+        print(f"  chunk of {c} steps", flush=True)
+        s1 = st.step(c)
+        rc, s2 = orc.step(c)
+        assert rc == 0 and s1.status == 0, (rc, s1.status)
+        a, b = st.get_mesh(), orc.get_mesh()
+        assert diff_fields(a, b) == {}
+        for k in STAT_KEYS:
+            assert getattr(s1, k) == getattr(s2, k), (k, getattr(s1, k), getattr(s2, k))
+        if c == 1 and factor > 1.0:
+            # the candidate rule on the state after one step, as an estimate of
+            # the list length polling sees: well above 2,048
+            bt, bl, en = b["b_t_ext"], b["b_arclength"], b["b_enabled"]
+            nc = np.sum((((bt >= 0) & (bt <= o.dt)) | ((bt >= 0) & (bl > 0) & (bl < o.flip_threshold))) & (en == 1))
+            assert nc > 2048, nc
+    st.close()
