@@ -47,7 +47,7 @@ __global__ void k_nucl_area_up(Dev d);
 __global__ void k_nucl_factor(Dev d);
 __global__ void k_rate_cand(Dev d);
 __global__ void k_nucl_manager(Dev d);
-__global__ void k_nucl_refresh(Dev d);
+__global__ void k_nucl_insert(Dev d);
 // kernels of sevc_adjoint.hip
 __global__ void k_win_prepare(Dev d, double* ebase);
 __global__ void k_win_area(Dev d, const double* X, double* A, int to_state);
@@ -185,6 +185,8 @@ constexpr int kListGrid = 512;
 // one wave per flagged grain in the fix stages: enough workgroups that
 // every grain of a busy step has its own
 constexpr int kFixGrid = 2048;
+// one wave per nucleation event
+constexpr int kNuclInsGrid = 256;
 constexpr int kTileMin = 16;             // smallest tile the autotuner tries
 
 // Tile table of velocity pass 1: T consecutive grains of one member per
@@ -220,7 +222,7 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
         launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
         launch(s, "nucl_manager", k_nucl_manager, s->B, kNuclThreads, d);
-        launch(s, "nucl_refresh", k_nucl_refresh, kListGrid, 256, d);
+        launch(s, "nucl_insert", k_nucl_insert, kNuclInsGrid, 64, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     }
     if (diag) {
@@ -365,7 +367,8 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     for (int k = 0; k < kFixStages; ++k) DA(d.l_fix[k], G);
     DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
     DA(d.nscr, 12 * G + 16);
-    DA(d.l_nde, 6 * G + 16); DA(d.l_ndg, 4 * G + 16); DA(d.l_vfall, V + 16);
+    DA(d.l_vfall, V + 16);
+    DA(d.nrec, 4 * G + 16); DA(d.nal, G + 16); DA(d.l_nev, 8 * G + 16);
 
     // block tables
     std::vector<Blk> vt, et, gt;

@@ -13,6 +13,9 @@
 //  * deferred grain-list edits (topological.h:1209-1227) touch only the
 //    edited grain's own list, so they run one lane per flagged grain.
 #include <hip/hip_runtime.h>
+
+#include <climits>
+
 #include "sevc_state.h"
 #include "sevc_geom.h"
 #include "../../include/sevc.h"
@@ -1582,7 +1585,7 @@ struct NuclGeom {
 };
 
 // generate_grain_for_nucleation, nucleation2.cu:116-212
-__device__ int generate_grain(const Dev& d, const Params& P, int A, double eps, double scaling, int do_scale,
+__device__ __forceinline__ int generate_grain(const Dev& d, const Params& P, int A, double eps, double scaling, int do_scale,
                               NuclGeom* r) {
     const double Dm = P.D;
     r->AD = d.vb[3 * A + 0]; r->AE = d.vb[3 * A + 1]; r->AF = d.vb[3 * A + 2];
@@ -1662,7 +1665,11 @@ __global__ void k_nucl_factor(Dev d) {
 }
 
 // Rate mode: grain g of a nucleating member is a candidate with
-// probability nucleation_rate * SE_g (Philox(seed; step, local grain, 3, 0)).
+// probability nucleation_rate * SE_g (Philox(seed; step, local grain, 3, 0),
+// first uniform).  The second uniform of the same draw picks the site, the
+// list entry floor(u1 * n) of the grain; the candidate's record holds the
+// site (-1 when its shortest boundary has zero length: no candidate), the
+// site's three grains and that shortest length.
 __global__ void k_rate_cand(Dev d) {
     TABLE_BLOCK(d.gblk, d.ngb)
     if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_RATE) return;
@@ -1671,8 +1678,24 @@ __global__ void k_rate_cand(Dev d) {
     double u0, u1;
     sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(i - P.goff), kStreamRate, 0u, &u0, &u1);
     if (!(u0 < p)) return;
+    const int len = d.glen[i];
+    int k = (int)(u1 * (double)len);
+    if (k >= len) k = len - 1;
+    const int A = glist(d, i)[k];
+    int vb[3], vg[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) { vb[q] = d.vb[3 * A + q]; vg[q] = d.vg[3 * A + q]; }
+    double al[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) al[q] = d.blen[vb[q]];
+    double minAL = al[0];
+    for (int q = 0; q < 3; q++) if (minAL > al[q]) minAL = al[q];
     int s = atomicAdd(&d.cnt[C_NUCL], 1);
     d.l_nucl[s] = i;
+    d.nrec[4 * s] = (minAL > 0) ? A : -1;
+#pragma unroll
+    for (int q = 0; q < 3; q++) d.nrec[4 * s + 1 + q] = vg[q];
+    d.nal[s] = minAL;
 }
 
 // Exclusive prefix sum of one int per thread over the workgroup: an
@@ -1740,47 +1763,126 @@ __device__ RunScan run_scan(const Blk* tbl, int first, int nblk, const int2* bc,
     return r;
 }
 
-// Collect, in index order, the first R slots of a member whose enable flag
-// equals `want`, cooperatively in one workgroup.  Lanes whose run holds
-// ranks below R list (block, first rank) for each block with wanted slots;
-// the waves then sweep the listed blocks 64 slots at a time and write each
-// wanted slot at its rank.  `blist` holds at least nblk entries; `sh` is
-// LDS scratch of blockDim.x + 1 ints.  Returns min(R, wanted slots).
-__device__ int collect_slots(const Blk* tbl, int first, int nblk, const int2* bc, const uint8_t* en, int want,
-                             int R, int* out, int* sh, int2* blist) {
-    __shared__ int nl;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const RunScan rs = run_scan(tbl, first, nblk, bc, want, sh);
-    if (tid == 0) nl = 0;
-    __syncthreads();
-    if (rs.cnt > 0 && rs.pre < R) {
-        int rank = rs.pre;
-        for (int b0 = rs.lo; b0 < rs.hi && rank < R; b0 += 8) {
-            int t[8];                              // 8 independent loads, then in order
+// Exclusive prefix sums of three ints per thread over the workgroup, in one
+// pass (wg_exclusive_scan for each of them, sharing its two barriers).
+// `sh` holds >= 3 * 65 ints.
+__device__ __forceinline__ void wg_exclusive_scan3(const int x[3], int* sh, int pre[3], int tot[3]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
+    int inc[3] = {x[0], x[1], x[2]};
+    for (int o = 1; o < 64; o <<= 1) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) t[q] = (b0 + q < rs.hi) ? block_matches(bc, b0 + q, want) : 0;
+        for (int t = 0; t < 3; t++) {
+            const int v = __shfl_up(inc[t], o, 64);
+            if (lane >= o) inc[t] += v;
+        }
+    }
+    if (lane == 63)
+#pragma unroll
+        for (int t = 0; t < 3; t++) sh[65 * t + wave] = inc[t];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int v = lane < nw ? sh[65 * t + lane] : 0;
+            int vi = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(vi, o, 64);
+                if (lane >= o) vi += u;
+            }
+            if (lane < nw) sh[65 * t + lane] = vi - v;
+            if (lane == nw - 1) sh[65 * t + 64] = vi;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        pre[t] = sh[65 * t + wave] + inc[t] - x[t];
+        tot[t] = sh[65 * t + 64];
+    }
+    __syncthreads();
+}
+
+// One kind of slot of a member to collect: its block table range, per-block
+// (enabled, free) counts, enable flags, how many free slots are wanted,
+// where they go, and a (block, rank) list of at least nblk entries.
+struct SlotKind {
+    const Blk* tbl;
+    int first, nblk;
+    const int2* bc;
+    const uint8_t* en;
+    int R;
+    int* out;
+    int2* blist;
+};
+
+// The first R free slots (enable flag 0) of each of three kinds, in index
+// order, cooperatively in one workgroup: each lane takes one run of
+// consecutive blocks of every kind; a scan of the runs' free counts tells
+// each lane which ranks its run holds; lanes whose run holds wanted ranks
+// list (block, first rank) for its blocks with free slots; the waves then
+// sweep the listed blocks of all three kinds 64 slots at a time and write
+// each free slot at its rank.  got[t] = min(R_t, free slots of kind t).
+// `sh` holds >= 3 * 65 ints.
+__device__ __forceinline__ void collect_free3(const SlotKind* K, int* got, int* sh) {
+    __shared__ int nl[3];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int lo[3], hi[3], c[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int C = (K[t].nblk + nt - 1) / nt;
+        lo[t] = K[t].first + min(K[t].nblk, tid * C);
+        hi[t] = K[t].first + min(K[t].nblk, (tid + 1) * C);
+        int n = 0, b = lo[t];
+        for (; b + 8 <= hi[t]; b += 8) {
+            int v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = K[t].bc[b + q].y;
+#pragma unroll
+            for (int q = 0; q < 8; q++) n += v[q];
+        }
+        for (; b < hi[t]; b++) n += K[t].bc[b].y;
+        c[t] = n;
+    }
+    if (tid < 3) nl[tid] = 0;
+    int pre[3], tot[3];
+    wg_exclusive_scan3(c, sh, pre, tot);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        if (c[t] == 0 || pre[t] >= K[t].R) continue;
+        int rank = pre[t];
+        for (int b0 = lo[t]; b0 < hi[t] && rank < K[t].R; b0 += 8) {
+            int v[8];                               // 8 independent loads, then in order
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (b0 + q < hi[t]) ? K[t].bc[b0 + q].y : 0;
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                if (t[q] > 0 && rank < R) {
-                    blist[atomicAdd(&nl, 1)] = make_int2(b0 + q, rank);
-                    rank += t[q];
+                if (v[q] > 0 && rank < K[t].R) {
+                    K[t].blist[atomicAdd(&nl[t], 1)] = make_int2(b0 + q, rank);
+                    rank += v[q];
                 }
         }
     }
     __syncthreads();
-    const int n = nl;
+    const int n0 = nl[0], n1 = nl[1], n = n0 + n1 + nl[2];
     const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int q = wave; q < n; q += nw) {
-        // one wave per listed block: the block's flags are loaded at once
-        // (kBlock/64 per lane), then ranked 64 slots at a time
-        const int2 e = blist[q];
-        const int lo = tbl[e.x].b, hi = tbl[e.x].e;
+        // one wave per listed block: its flags are loaded at once (kBlock/64
+        // per lane), then ranked 64 slots at a time
+        const int t = q < n0 ? 0 : (q < n0 + n1 ? 1 : 2);
+        // the kind's fields by selection (no indexing of K by a run-time t)
+        const Blk* tbl = t == 0 ? K[0].tbl : (t == 1 ? K[1].tbl : K[2].tbl);
+        const uint8_t* en = t == 0 ? K[0].en : (t == 1 ? K[1].en : K[2].en);
+        int* out = t == 0 ? K[0].out : (t == 1 ? K[1].out : K[2].out);
+        const int2* bl = t == 0 ? K[0].blist : (t == 1 ? K[1].blist - n0 : K[2].blist - (n0 + n1));
+        const int R = t == 0 ? K[0].R : (t == 1 ? K[1].R : K[2].R);
+        const int2 e = bl[q];
+        const int blo = tbl[e.x].b, bhi = tbl[e.x].e;
         bool hit[kBlock / 64];
 #pragma unroll
         for (int k = 0; k < kBlock / 64; k++) {
-            const int idx = lo + 64 * k + lane;
-            hit[k] = idx < hi && ((en[idx] != 0) == (want != 0));
+            const int idx = blo + 64 * k + lane;
+            hit[k] = idx < bhi && en[idx] == 0;
         }
         int rank = e.y;
 #pragma unroll
@@ -1788,93 +1890,14 @@ __device__ int collect_slots(const Blk* tbl, int first, int nblk, const int2* bc
             const unsigned long long hb = __ballot(hit[k]);
             if (hit[k]) {
                 const int r = rank + (int)__popcll(hb & below);
-                if (r < R) out[r] = lo + 64 * k + lane;
+                if (r < R) out[r] = blo + 64 * k + lane;
             }
             rank += (int)__popcll(hb);
         }
     }
     __syncthreads();
-    return rs.total < R ? rs.total : R;
-}
-
-// nucleate, nucleation2.cu:474-676 (after its checks), with given slots.
-__device__ int nucleate_at(const Dev& d, const Params& P, int A, int B, int C, int AB, int BC, int CA, int g) {
-    NuclGeom r;
-    // nucleation2.cu:526 passes (scaling, eps) swapped; only dE is affected
-    if (generate_grain(d, P, A, P.gb_scaling, P.gb_eps, 1, &r) != 0) return -1;
-    // init_vertex (geometry.h:346-361) for B and C
-    const int nv2[2] = {B, C};
-    const double* np2[2] = {r.Bpos, r.Cpos};
-    for (int q = 0; q < 2; q++) {
-        int v = nv2[q];
-        d.pos[2 * v] = np2[q][0]; d.pos[2 * v + 1] = np2[q][1];
-        d.vel[2 * v] = 0.0; d.vel[2 * v + 1] = 0.0;
-        d.venergy[v] = 0.0;
-        d.ven[v] = 1;
-        d.vnuc[v] = 0;
-        for (int k = 0; k < 3; k++) { d.vg[3 * v + k] = -1; d.vb[3 * v + k] = -1; }
-        reset_mods(d, v);
-    }
-    d.pos[2 * A] = r.Apos[0]; d.pos[2 * A + 1] = r.Apos[1];
-    for (int k = 0; k < 3; k++) d.vb[3 * A + k] = -1;
-    auto add_bnd = [&](int v, int b) {
-        for (int k = 0; k < 3; k++) if (d.vb[3 * v + k] < 0) { d.vb[3 * v + k] = b; return; }
-        set_error(d, SEVC_E_STATE);
-    };
-    auto add_grn = [&](int v, int gg) {
-        for (int k = 0; k < 3; k++) if (d.vg[3 * v + k] < 0) { d.vg[3 * v + k] = gg; return; }
-        set_error(d, SEVC_E_STATE);
-    };
-    auto init_bnd = [&](int b, int ini, int end) {          // geometry.h:491-504
-        d.bvx[2 * b] = ini; add_bnd(ini, b);
-        d.bvx[2 * b + 1] = end; add_bnd(end, b);
-        d.blen[b] = 0;
-        d.bt[b] = SEVC_MAX_T_EXT;
-        d.benab[b] = 1;
-        d.bcand[b] = 0;
-        d.conf[b] = -1;
-    };
-    init_bnd(AB, A, B);
-    init_bnd(BC, B, C);
-    init_bnd(CA, C, A);
-    add_bnd(A, r.AD);
-    int* Lg = glist(d, g);                                  // nucleate_grain, geometry.h:596-606
-    for (int k = 0; k < d.S; k++) Lg[k] = -1;
-    d.glen[g] = 0;
-    d.garea[g] = 0;
-    d.gse[g] = 0;
-    d.gori[g] = 0;
-    d.genab[g] = 1;
-    d.gcand[g] = 0;
-    d.gfix[g] = 0;
-    d.gtype[g] = T_NUCLEATED;
-    d.gori[g] = d.nucl_ori[A];
-    Lg[0] = A; Lg[1] = B; Lg[2] = C;
-    d.glen[g] = 3;
-    for (int k = 0; k < 3; k++) if (d.vg[3 * A + k] == r.g3) { d.vg[3 * A + k] = g; break; }
-    add_grn(B, g); add_grn(B, r.g2); add_grn(B, r.g3);
-    add_grn(C, g); add_grn(C, r.g3); add_grn(C, r.g1);
-    list_add_after(d, r.g1, C, A);
-    if (d.bvx[2 * r.AE] == A) list_add_after(d, r.g2, B, d.bvx[2 * r.AE + 1]);
-    else if (d.bvx[2 * r.AE + 1] == A) list_add_after(d, r.g2, B, d.bvx[2 * r.AE]);
-    list_add_after(d, r.g3, C, A);
-    list_add_after(d, r.g3, B, C);
-    list_remove(d, r.g3, A);
-    int t = d.vb[3 * B + 1]; d.vb[3 * B + 1] = d.vb[3 * B]; d.vb[3 * B] = t;
-    t = d.vb[3 * C + 1]; d.vb[3 * C + 1] = d.vb[3 * C]; d.vb[3 * C] = t;
-    if (d.bvx[2 * r.AE] == A) d.bvx[2 * r.AE] = B;
-    else if (d.bvx[2 * r.AE + 1] == A) d.bvx[2 * r.AE + 1] = B;
-    d.vb[3 * B + 2] = r.AE;
-    if (d.bvx[2 * r.AF] == A) d.bvx[2 * r.AF] = C;
-    else d.bvx[2 * r.AF + 1] = C;
-    d.vb[3 * C + 2] = r.AF;
-    const int de[6] = {AB, BC, CA, r.AD, r.AE, r.AF};
-    int k0 = atomicAdd(&d.cnt[C_NDE], 6);
-    for (int q = 0; q < 6; q++) if (k0 + q < 6 * d.G) d.l_nde[k0 + q] = de[q];
-    const int dg[4] = {g, r.g1, r.g2, r.g3};
-    int k1 = atomicAdd(&d.cnt[C_NDG], 4);
-    for (int q = 0; q < 4; q++) if (k1 + q < 4 * d.G) d.l_ndg[k1 + q] = dg[q];
-    return 0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) got[t] = tot[t] < K[t].R ? tot[t] : K[t].R;
 }
 
 // The k-th (0-based) slot of a member whose enable flag equals `want`,
@@ -1920,26 +1943,31 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
     return f;
 }
 
-// One workgroup per member: picks the nucleation events of this step,
-// assigns free slots and applies the events.
+// One workgroup per member: picks the nucleation events of this step and
+// assigns their free slots; k_nucl_insert applies them.
 //  MC mode (nucleation2.cu:292-443, 33-72, 474): one uniformly chosen alive
 //  vertex, Metropolis acceptance, first free grain / 3 boundaries / 2
 //  vertices.
 //  RATE mode: candidates in grain order; a candidate is dropped when an
 //  earlier candidate's site shares a grain with it; kept events take the
-//  free slots in rank order and, being grain-disjoint, are applied in
-//  parallel.
+//  free slots in rank order.  Kept events are grain-disjoint, so they can
+//  be applied in parallel.
 // Scratch: 12 ints per grain slot of the member, at 12*goff in d.nscr:
 //  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
 //  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free
 //  vertices, [11ng,12ng) candidate sites.
 constexpr int kNuclLds = 1024;       // rate-mode candidates held in LDS
+constexpr int kNuclHash = 4096;      // grain table of the clash test (> 3 kNuclLds keys: never full)
+constexpr int kHashEmpty = INT_MIN;
+
+__device__ __forceinline__ int nucl_hash(int g) { return (int)(((unsigned)g * 2654435761u) >> 20); }
 
 __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     __shared__ int sh[kNuclThreads + 1];
-    __shared__ int nkeep, nmade, fg_n, fe_n, fv_n, ncand;
-    __shared__ int c_s[kNuclLds], site_s[kNuclLds], sg_s[3 * kNuclLds], k_s[kNuclLds];
+    __shared__ int nkeep, ncand, ev0;
+    __shared__ int c_s[kNuclLds], site_s[kNuclLds], sg_s[3 * kNuclLds], k_s[kNuclLds], q_s[kNuclLds];
     __shared__ double al_s[kNuclLds];         // shortest edge at each candidate's site
+    __shared__ int hk[kNuclHash], hv[kNuclHash];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
@@ -1955,7 +1983,7 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     int* fe = base + 6 * (size_t)ng;
     int* fv = base + 9 * (size_t)ng;
     int* site = base + 11 * (size_t)ng;
-    if (tid == 0) { nkeep = 0; nmade = 0; ncand = 0; }
+    if (tid == 0) { nkeep = 0; ncand = 0; }
     __syncthreads();
     if (P.nucl_mode == SEVC_NUCL_MC) {
         const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
@@ -1977,142 +2005,368 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
             if (d.vnuc[A]) { kept[0] = A; nkeep = 1; }
         }
     } else {
-        // this member's candidates, then grain order.  Up to kNuclLds of
-        // them are handled in LDS (c_s, site_s, sg_s, k_s); beyond that the
-        // same code runs on the global scratch.
+        // this member's candidates (grain, list index), then grain order
         const int total = d.cnt[C_NUCL];
         for (int q = tid; q < total; q += nt) {
-            int g = d.l_nucl[q];
+            const int g = d.l_nucl[q];
             if (g >= P.goff && g < P.goff + ng) {
                 const int e = atomicAdd(&ncand, 1);
                 cand[e] = g;
-                if (e < kNuclLds) k_s[e] = g;
+                if (e < kNuclLds) { k_s[e] = g; q_s[e] = q; }
             }
         }
         __syncthreads();
         const int nc = ncand;
-        const bool small = nc <= kNuclLds;
-        int* uns = small ? k_s : kept;            // unsorted copy, later the keep flags
-        int* srt = small ? c_s : cand;            // candidates in grain order
-        int* ss = small ? site_s : site;
-        int* g3 = small ? sg_s : sgr;
-        double* al = small ? al_s : nullptr;      // beyond kNuclLds: recomputed when kept
-        if (!small) {
+        if (nc <= kNuclLds) {
+            // In LDS.  Rank sort (distinct grain ids); the sorted list
+            // indices wait in site_s until the records replace them.
+            for (int e = tid; e < nc; e += nt) {
+                const int g = k_s[e];
+                int r = 0;
+                for (int f = 0; f < nc; f++) r += k_s[f] < g ? 1 : 0;
+                c_s[r] = g;
+                site_s[r] = q_s[e];
+            }
+            for (int i = tid; i < kNuclHash; i += nt) { hk[i] = kHashEmpty; hv[i] = INT_MAX; }
+            __syncthreads();
+            for (int e = tid; e < nc; e += nt) {
+                const int q = site_s[e];
+                site_s[e] = d.nrec[4 * q];
+#pragma unroll
+                for (int a = 0; a < 3; a++) sg_s[3 * e + a] = d.nrec[4 * q + 1 + a];
+                al_s[e] = d.nal[q];
+            }
+            __syncthreads();
+            // every grain at a valid site -> the first candidate (in grain
+            // order) whose site touches it
+            for (int e = tid; e < nc; e += nt) {
+                if (site_s[e] < 0) continue;
+                for (int a = 0; a < 3; a++) {
+                    const int key = sg_s[3 * e + a];
+                    for (int h = nucl_hash(key);; h = (h + 1) & (kNuclHash - 1)) {
+                        const int prev = atomicCAS(&hk[h], kHashEmpty, key);
+                        if (prev == kHashEmpty || prev == key) { atomicMin(&hv[h], e); break; }
+                    }
+                }
+            }
+            __syncthreads();
+            // kept iff no earlier candidate touches one of its grains
+            for (int e = tid; e < nc; e += nt) {
+                bool clash = site_s[e] < 0;
+                for (int a = 0; a < 3 && !clash; a++) {
+                    const int key = sg_s[3 * e + a];
+                    for (int h = nucl_hash(key);; h = (h + 1) & (kNuclHash - 1))
+                        if (hk[h] == key) { clash = hv[h] < e; break; }
+                }
+                k_s[e] = clash ? 0 : 1;
+            }
+            __syncthreads();
+            // stable compaction of the kept candidates, one workgroup scan
+            // per nt candidates (values are read before the scan, written
+            // after)
+            int off = 0;
+            for (int b0 = 0; b0 < nc; b0 += nt) {
+                const int e = b0 + tid;
+                const int flag = e < nc ? k_s[e] : 0;
+                const int g = flag ? c_s[e] : 0, A = flag ? site_s[e] : 0;
+                const double la = flag ? al_s[e] : 0.0;
+                int tot;
+                const int pre = wg_exclusive_scan(flag, sh, &tot);
+                if (flag) {
+                    c_s[off + pre] = g;
+                    site_s[off + pre] = A;
+                    al_s[off + pre] = la;
+                }
+                off += tot;
+                __syncthreads();
+            }
+            if (tid == 0) nkeep = off;
+            __syncthreads();
+            for (int r = tid; r < nkeep; r += nt) {
+                const int g = c_s[r], A = site_s[r];
+                double u2, u3;
+                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
+                d.nucl_r[A] = P.nucl_scaling * al_s[r] * u2;
+                d.nucl_ori[A] = 2 * SEVC_PI * u3;
+                kept[r] = A;
+            }
+        } else {
+            // Beyond kNuclLds: the same selection on the global scratch,
+            // with the clash test over all earlier candidates.
+            int* uns = kept;                      // unsorted copy, later the keep flags
             for (int e = tid; e < nc; e += nt) uns[e] = cand[e];
             __syncthreads();
-        }
-        // rank sort (distinct grain ids)
-        for (int e = tid; e < nc; e += nt) {
-            const int g = uns[e];
-            int r = 0;
-            for (int f = 0; f < nc; f++) r += uns[f] < g ? 1 : 0;
-            srt[r] = g;
-        }
-        __syncthreads();
-        for (int e = tid; e < nc; e += nt) {
-            const int g = srt[e];
-            double u0, u1;
-            sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
-            const int len = d.glen[g];
-            int k = (int)(u1 * (double)len);
-            if (k >= len) k = len - 1;
-            const int A = glist(d, g)[k];
-            double minAL = d.blen[d.vb[3 * A]];
-            for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
-            // a site on a zero-length boundary is not a candidate
-            ss[e] = (minAL > 0) ? A : -1;
-            if (al) al[e] = minAL;
-            for (int a = 0; a < 3; a++) g3[3 * e + a] = d.vg[3 * A + a];
-        }
-        __syncthreads();
-        // kept iff no earlier candidate touches one of its grains
-        for (int e = tid; e < nc; e += nt) {
-            bool clash = ss[e] < 0;
-            for (int f = 0; f < e && !clash; f++) {
-                if (ss[f] < 0) continue;
-                for (int a = 0; a < 3 && !clash; a++)
-                    for (int b = 0; b < 3; b++)
-                        if (g3[3 * e + a] == g3[3 * f + b]) { clash = true; break; }
+            for (int e = tid; e < nc; e += nt) {
+                const int g = uns[e];
+                int r = 0;
+                for (int f = 0; f < nc; f++) r += uns[f] < g ? 1 : 0;
+                cand[r] = g;
             }
-            uns[e] = clash ? 0 : 1;
-        }
-        __syncthreads();
-        // stable compaction of the kept candidates, one workgroup scan per
-        // nt candidates (values are read before the scan, written after)
-        int off = 0;
-        for (int b0 = 0; b0 < nc; b0 += nt) {
-            const int e = b0 + tid;
-            const int flag = e < nc ? uns[e] : 0;
-            const int g = flag ? srt[e] : 0, A = flag ? ss[e] : 0;
-            const double la = (flag && al) ? al[e] : 0.0;
-            int tot;
-            const int pre = wg_exclusive_scan(flag, sh, &tot);
-            if (flag) {
-                srt[off + pre] = g;
-                ss[off + pre] = A;
-                if (al) al[off + pre] = la;
-            }
-            off += tot;
             __syncthreads();
-        }
-        if (tid == 0) nkeep = off;
-        __syncthreads();
-        for (int r = tid; r < nkeep; r += nt) {
-            const int g = srt[r], A = ss[r];
-            double u2, u3;
-            sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
-            double minAL;
-            if (al) {
-                minAL = al[r];
-            } else {
-                minAL = d.blen[d.vb[3 * A]];
+            for (int e = tid; e < nc; e += nt) {
+                const int g = cand[e];
+                double u0, u1;
+                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 0u, &u0, &u1);
+                const int len = d.glen[g];
+                int k = (int)(u1 * (double)len);
+                if (k >= len) k = len - 1;
+                const int A = glist(d, g)[k];
+                double minAL = d.blen[d.vb[3 * A]];
                 for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+                site[e] = (minAL > 0) ? A : -1;    // a site on a zero-length boundary is not a candidate
+                for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
             }
-            d.nucl_r[A] = P.nucl_scaling * minAL * u2;
-            d.nucl_ori[A] = 2 * SEVC_PI * u3;
-            kept[r] = A;
+            __syncthreads();
+            for (int e = tid; e < nc; e += nt) {
+                bool clash = site[e] < 0;
+                for (int f = 0; f < e && !clash; f++) {
+                    if (site[f] < 0) continue;
+                    for (int a = 0; a < 3 && !clash; a++)
+                        for (int b = 0; b < 3; b++)
+                            if (sgr[3 * e + a] == sgr[3 * f + b]) { clash = true; break; }
+                }
+                uns[e] = clash ? 0 : 1;
+            }
+            __syncthreads();
+            int off = 0;
+            for (int b0 = 0; b0 < nc; b0 += nt) {
+                const int e = b0 + tid;
+                const int flag = e < nc ? uns[e] : 0;
+                const int g = flag ? cand[e] : 0, A = flag ? site[e] : 0;
+                int tot;
+                const int pre = wg_exclusive_scan(flag, sh, &tot);
+                if (flag) {
+                    cand[off + pre] = g;
+                    site[off + pre] = A;
+                }
+                off += tot;
+                __syncthreads();
+            }
+            if (tid == 0) nkeep = off;
+            __syncthreads();
+            for (int r = tid; r < nkeep; r += nt) {
+                const int g = cand[r], A = site[r];
+                double u2, u3;
+                sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
+                double minAL = d.blen[d.vb[3 * A]];
+                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+                d.nucl_r[A] = P.nucl_scaling * minAL * u2;
+                d.nucl_ori[A] = 2 * SEVC_PI * u3;
+                kept[r] = A;
+            }
         }
     }
     __syncthreads();
     const int K = nkeep;
     if (K == 0) return;
     // free slots in rank order, taken before any insertion of this step
-    int a = collect_slots(d.gblk, d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], d.bc_g, d.genab, 0,
-                          K, fg, sh, d.slist + d.gblk_first[m]);
-    int b = collect_slots(d.eblk, d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], d.bc_e, d.benab, 0,
-                          3 * K, fe, sh, d.slist + d.ngb + d.eblk_first[m]);
-    int c = collect_slots(d.vblk, d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], d.bc_v, d.ven, 0,
-                          2 * K, fv, sh, d.slist + d.ngb + d.neb + d.vblk_first[m]);
-    if (tid == 0) { fg_n = a; fe_n = b; fv_n = c; }
+    const SlotKind kinds[3] = {
+        {d.gblk, d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], d.bc_g, d.genab, K, fg,
+         d.slist + d.gblk_first[m]},
+        {d.eblk, d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], d.bc_e, d.benab, 3 * K, fe,
+         d.slist + d.ngb + d.eblk_first[m]},
+        {d.vblk, d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], d.bc_v, d.ven, 2 * K, fv,
+         d.slist + d.ngb + d.neb + d.vblk_first[m]}};
+    int got[3];
+    collect_free3(kinds, got, sh);
+    // event r needs grain slot r, boundary slots 3r..3r+2, vertex slots 2r, 2r+1
+    const int n_ev = min(K, min(got[0], min(got[1] / 3, got[2] / 2)));
+    if (tid == 0) ev0 = n_ev > 0 ? atomicAdd(&d.cnt[C_NEV], n_ev) : 0;
     __syncthreads();
-    for (int r = tid; r < K; r += nt) {
-        if (r + 1 > fg_n || 3 * r + 3 > fe_n || 2 * r + 2 > fv_n) continue;
-        if (nucleate_at(d, P, kept[r], fv[2 * r], fv[2 * r + 1], fe[3 * r], fe[3 * r + 1], fe[3 * r + 2], fg[r]) == 0)
-            atomicAdd(&nmade, 1);
+    for (int r = tid; r < n_ev; r += nt) {
+        int* ev = d.l_nev + 8 * (size_t)(ev0 + r);
+        ev[0] = m;
+        ev[1] = kept[r];
+        ev[2] = fv[2 * r];
+        ev[3] = fv[2 * r + 1];
+        ev[4] = fe[3 * r];
+        ev[5] = fe[3 * r + 1];
+        ev[6] = fe[3 * r + 2];
+        ev[7] = fg[r];
     }
-    __syncthreads();
-    if (tid == 0) M.nucl += nmade;
 }
 
-// Lengths, energies (calculus.h:83-114) and areas (:224-234) of the
-// entities a nucleation changed -- the recomputation of main.cu:171-177
-// restricted to the only values it can change.
-__global__ void k_nucl_refresh(Dev d) {
-    const int ne = min(d.cnt[C_NDE], 6 * d.G), ng = min(d.cnt[C_NDG], 4 * d.G);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ne + ng; k += gridDim.x * blockDim.x) {
-        if (k < ne) {
-            int b = d.l_nde[k];
-            if (!d.benab[b]) continue;
-            const Params& P = d.prm[member_of_edge(d, b)];
-            int a0 = d.bvx[2 * b], a1 = d.bvx[2 * b + 1];
-            d.blen[b] = arclen(d.pos[2 * a0], d.pos[2 * a0 + 1], d.pos[2 * a1], d.pos[2 * a1 + 1], P.D);
-            double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
-            d.ben[b] = e * P.gb_scaling;
-        } else {
-            int g = d.l_ndg[k - ne];
-            if (!d.genab[g]) continue;
-            d.garea[g] = grain_area(d, g, d.prm[member_of_grain(d, g)].D);
+// nucleate (nucleation2.cu:474-676, after its checks) for the events the
+// managers listed, one wave (workgroup of 64) per event, followed by the
+// recomputation of main.cu:171-177 restricted to what an event changes:
+// the lengths and energies (calculus.h:83-114) of its six boundaries and the
+// areas (:224-234) of its four grains.  Events are grain-disjoint: no event
+// reads a vertex, boundary or grain list another one writes, so they run
+// concurrently, and each event's refresh reads only its own writes.
+// The lists of the three grains at the site are staged in LDS with one
+// round of loads; lane 0 applies the edits of grain_add_vertex and
+// grain_remove_vertex (geometry.h:629-666) to the staged copies in the
+// reference's order (a grain named twice keeps one copy, so the edits
+// compose), and the wave writes them back.
+__global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
+    __shared__ int Ls[3][kMaxStride];
+    __shared__ int n_s[3];
+    __shared__ double sdx[4][kMaxStride], sdy[4][kMaxStride];
+    const int lane = threadIdx.x;
+    const int n_ev = min(d.cnt[C_NEV], d.G);
+    for (int item = blockIdx.x; item < n_ev; item += gridDim.x) {
+        const int* ev = d.l_nev + 8 * (size_t)item;
+        const int m = ev[0], A = ev[1], B = ev[2], C = ev[3], AB = ev[4], BC = ev[5], CA = ev[6], g = ev[7];
+        const Params& P = d.prm[m];
+        NuclGeom r;
+        // nucleation2.cu:526 passes (scaling, eps) swapped; only dE is affected
+        if (generate_grain(d, P, A, P.gb_scaling, P.gb_eps, 1, &r) != 0) continue;
+        const int gl[3] = {r.g1, r.g2, r.g3};
+        int sl[3];
+        sl[0] = 0;
+        sl[1] = gl[1] == gl[0] ? 0 : 1;
+        sl[2] = gl[2] == gl[0] ? 0 : (gl[2] == gl[1] ? sl[1] : 2);
+        int n_old[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) n_old[t] = sl[t] == t ? d.glen[gl[t]] : 0;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            if (sl[t] != t) continue;
+            const int* L = glist(d, gl[t]);
+            for (int i = lane; i < n_old[t]; i += 64) Ls[t][i] = L[i];
         }
+        if (lane == 0) { n_s[0] = n_old[0]; n_s[1] = n_old[1]; n_s[2] = n_old[2]; }
+        __syncthreads();
+        if (lane == 0) {
+            const int S = d.S;
+            auto ladd = [&](int t, int v, int prev) {               // grain_add_vertex
+                const int n = n_s[t];
+                int pos = -1;
+                for (int i = 0; i < n; i++) if (Ls[t][i] == prev) { pos = i + 1; break; }
+                if (pos < 0) { set_error(d, SEVC_E_STATE); return; }
+                if (n >= S) { set_error(d, SEVC_E_GRAIN_FULL); return; }
+                for (int q = n; q > pos; q--) Ls[t][q] = Ls[t][q - 1];
+                Ls[t][pos] = v;
+                n_s[t] = n + 1;
+            };
+            auto lremove = [&](int t, int v) {                      // grain_remove_vertex
+                const int n = n_s[t];
+                for (int i = 0; i < n; i++)
+                    if (Ls[t][i] == v) {
+                        for (int j = i; j < n - 1; j++) Ls[t][j] = Ls[t][j + 1];
+                        n_s[t] = n - 1;
+                        return;
+                    }
+            };
+            // init_vertex (geometry.h:346-361) for B and C
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int v = q == 0 ? B : C;
+                d.pos[2 * v] = q == 0 ? r.Bpos[0] : r.Cpos[0];
+                d.pos[2 * v + 1] = q == 0 ? r.Bpos[1] : r.Cpos[1];
+                d.vel[2 * v] = 0.0; d.vel[2 * v + 1] = 0.0;
+                d.venergy[v] = 0.0;
+                d.ven[v] = 1;
+                d.vnuc[v] = 0;
+                reset_mods(d, v);
+            }
+            d.pos[2 * A] = r.Apos[0]; d.pos[2 * A + 1] = r.Apos[1];
+            // init_boundary (geometry.h:491-504) for AB, BC, CA and the
+            // boundary slots they and AD, AE, AF take at A, B, C: each
+            // vertex's table was cleared, so add_boundary fills it in order
+            // (vb[A] = AB, CA, AD; vb[B] = AB, BC; vb[C] = BC, CA), after
+            // which the reference swaps the first two slots of B and C and
+            // puts AE, AF third
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int b = q == 0 ? AB : (q == 1 ? BC : CA);
+                d.bvx[2 * b] = q == 0 ? A : (q == 1 ? B : C);
+                d.bvx[2 * b + 1] = q == 0 ? B : (q == 1 ? C : A);
+                d.blen[b] = 0;
+                d.bt[b] = SEVC_MAX_T_EXT;
+                d.benab[b] = 1;
+                d.bcand[b] = 0;
+                d.conf[b] = -1;
+            }
+            d.vb[3 * A] = AB; d.vb[3 * A + 1] = CA; d.vb[3 * A + 2] = r.AD;
+            d.vb[3 * B] = BC; d.vb[3 * B + 1] = AB; d.vb[3 * B + 2] = r.AE;
+            d.vb[3 * C] = CA; d.vb[3 * C + 1] = BC; d.vb[3 * C + 2] = r.AF;
+            // nucleate_grain (geometry.h:596-606); its list is written below
+            d.glen[g] = 3;
+            d.garea[g] = 0;
+            d.gse[g] = 0;
+            d.genab[g] = 1;
+            d.gcand[g] = 0;
+            d.gfix[g] = 0;
+            d.gtype[g] = T_NUCLEATED;
+            d.gori[g] = d.nucl_ori[A];
+            for (int k = 0; k < 3; k++) if (d.vg[3 * A + k] == r.g3) { d.vg[3 * A + k] = g; break; }
+            d.vg[3 * B] = g; d.vg[3 * B + 1] = r.g2; d.vg[3 * B + 2] = r.g3;
+            d.vg[3 * C] = g; d.vg[3 * C + 1] = r.g3; d.vg[3 * C + 2] = r.g1;
+            ladd(sl[0], C, A);
+            const int e0 = d.bvx[2 * r.AE], e1 = d.bvx[2 * r.AE + 1];
+            if (e0 == A) ladd(sl[1], B, e1);
+            else if (e1 == A) ladd(sl[1], B, e0);
+            ladd(sl[2], C, A);
+            ladd(sl[2], B, C);
+            lremove(sl[2], A);
+            if (e0 == A) d.bvx[2 * r.AE] = B;
+            else if (e1 == A) d.bvx[2 * r.AE + 1] = B;
+            if (d.bvx[2 * r.AF] == A) d.bvx[2 * r.AF] = C;
+            else d.bvx[2 * r.AF + 1] = C;
+            atomicAdd((unsigned long long*)&d.ms[m].nucl, 1ull);
+        }
+        __syncthreads();
+        // write the edited lists back (-1 after the end, as the reference's
+        // removal leaves) and the new grain's list
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            if (sl[t] != t) continue;
+            const int n_new = n_s[t];
+            int* L = glist(d, gl[t]);
+            for (int i = lane; i < max(n_old[t], n_new); i += 64) L[i] = i < n_new ? Ls[t][i] : -1;
+            if (lane == 0) d.glen[gl[t]] = n_new;
+        }
+        {
+            int* Lg = glist(d, g);
+            for (int i = lane; i < d.S; i += 64) Lg[i] = i == 0 ? A : (i == 1 ? B : (i == 2 ? C : -1));
+        }
+        __syncthreads();
+        // refresh: the six boundaries (lanes 0-5), then the areas of g1, g2,
+        // g3 and g from the wrapped deltas of their lists, summed in list
+        // order by lanes 0-3 as grain_area does
+        const double Dm = P.D;
+        if (lane < 6) {
+            const int b = lane == 0 ? AB : lane == 1 ? BC : lane == 2 ? CA : lane == 3 ? r.AD : lane == 4 ? r.AE : r.AF;
+            if (d.benab[b]) {
+                const int a0 = d.bvx[2 * b], a1 = d.bvx[2 * b + 1];
+                d.blen[b] = arclen(d.pos[2 * a0], d.pos[2 * a0 + 1], d.pos[2 * a1], d.pos[2 * a1 + 1], Dm);
+                const double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
+                d.ben[b] = e * P.gb_scaling;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int n = n_s[sl[t]];
+            for (int i = 1 + lane; i < n; i += 64) {
+                const int w = Ls[sl[t]][i], wp = Ls[sl[t]][i - 1];
+                sdx[t][i] = wrapd(d.pos[2 * w], d.pos[2 * wp], Dm);
+                sdy[t][i] = wrapd(d.pos[2 * w + 1], d.pos[2 * wp + 1], Dm);
+            }
+        }
+        if (lane < 2) {                                  // g: A, B, C
+            const int w = lane == 0 ? B : C, wp = lane == 0 ? A : B;
+            sdx[3][1 + lane] = wrapd(d.pos[2 * w], d.pos[2 * wp], Dm);
+            sdy[3][1 + lane] = wrapd(d.pos[2 * w + 1], d.pos[2 * wp + 1], Dm);
+        }
+        __syncthreads();
+        if (lane < 4) {
+            const int t = lane;
+            const int gg = t == 0 ? gl[0] : (t == 1 ? gl[1] : (t == 2 ? gl[2] : g));
+            const int st = t == 0 ? sl[0] : (t == 1 ? sl[1] : sl[2]);
+            const int n = t < 3 ? n_s[st] : 3;
+            if (d.genab[gg]) {
+                double a = 0.0;
+                if (n > 0) {
+                    AreaAcc acc;
+                    acc.start(0.0, 0.0);
+                    for (int i = 1; i < n; i++) acc.push_delta(sdx[t][i], sdy[t][i]);
+                    a = acc.finish();
+                }
+                d.garea[gg] = a;
+            }
+        }
+        __syncthreads();
     }
 }
 

@@ -41,7 +41,7 @@ struct Blk { int m, b, e; };
 
 enum Counter {
     C_CAND = 0, C_TWO, C_T2, C_T2C, C_FLIPC,
-    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NDE, C_NDG, C_VFALL, C_ERR, C_COUNT
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_ERR, C_COUNT
 };
 
 struct Dev {
@@ -69,7 +69,9 @@ struct Dev {
     // work lists and scratch
     int* cnt;
     int* l_cand; int* l_two; int* l_t2; int* l_t2c; int* l_flipc;
-    int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_nde; int* l_ndg; int* l_vfall;
+    int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_vfall;
+    int* nrec; double* nal;               // rate-mode candidate sites: (site or -1, its 3 grains), shortest edge
+    int* l_nev;                           // nucleation events of the step: (member, A, B, C, AB, BC, CA, g)
     int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
     double* partial;                      // per grain block partial sums (2 levels)
     unsigned long long* bmin;             // per edge block min t_ext bits
