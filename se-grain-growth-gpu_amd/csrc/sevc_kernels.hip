@@ -1749,15 +1749,14 @@ __device__ RunScan run_scan(const Blk* tbl, int first, int nblk, const int2* bc,
     RunScan r;
     r.lo = first + min(nblk, tid * C);
     r.hi = first + min(nblk, (tid + 1) * C);
-    int c = 0, b = r.lo;
-    for (; b + 8 <= r.hi; b += 8) {
+    int c = 0;
+    for (int b = r.lo; b < r.hi; b += 8) {            // 8 independent loads per round
         int t[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = block_matches(bc, b + q, want);
+        for (int q = 0; q < 8; q++) t[q] = b + q < r.hi ? block_matches(bc, b + q, want) : 0;
 #pragma unroll
         for (int q = 0; q < 8; q++) c += t[q];
     }
-    for (; b < r.hi; b++) c += block_matches(bc, b, want);
     r.cnt = c;
     r.pre = wg_exclusive_scan(c, sh, &r.total);
     return r;
@@ -1802,29 +1801,32 @@ __device__ __forceinline__ void wg_exclusive_scan3(const int x[3], int* sh, int 
     __syncthreads();
 }
 
-// One kind of slot of a member to collect: its block table range, per-block
-// (enabled, free) counts, enable flags, how many free slots are wanted,
-// where they go, and a (block, rank) list of at least nblk entries.
+// One kind of slot of a member to collect: its blocks [first, first+nblk)
+// of the block table, which cover slots [off, off+n) kBlock at a time, the
+// per-block (enabled, free) counts, the enable flags, how many free slots
+// are wanted and where they go.
 struct SlotKind {
-    const Blk* tbl;
-    int first, nblk;
+    int first, nblk, off, n;
     const int2* bc;
     const uint8_t* en;
     int R;
     int* out;
-    int2* blist;
 };
 
 // The first R free slots (enable flag 0) of each of three kinds, in index
 // order, cooperatively in one workgroup: each lane takes one run of
 // consecutive blocks of every kind; a scan of the runs' free counts tells
 // each lane which ranks its run holds; lanes whose run holds wanted ranks
-// list (block, first rank) for its blocks with free slots; the waves then
-// sweep the listed blocks of all three kinds 64 slots at a time and write
-// each free slot at its rank.  got[t] = min(R_t, free slots of kind t).
-// `sh` holds >= 3 * 65 ints.
-__device__ __forceinline__ void collect_free3(const SlotKind* K, int* got, int* sh) {
-    __shared__ int nl[3];
+// list (kind, block, first rank) for its blocks with free slots in `bl`
+// (LDS when the caller's bound on the entries fits there, else global
+// scratch); the waves then sweep the listed blocks, kSweep at a time, and
+// write each free slot at its rank.  got[t] = min(R_t, free slots of kind
+// t).  `sh` holds >= 3 * 65 ints; `bl` holds at least R_0 + R_1 + R_2
+// entries (a listed block holds a wanted rank).
+__device__ __forceinline__ void collect_free3(const SlotKind K0, const SlotKind K1, const SlotKind K2, int* got,
+                                              int* sh, int4* bl) {
+    const SlotKind K[3] = {K0, K1, K2};    // by value: kept in registers
+    __shared__ int nl;
     const int tid = threadIdx.x, nt = blockDim.x;
     int lo[3], hi[3], c[3];
 #pragma unroll
@@ -1832,18 +1834,17 @@ __device__ __forceinline__ void collect_free3(const SlotKind* K, int* got, int* 
         const int C = (K[t].nblk + nt - 1) / nt;
         lo[t] = K[t].first + min(K[t].nblk, tid * C);
         hi[t] = K[t].first + min(K[t].nblk, (tid + 1) * C);
-        int n = 0, b = lo[t];
-        for (; b + 8 <= hi[t]; b += 8) {
+        int n = 0;
+        for (int b = lo[t]; b < hi[t]; b += 8) {     // 8 independent loads per round
             int v[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = K[t].bc[b + q].y;
+            for (int q = 0; q < 8; q++) v[q] = b + q < hi[t] ? K[t].bc[b + q].y : 0;
 #pragma unroll
             for (int q = 0; q < 8; q++) n += v[q];
         }
-        for (; b < hi[t]; b++) n += K[t].bc[b].y;
         c[t] = n;
     }
-    if (tid < 3) nl[tid] = 0;
+    if (tid == 0) nl = 0;
     int pre[3], tot[3];
     wg_exclusive_scan3(c, sh, pre, tot);
 #pragma unroll
@@ -1857,42 +1858,58 @@ __device__ __forceinline__ void collect_free3(const SlotKind* K, int* got, int* 
 #pragma unroll
             for (int q = 0; q < 8; q++)
                 if (v[q] > 0 && rank < K[t].R) {
-                    K[t].blist[atomicAdd(&nl[t], 1)] = make_int2(b0 + q, rank);
+                    bl[atomicAdd(&nl, 1)] = make_int4(t, b0 + q, rank, 0);
                     rank += v[q];
                 }
         }
     }
     __syncthreads();
-    const int n0 = nl[0], n1 = nl[1], n = n0 + n1 + nl[2];
+    const int n = nl;
     const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int q = wave; q < n; q += nw) {
-        // one wave per listed block: its flags are loaded at once (kBlock/64
-        // per lane), then ranked 64 slots at a time
-        const int t = q < n0 ? 0 : (q < n0 + n1 ? 1 : 2);
-        // the kind's fields by selection (no indexing of K by a run-time t)
-        const Blk* tbl = t == 0 ? K[0].tbl : (t == 1 ? K[1].tbl : K[2].tbl);
-        const uint8_t* en = t == 0 ? K[0].en : (t == 1 ? K[1].en : K[2].en);
-        int* out = t == 0 ? K[0].out : (t == 1 ? K[1].out : K[2].out);
-        const int2* bl = t == 0 ? K[0].blist : (t == 1 ? K[1].blist - n0 : K[2].blist - (n0 + n1));
-        const int R = t == 0 ? K[0].R : (t == 1 ? K[1].R : K[2].R);
-        const int2 e = bl[q];
-        const int blo = tbl[e.x].b, bhi = tbl[e.x].e;
-        bool hit[kBlock / 64];
+    constexpr int kSweep = 8;
+    for (int q0 = wave; q0 < n; q0 += nw * kSweep) {
+        int4 e[kSweep];
 #pragma unroll
-        for (int k = 0; k < kBlock / 64; k++) {
-            const int idx = blo + 64 * k + lane;
-            hit[k] = idx < bhi && en[idx] == 0;
+        for (int j = 0; j < kSweep; j++) {
+            const int q = q0 + nw * j;
+            e[j] = q < n ? bl[q] : make_int4(0, 0, 0, -1);
         }
-        int rank = e.y;
+        bool hit[kSweep][kBlock / 64];
+        int blo[kSweep];
 #pragma unroll
-        for (int k = 0; k < kBlock / 64; k++) {
-            const unsigned long long hb = __ballot(hit[k]);
-            if (hit[k]) {
-                const int r = rank + (int)__popcll(hb & below);
-                if (r < R) out[r] = blo + 64 * k + lane;
+        for (int j = 0; j < kSweep; j++) {
+            // the kind's fields by selection (no indexing of K by a run-time t)
+            const int t = e[j].x;
+            const int first = t == 0 ? K[0].first : (t == 1 ? K[1].first : K[2].first);
+            const int off = t == 0 ? K[0].off : (t == 1 ? K[1].off : K[2].off);
+            const int nn = t == 0 ? K[0].n : (t == 1 ? K[1].n : K[2].n);
+            const uint8_t* en = t == 0 ? K[0].en : (t == 1 ? K[1].en : K[2].en);
+            // padding entries (w < 0) get the empty range [0, 0), so no lane loads
+            const bool listed = e[j].w >= 0;
+            blo[j] = listed ? off + kBlock * (e[j].y - first) : 0;
+            const int bhi = listed ? min(blo[j] + kBlock, off + nn) : 0;
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; k++) {
+                const int idx = blo[j] + 64 * k + lane;
+                hit[j][k] = idx < bhi && en[idx] == 0;
             }
-            rank += (int)__popcll(hb);
+        }
+#pragma unroll
+        for (int j = 0; j < kSweep; j++) {
+            const int t = e[j].x;
+            int* out = t == 0 ? K[0].out : (t == 1 ? K[1].out : K[2].out);
+            const int R = t == 0 ? K[0].R : (t == 1 ? K[1].R : K[2].R);
+            int rank = e[j].z;
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; k++) {
+                const unsigned long long hb = __ballot(hit[j][k]);
+                if (hit[j][k]) {
+                    const int r = rank + (int)__popcll(hb & below);
+                    if (r < R) out[r] = blo[j] + 64 * k + lane;
+                }
+                rank += (int)__popcll(hb);
+            }
         }
     }
     __syncthreads();
@@ -1959,6 +1976,7 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
 constexpr int kNuclLds = 1024;       // rate-mode candidates held in LDS
 constexpr int kNuclHash = 4096;      // grain table of the clash test (> 3 kNuclLds keys: never full)
 constexpr int kHashEmpty = INT_MIN;
+constexpr int kSlotLds = 1024;       // listed blocks of the free-slot search held in LDS
 
 __device__ __forceinline__ int nucl_hash(int g) { return (int)(((unsigned)g * 2654435761u) >> 20); }
 
@@ -1968,6 +1986,7 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     __shared__ int c_s[kNuclLds], site_s[kNuclLds], sg_s[3 * kNuclLds], k_s[kNuclLds], q_s[kNuclLds];
     __shared__ double al_s[kNuclLds];         // shortest edge at each candidate's site
     __shared__ int hk[kNuclHash], hv[kNuclHash];
+    __shared__ int4 bl_s[kSlotLds];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
@@ -2161,14 +2180,12 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     if (K == 0) return;
     // free slots in rank order, taken before any insertion of this step
     const SlotKind kinds[3] = {
-        {d.gblk, d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], d.bc_g, d.genab, K, fg,
-         d.slist + d.gblk_first[m]},
-        {d.eblk, d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], d.bc_e, d.benab, 3 * K, fe,
-         d.slist + d.ngb + d.eblk_first[m]},
-        {d.vblk, d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], d.bc_v, d.ven, 2 * K, fv,
-         d.slist + d.ngb + d.neb + d.vblk_first[m]}};
+        {d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], P.goff, P.ng, d.bc_g, d.genab, K, fg},
+        {d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], P.eoff, P.ne, d.bc_e, d.benab, 3 * K, fe},
+        {d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], P.voff, P.nv, d.bc_v, d.ven, 2 * K, fv}};
     int got[3];
-    collect_free3(kinds, got, sh);
+    if (6 * K <= kSlotLds) collect_free3(kinds[0], kinds[1], kinds[2], got, sh, bl_s);
+    else collect_free3(kinds[0], kinds[1], kinds[2], got, sh, d.slist + d.gblk_first[m] + d.eblk_first[m] + d.vblk_first[m]);
     // event r needs grain slot r, boundary slots 3r..3r+2, vertex slots 2r, 2r+1
     const int n_ev = min(K, min(got[0], min(got[1] / 3, got[2] / 2)));
     if (tid == 0) ev0 = n_ev > 0 ? atomicAdd(&d.cnt[C_NEV], n_ev) : 0;

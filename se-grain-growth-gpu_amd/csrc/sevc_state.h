@@ -64,7 +64,7 @@ struct Dev {
     int nvb, neb, ngb;
     const Blk* gtile; int ntile;         // 64-grain tiles for the velocity pass 1
     double* vterm;                       // 6 per vertex: stored-energy terms (x,y) per grain slot
-    int2* slist;                         // free-slot search: (block, rank) lists, ngb+neb+nvb entries
+    int4* slist;                         // free-slot search: (kind, block, rank) lists, ngb+neb+nvb entries
     int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
     // work lists and scratch
     int* cnt;

@@ -353,3 +353,18 @@ def test_long_candidate_lists(n, factor):
             nc = np.sum((((bt >= 0) & (bt <= o.dt)) | ((bt >= 0) & (bl > 0) & (bl < o.flip_threshold))) & (en == 1))
             assert nc > 2048, nc
     st.close()
+
+
+@pytest.mark.parametrize("rate", [0.05, 0.4])
+def test_nucleation_bursts(rate):
+    """Hundreds to thousands of rate-mode nucleations per step on 20,000
+    grains: about 270 kept events per step at rate 0.05 (more free slots
+    wanted than the in-LDS slot list holds), about 2,400 candidates per step
+    at rate 0.4 (beyond the in-LDS candidate table), where the free grain
+    slots run out in the second step and the remaining events are skipped.
+    State and event counts match the oracle bit for bit."""
+    o, mesh = _generated(20000, 31, dt=1e-6, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                         nucleation_mode=sevc.NUCL_RATE, nucleation_rate=rate, nucleation_scaling=0.9, seed=5,
+                         extra_capacity=0.15)
+    tot = _run_pair(o, mesh, [1, 1, 2])
+    assert tot["nucl"] >= (1000 if rate < 0.1 else 3000), tot
