@@ -84,6 +84,7 @@ struct sevc_state {
     hipStream_t stream = nullptr;
     std::string err;
     bool any_nucl = false, any_events = false, any_adaptive = false, any_analytic = false;
+    bool any_mc = false, any_rate = false;   // nucleation modes present
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
     // graph replay
@@ -219,8 +220,8 @@ void launch_step(sevc_state* s, bool diag) {
     if (diag || s->any_analytic) launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
     if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     if (s->any_nucl) {
-        launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
-        launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
+        if (s->any_mc) launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
+        if (s->any_rate) launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
         launch(s, "nucl_manager", k_nucl_manager, s->B, kNuclThreads, d);
         launch(s, "nucl_insert", k_nucl_insert, kNuclInsGrid, 64, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
@@ -337,7 +338,11 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         s->nv.push_back(me.n_vertices); s->ne.push_back(me.n_boundaries); s->ng.push_back(me.n_grains);
         s->V += me.n_vertices; s->E += me.n_boundaries; s->G += me.n_grains;
         if (opts[m].dt <= 0) { delete s; return SEVC_E_INVALID_ARG; }
-        if (opts[m].events && opts[m].do_nucleation) s->any_nucl = true;
+        if (opts[m].events && opts[m].do_nucleation) {
+            s->any_nucl = true;
+            if (opts[m].nucleation_mode == SEVC_NUCL_MC) s->any_mc = true;
+            else s->any_rate = true;
+        }
         if (opts[m].events) s->any_events = true;
         if (opts[m].adaptive_dt) s->any_adaptive = true;
         if (opts[m].velocity_model == SEVC_VEL_ANALYTIC) s->any_analytic = true;

@@ -258,15 +258,17 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
 // area the reference's left-to-right shoelace sum, so the results are
 // bit-identical.  Pass 2 sends vertices with dom_mod(x) != x or
 // dom_mod(y) != y to the two-pass path, which this form does not cover.
-constexpr int kTilePts = 512;
+constexpr int kTilePts = 480;   // with the per-slot and per-grain arrays: 8 workgroups of 256 per CU
 static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
 __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     __shared__ double2 Ps[kTilePts];
     __shared__ double2 Ds[kTilePts];        // wrap(P_k - P_{k-1}) of list slot k > 0
     __shared__ int Ls[kTilePts];
     __shared__ unsigned char pgl[kTilePts];
+    __shared__ unsigned char vmask[kTilePts]; // bit j: vg[j] of the slot's vertex is the slot's grain
     __shared__ int off[kTile + 1];
-    __shared__ double area_s[kTile];
+    __shared__ double area_s[kTile], se_s[kTile];
+    __shared__ unsigned char en_s[kTile];
     __shared__ int gend_s;
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (lb >= d.ntile) return;
@@ -283,7 +285,11 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             // wave 0: inclusive scan of list lengths; the tile takes the
             // longest prefix of grains whose lists fit in kTilePts points
             const int g = gs + tid;
-            const int n = (g < bd.e && d.genab[g]) ? d.glen[g] : 0;
+            const bool en = g < bd.e && d.genab[g];
+            const int n = en ? d.glen[g] : 0;
+            // the values the later phases need from global memory, loaded here
+            en_s[tid] = en ? 1 : 0;
+            se_s[tid] = g < bd.e ? d.gse[g] : 0.0;
             int inc = n;
             for (int o = 1; o < kTile; o <<= 1) {
                 const int t = __shfl_up(inc, o, kTile);
@@ -308,6 +314,9 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             const int w = glist(d, gs + g)[p - off[g]];
             Ls[p] = w;
             Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
+            unsigned char mk = 0;
+            for (int j = 0; j < 3; j++) mk |= (d.vg[3 * w + j] == gs + g) ? (unsigned char)(1 << j) : 0;
+            vmask[p] = mk;
         }
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
@@ -335,7 +344,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             a *= 0.5;
             if (n <= 0) a = 0.0;
             const int G = gs + g;
-            if (d.genab[G] && !tuning) d.garea[G] = a;
+            if (en_s[g] && !tuning) d.garea[G] = a;
             area_s[g] = a;
         }
         __syncthreads();
@@ -369,10 +378,11 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             sx += xx * z - uy * z;
             sy += ux * z - yy * z;
             const double ax = sx * 0.5, ay = sy * 0.5;
-            const double a0 = fabs(area_s[g]), se = d.gse[gg];
+            const double a0 = fabs(area_s[g]), se = se_s[g];
             const double tx = se * (fabs(ax) - a0), ty = se * (fabs(ay) - a0);
+            const unsigned mk = vmask[p];
             for (int j = 0; j < 3; j++)
-                if (d.vg[3 * v + j] == gg) {
+                if ((mk >> j) & 1u) {
                     d.vterm[6 * (size_t)v + 2 * j] = tx;
                     d.vterm[6 * (size_t)v + 2 * j + 1] = ty;
                 }
