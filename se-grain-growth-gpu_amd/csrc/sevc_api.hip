@@ -85,6 +85,8 @@ struct sevc_state {
     std::string err;
     bool any_nucl = false, any_events = false, any_adaptive = false, any_analytic = false;
     bool any_mc = false, any_rate = false;   // nucleation modes present
+    bool any_aniso = false;                  // a member with GB_eps != 0 (energies change with the topology)
+    bool ben_dirty = true;                   // boundary energies not computed since creation / a new scale
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
     // graph replay
@@ -216,7 +218,12 @@ void launch_step(sevc_state* s, bool diag) {
     d.diag = diag ? 1 : 0;
     const int nb = (s->B + 63) / 64;
     launch(s, "begin", k_begin, nb, 64, d);
-    launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
+    // Lengths come from k_text.  Energies change from step to step only with
+    // GB_eps != 0; with GB_eps = 0 they are the scale, computed at the first
+    // step after creation or after a new scale (for the members active then,
+    // as the reference's per-step pass would).
+    if (diag || s->any_aniso || s->ben_dirty) launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
+    s->ben_dirty = false;
     if (diag || s->any_analytic) launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
     if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     if (s->any_nucl) {
@@ -346,6 +353,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         if (opts[m].events) s->any_events = true;
         if (opts[m].adaptive_dt) s->any_adaptive = true;
         if (opts[m].velocity_model == SEVC_VEL_ANALYTIC) s->any_analytic = true;
+        if (opts[m].GB_eps != 0.0) s->any_aniso = true;
     }
     if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
     const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
@@ -606,16 +614,22 @@ int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     if (!s->gexec) {
         bool timing = s->timing;
         s->timing = false;
+        // the graph holds the steady step; a step that must recompute the
+        // energies runs outside it
+        const bool dirty = s->ben_dirty;
+        s->ben_dirty = false;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         launch_step(s, false);
         HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
+        s->ben_dirty = dirty;
         HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
         s->timing = timing;
     }
     for (int k = 0; k < n_steps; ++k) {
-        if (s->diag_next) {
+        if (s->diag_next || s->ben_dirty) {
+            const bool diag = s->diag_next;
             s->diag_next = false;
-            launch_step(s, true);
+            launch_step(s, diag);
         } else {
             HIPCHK(hipGraphLaunch(s->gexec, s->stream));
         }
@@ -736,6 +750,7 @@ int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const do
     if (gb_scaling > 0) { P.gb_scaling = gb_scaling; s->opts[m].GB_scaling = gb_scaling; }
     if (mobility > 0) { P.mobility = mobility; s->opts[m].mobility = mobility; }
     HIPCHK(hipMemcpy(const_cast<Params*>(d.prm) + m, &P, sizeof(Params), hipMemcpyHostToDevice));
+    if (gb_scaling > 0) s->ben_dirty = true;      // energies at the new scale on the next step
     if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
     s->win.ready = false;

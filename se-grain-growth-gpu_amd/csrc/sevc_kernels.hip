@@ -162,6 +162,14 @@ __global__ void k_edge_len_energy(Dev d, int only_nucl) {
     d.ben[i] = e * P.gb_scaling;
 }
 
+// The stored length of edge b seen from its end v: |wrap(p_v - p_nb)|,
+// bit-identical to compute_arclength from either end (wrap_dist is odd and
+// vector2_mag sign-symmetric).
+__device__ __forceinline__ double edge_len_from(const Dev& d, int b, int v, double D) {
+    const int nb = other_end(d, b, v);
+    return arclen(d.pos[2 * v], d.pos[2 * v + 1], d.pos[2 * nb], d.pos[2 * nb + 1], D);
+}
+
 // compute_grain_areas, calculus.h:224-234
 __global__ void k_grain_area(Dev d, int only_nucl) {
     TABLE_BLOCK(d.gblk, d.ngb)
@@ -193,11 +201,12 @@ __device__ __forceinline__ double energy_diff(const Dev& d, const Params& P, int
     else ny += SEVC_GRAD_EPS;
     nx = dom_mod(nx, D);
     ny = dom_mod(ny, D);
-    double arcl[3];
+    double arcl[3], arcl0[3];
     for (int j = 0; j < 3; j++) {
         int b = d.vb[3 * v + j];
         int nb = other_end(d, b, v);
         arcl[j] = arclen(nx, ny, d.pos[2 * nb], d.pos[2 * nb + 1], D);
+        arcl0[j] = arclen(d.pos[2 * v], d.pos[2 * v + 1], d.pos[2 * nb], d.pos[2 * nb + 1], D);   // stored length
     }
     double areas[3];
     for (int j = 0; j < 3; j++) {
@@ -224,7 +233,7 @@ __device__ __forceinline__ double energy_diff(const Dev& d, const Params& P, int
     double bnd_term = 0.0, grn_term = 0.0;
     for (int j = 0; j < 3; j++) {
         int b = d.vb[3 * v + j], g = d.vg[3 * v + j];
-        bnd_term += (d.ben[b] * (arcl[j] - d.blen[b]));
+        bnd_term += (d.ben[b] * (arcl[j] - arcl0[j]));
         grn_term += (d.gse[g] * (fabs(areas[j]) - fabs(d.garea[g])));
     }
     *bnd_out = bnd_term;
@@ -537,6 +546,8 @@ __global__ void k_text(Dev d) {
         double y1 = 0 + wrapd(d.pos[2 * b + 1], d.pos[2 * a + 1], D);
         double dx = x1 - 0, dy = y1 - 0;
         double norm = mag(dx, dy);
+        // compute_boundary_arclengths (calculus.h:83-93): the same |wrap(end - ini)|
+        d.blen[i] = norm;
         double tx, ty;
         if (norm == 0) { tx = 0; ty = 0; }
         else {
@@ -549,7 +560,7 @@ __global__ void k_text(Dev d) {
         double aux = tx * px + ty * py;
         double te = (aux == 0) ? 100.0 : -norm / aux;
         d.bt[i] = te;
-        if (P.adaptive && te > 0 && !(d.blen[i] < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
+        if (P.adaptive && te > 0 && !(norm < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
     }
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(v, off);
@@ -1659,7 +1670,7 @@ __global__ void k_nucl_factor(Dev d) {
     d.dE[v] = 0;
     d.nfac[v] = 0;
     double ALs[3];
-    for (int q = 0; q < 3; q++) ALs[q] = d.blen[d.vb[3 * v + q]];
+    for (int q = 0; q < 3; q++) ALs[q] = edge_len_from(d, d.vb[3 * v + q], v, P.D);
     double minAL = ALs[0];
     for (int q = 0; q < 3; q++) if (minAL > ALs[q]) minAL = ALs[q];
     double u0, u1;
@@ -1697,7 +1708,7 @@ __global__ void k_rate_cand(Dev d) {
     for (int q = 0; q < 3; q++) { vb[q] = d.vb[3 * A + q]; vg[q] = d.vg[3 * A + q]; }
     double al[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) al[q] = d.blen[vb[q]];
+    for (int q = 0; q < 3; q++) al[q] = edge_len_from(d, vb[q], A, P.D);
     double minAL = al[0];
     for (int q = 0; q < 3; q++) if (minAL > al[q]) minAL = al[q];
     int s = atomicAdd(&d.cnt[C_NUCL], 1);
@@ -2140,8 +2151,10 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
                 int k = (int)(u1 * (double)len);
                 if (k >= len) k = len - 1;
                 const int A = glist(d, g)[k];
-                double minAL = d.blen[d.vb[3 * A]];
-                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+                double al[3];
+                for (int q = 0; q < 3; q++) al[q] = edge_len_from(d, d.vb[3 * A + q], A, P.D);
+                double minAL = al[0];
+                for (int q = 0; q < 3; q++) if (minAL > al[q]) minAL = al[q];
                 site[e] = (minAL > 0) ? A : -1;    // a site on a zero-length boundary is not a candidate
                 for (int a = 0; a < 3; a++) sgr[3 * e + a] = d.vg[3 * A + a];
             }
@@ -2177,8 +2190,10 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
                 const int g = cand[r], A = site[r];
                 double u2, u3;
                 sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(g - P.goff), kStreamRate, 1u, &u2, &u3);
-                double minAL = d.blen[d.vb[3 * A]];
-                for (int q = 0; q < 3; q++) if (minAL > d.blen[d.vb[3 * A + q]]) minAL = d.blen[d.vb[3 * A + q]];
+                double al[3];
+                for (int q = 0; q < 3; q++) al[q] = edge_len_from(d, d.vb[3 * A + q], A, P.D);
+                double minAL = al[0];
+                for (int q = 0; q < 3; q++) if (minAL > al[q]) minAL = al[q];
                 d.nucl_r[A] = P.nucl_scaling * minAL * u2;
                 d.nucl_ori[A] = 2 * SEVC_PI * u3;
                 kept[r] = A;
