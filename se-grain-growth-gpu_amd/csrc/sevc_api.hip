@@ -34,9 +34,9 @@ __global__ void k_poll_gather(Dev d);
 __global__ void k_poll(Dev d);
 __global__ void k_fix(Dev d, int stage);
 __global__ void k_fix_clear(Dev d, int stage, int with_area);
-__global__ void k_two_scan(Dev d);
 __global__ void k_two_grain(Dev d);
 __global__ void k_label3(Dev d);
+__global__ void k_scan3(Dev d);
 __global__ void k_t2_par(Dev d);
 __global__ void k_t2_serial(Dev d);
 __global__ void k_st2(Dev d);
@@ -250,9 +250,9 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "label", k_label, d.neb, kBlock, d);
         launch(s, "poll_gather", k_poll_gather, kPollFast / 64, 64, d);
         launch(s, "poll", k_poll, 1, 1024, d);
-        launch(s, "two_scan", k_two_scan, d.ngb, kBlock, d);
+        launch(s, "scan3", k_scan3, d.ngb, kBlock, d);
         launch(s, "two_grain", k_two_grain, 1, 64, d);
-        launch(s, "label3", k_label3, d.ngb, kBlock, d);
+        launch(s, "label3", k_label3, d.ngb, kBlock, d);     // only after a two-triangle event
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
         launch(s, "fix", k_fix, kFixGrid, 64, d, 1);
@@ -371,7 +371,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.mfix, 4 * V); DA(d.mgrn, 4 * V); DA(d.mvrt, 4 * V); DA(d.mprev, 4 * V);
     DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
-    DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast);
+    DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E);
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
     DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
     DA(d.cnt, C_COUNT);

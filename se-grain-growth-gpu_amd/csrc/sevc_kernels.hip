@@ -97,40 +97,67 @@ __global__ void k_count_all(Dev d) {
 // One workgroup per member: sums / minimises the per-block partials.
 // what = 0: tau (adaptive dt); 1: grain and vertex counts.
 __global__ void __launch_bounds__(1024) k_member_reduce(Dev d, int what) {
-    __shared__ unsigned long long sm[1024];
-    __shared__ long long sa[1024], sb[1024];
+    __shared__ unsigned long long sm[16];
+    __shared__ long long sa[16], sb[16];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     MState& M = d.ms[m];
     if (!M.active) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    // each lane reads its share of the partials 16 at a time (independent
+    // loads), then the waves reduce with shuffles and wave 0 reduces the
+    // wave results: min and integer sums, so any order gives the same value
+    constexpr int kBatch = 16;
     if (what == 0) {
         unsigned long long v = 0x7FF0000000000000ull;
-#pragma unroll 4
-        for (int b = d.eblk_first[m] + tid; b < d.eblk_first[m + 1]; b += blockDim.x) v = d.bmin[b] < v ? d.bmin[b] : v;
-        sm[tid] = v;
-        __syncthreads();
-        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-            if (tid < st && sm[tid + st] < sm[tid]) sm[tid] = sm[tid + st];
-            __syncthreads();
+        const int lo = d.eblk_first[m], hi = d.eblk_first[m + 1];
+        for (int b0 = lo + tid; b0 < hi; b0 += kBatch * nt) {
+            unsigned long long t[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < hi ? d.bmin[b0 + q * nt] : 0x7FF0000000000000ull;
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) v = t[q] < v ? t[q] : v;
         }
-        if (tid == 0) M.tau_bits = sm[0];
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long u = __shfl_xor(v, o, 64);
+            v = u < v ? u : v;
+        }
+        if (lane == 0) sm[wave] = v;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long r = sm[0];
+            for (int w = 1; w < nw; w++) r = sm[w] < r ? sm[w] : r;
+            M.tau_bits = r;
+        }
     } else {
         long long a = 0, b2 = 0;
-#pragma unroll 4
-        for (int b = d.gblk_first[m] + tid; b < d.gblk_first[m + 1]; b += blockDim.x) a += d.bc_g[b].x;
-#pragma unroll 4
-        for (int b = d.vblk_first[m] + tid; b < d.vblk_first[m + 1]; b += blockDim.x) b2 += d.bc_v[b].x;
-        sa[tid] = a;
-        sb[tid] = b2;
-        __syncthreads();
-        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-            if (tid < st) { sa[tid] += sa[tid + st]; sb[tid] += sb[tid + st]; }
-            __syncthreads();
+        const int glo = d.gblk_first[m], ghi = d.gblk_first[m + 1];
+        const int vlo = d.vblk_first[m], vhi = d.vblk_first[m + 1];
+        for (int b0 = glo + tid; b0 < ghi; b0 += kBatch * nt) {
+            int t[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < ghi ? d.bc_g[b0 + q * nt].x : 0;
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) a += t[q];
         }
+        for (int b0 = vlo + tid; b0 < vhi; b0 += kBatch * nt) {
+            int t[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < vhi ? d.bc_v[b0 + q * nt].x : 0;
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) b2 += t[q];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b2 += __shfl_xor(b2, o, 64);
+        }
+        if (lane == 0) { sa[wave] = a; sb[wave] = b2; }
+        __syncthreads();
         if (tid == 0) {
-            M.curr_grains = (int)sa[0];
-            M.n_venabled = (int)sb[0];
+            long long ra = 0, rb = 0;
+            for (int w = 0; w < nw; w++) { ra += sa[w]; rb += sb[w]; }
+            M.curr_grains = (int)ra;
+            M.n_venabled = (int)rb;
             // main.cu:388-393: step counter, time and stop rules
             const Params& P = d.prm[m];
             M.dt = member_dt(P, M);
@@ -583,6 +610,8 @@ __global__ void k_label(Dev d) {
     const double dt = member_dt(P, M);
     bool c = is_candidate(d.bt[i], d.blen[i], dt, P.flip_thr);
     d.bcand[i] = c ? 1 : 0;
+    const uint8_t en = d.benab[i];
+    d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge
     if (c && d.benab[i]) {
         int k = atomicAdd(&d.cnt[C_CAND], 1);
         d.l_cand[k] = i;
@@ -1078,54 +1107,6 @@ __global__ void __launch_bounds__(64) k_fix_clear(Dev d, int stage, int with_are
 // two adjacent 3-sided grains (topological.h:22-73, 840-985)
 // ---------------------------------------------------------------------
 
-// Edges whose two grains are both 3-sided, found from the 3-sided grains.
-// For edge (a, c) of 3-sided grain i the two grains are vg(a) ∩ vg(c), the
-// set grains_intersect forms from the edge's ends; the edge is listed once,
-// by the lower of the two grains.  Loads are issued in rounds (list, then
-// the vertices' edges and grains, then the edges' ends and flags, then the
-// neighbours' list lengths).
-__global__ void k_two_scan(Dev d) {
-    TABLE_BLOCK(d.gblk, d.ngb)
-    if (!M.active || !P.events || i >= bd_.e) return;
-    if (!d.genab[i] || d.glen[i] != 3) return;
-    const int* L = glist(d, i);
-    const int vv[3] = {L[0], L[1], L[2]};
-    int bb[9], gv[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) {
-        bb[e] = d.vb[3 * vv[e / 3] + e % 3];
-        gv[e] = d.vg[3 * vv[e / 3] + e % 3];
-    }
-    int oe[9];
-    uint8_t en[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) {
-        const int b0 = d.bvx[2 * bb[e]], b1 = d.bvx[2 * bb[e] + 1];
-        oe[e] = (b0 == vv[e / 3]) ? b1 : b0;
-        en[e] = d.benab[bb[e]];
-    }
-    for (int k = 0; k < 3; k++) {
-        const int kc = (k + 1) % 3;
-        const int c = vv[kc];
-        for (int q = 0; q < 3; q++) {
-            const int e = 3 * k + q;
-            if (oe[e] != c || !en[e]) continue;
-            int gic[2] = {-1, -1}, cnt = 0;
-            for (int x = 0; x < 3 && cnt < 2; x++)
-                for (int y = 0; y < 3; y++)
-                    if (gv[3 * k + x] == gv[3 * kc + y]) { gic[cnt++] = gv[3 * k + x]; break; }
-            if (cnt < 2) continue;
-            if (d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3) {
-                int lo = gic[0] < gic[1] ? gic[0] : gic[1];
-                if (lo == i) {
-                    int s = atomicAdd(&d.cnt[C_TWO], 1);
-                    d.l_two[s] = bb[e];
-                }
-            }
-        }
-    }
-}
-
 __device__ bool two_qualifies(const Dev& d, int e) {
     if (!d.benab[e]) return false;
     int gic[2];
@@ -1202,6 +1183,9 @@ __global__ void k_two_grain(Dev d) {
             d.genab[gic[0]] = 0;
             d.genab[gic[1]] = 0;
             d.ms[member_of_edge(d, e)].two += 1;
+            // the T2 labels k_scan3 took before this stage are redone
+            d.cnt[C_RELABEL] = 1;
+            d.cnt[C_T2] = 0;
             // an edge whose endpoint moved may qualify later in this pass
             if (CE > e && two_qualifies(d, CE)) {
                 bool have = false;
@@ -1250,12 +1234,94 @@ __global__ void k_two_grain(Dev d) {
     }
 }
 
+// k_two_scan and k_label3 in one pass over the grains, sharing the loads of
+// a 3-sided grain's list, its vertices' edges and grains and those edges'
+// ends and flags: the two-triangle candidates (listed once, by the lower
+// of the two grains) and the T2 labels as they stand before the
+// two-triangle stage (redone by k_label3 if that stage changes anything).
+__global__ void k_scan3(Dev d) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
+    const int g = i;
+    uint8_t cand = 0;
+    if (d.glen[g] == 3) {
+        const int* L = glist(d, g);
+        const int v0 = L[0], v1 = L[1], v2 = L[2];
+        const int vv[3] = {v0, v1, v2};
+        int bb[9], gv[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            bb[e] = d.vb[3 * vv[e / 3] + e % 3];
+            gv[e] = d.vg[3 * vv[e / 3] + e % 3];
+        }
+        // the edges' ends, current labels and the (label, enabled) byte
+        // k_label left (its label is is_candidate at this step's dt, before
+        // polling): three loads per edge
+        int ini[9], end[9];
+        uint8_t bc[9], lab[9], be[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const int b = bb[e];
+            ini[e] = d.bvx[2 * b];
+            end[e] = d.bvx[2 * b + 1];
+            bc[e] = d.bcand[b];
+            lab[e] = d.blab[b];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; e++) be[e] = (lab[e] >> 1) & 1;
+        // label_3sided_grains (as k_label3)
+        auto in_g = [&](int x) { return x == v0 || x == v1 || x == v2; };
+        bool hit = false;
+        int ans = 0;
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const bool inn = in_g(ini[e]) && in_g(end[e]);
+            hit = hit || (inn && bc[e] && be[e]);
+            ans += (inn && (lab[e] & 1)) ? 1 : 0;
+        }
+        ans /= 2;
+        if (hit || ans == 3) cand = 1;
+        // two-triangle candidates (as k_two_scan): edge (a, c) of the grain
+        // with both of its grains 3-sided
+        for (int k = 0; k < 3; k++) {
+            const int kc = (k + 1) % 3;
+            const int c = vv[kc];
+            for (int q = 0; q < 3; q++) {
+                const int e = 3 * k + q;
+                const int oe = (ini[e] == vv[k]) ? end[e] : ini[e];
+                if (oe != c || !be[e]) continue;
+                int gic[2] = {-1, -1}, cnt = 0;
+                for (int x = 0; x < 3 && cnt < 2; x++)
+                    for (int y = 0; y < 3; y++)
+                        if (gv[3 * k + x] == gv[3 * kc + y]) { gic[cnt++] = gv[3 * k + x]; break; }
+                if (cnt < 2) continue;
+                if (d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3) {
+                    const int lo = gic[0] < gic[1] ? gic[0] : gic[1];
+                    if (lo == g) {
+                        const int s = atomicAdd(&d.cnt[C_TWO], 1);
+                        d.l_two[s] = bb[e];
+                    }
+                }
+            }
+        }
+    }
+    d.gcand[g] = cand;
+    if (cand) {
+        const int s = atomicAdd(&d.cnt[C_T2], 1);
+        d.l_t2[s] = g;
+    }
+}
+
 // ---------------------------------------------------------------------
 // T2: 3-sided grain removal (topological.h:614-668, 687-822, 994-1004)
 // ---------------------------------------------------------------------
 
 // clear_grains_state + label_3sided_grains; candidates are compacted.
+// k_scan3 takes these labels before the two-triangle stage; this kernel
+// redoes them when that stage removed a pair of grains (a step without such
+// an event leaves every datum the labels read unchanged).
 __global__ void k_label3(Dev d) {
+    if (d.cnt[C_RELABEL] == 0) return;
     TABLE_BLOCK(d.gblk, d.ngb)
     if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
     const int g = i;

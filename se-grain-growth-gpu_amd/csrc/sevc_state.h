@@ -41,7 +41,7 @@ struct Blk { int m, b, e; };
 
 enum Counter {
     C_CAND = 0, C_TWO, C_T2, C_T2C, C_FLIPC,
-    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_ERR, C_COUNT
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_ERR, C_COUNT
 };
 
 struct Dev {
@@ -53,6 +53,7 @@ struct Dev {
     // boundaries
     int* bvx; double* ben; double* blen; double* bt; uint8_t* benab; uint8_t* bcand; int* conf;
     int* pgat; double* pgt;          // polling gathers: 25 columns of kPollFast, t_ext
+    uint8_t* blab;                  // per edge, from k_label: bit 0 candidate and enabled, bit 1 enabled
     // grains
     int* gvx; int* glen; double* garea; double* gdadt; double* gori; double* gse;
     uint8_t* genab; uint8_t* gtype; uint8_t* gcand; int* gfix;
