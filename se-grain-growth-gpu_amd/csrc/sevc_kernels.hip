@@ -1234,11 +1234,15 @@ __global__ void k_two_grain(Dev d) {
     }
 }
 
-// k_two_scan and k_label3 in one pass over the grains, sharing the loads of
-// a 3-sided grain's list, its vertices' edges and grains and those edges'
-// ends and flags: the two-triangle candidates (listed once, by the lower
-// of the two grains) and the T2 labels as they stand before the
-// two-triangle stage (redone by k_label3 if that stage changes anything).
+// Two-triangle candidates and T2 labels in one pass over the grains,
+// sharing the loads of a 3-sided grain's list, its vertices' edges and
+// grains and those edges' ends and flags.  Two-triangle candidates
+// (topological.h:22-73): edges whose two grains are both 3-sided, found
+// from the 3-sided grains; for edge (a, c) of 3-sided grain g the two grains
+// are vg(a) ∩ vg(c), the set grains_intersect forms from the edge's ends,
+// and the edge is listed once, by the lower of the two grains.  The T2
+// labels (as k_label3) are those before the two-triangle stage; k_label3
+// redoes them if that stage changes anything.
 __global__ void k_scan3(Dev d) {
     TABLE_BLOCK(d.gblk, d.ngb)
     if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
@@ -1281,8 +1285,8 @@ __global__ void k_scan3(Dev d) {
         }
         ans /= 2;
         if (hit || ans == 3) cand = 1;
-        // two-triangle candidates (as k_two_scan): edge (a, c) of the grain
-        // with both of its grains 3-sided
+        // two-triangle candidates: edge (a, c) of the grain with both of its
+        // grains 3-sided
         for (int k = 0; k < 3; k++) {
             const int kc = (k + 1) % 3;
             const int c = vv[kc];
