@@ -216,8 +216,7 @@ int set_tiles(sevc_state* s, int T) {
 void launch_step(sevc_state* s, bool diag) {
     Dev d = s->d;
     d.diag = diag ? 1 : 0;
-    const int nb = (s->B + 63) / 64;
-    launch(s, "begin", k_begin, nb, 64, d);
+    launch(s, "begin", k_begin, 1, 1024, d);
     // Lengths come from k_text.  Energies change from step to step only with
     // GB_eps != 0; with GB_eps = 0 they are the scale, computed at the first
     // step after creation or after a new scale (for the members active then,
@@ -371,7 +370,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.mfix, 4 * V); DA(d.mgrn, 4 * V); DA(d.mvrt, 4 * V); DA(d.mprev, 4 * V);
     DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
-    DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E);
+    DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
     DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
     DA(d.cnt, C_COUNT);
@@ -517,6 +516,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         hipMemcpy(d.conf, minus1.data(), E * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(d.gfix, 0, G * 4) != hipSuccess || hipMemset(d.gcand, 0, G) != hipSuccess ||
         hipMemset(d.vnuc, 0, V) != hipSuccess || hipMemset(d.cnt, 0, C_COUNT * 4) != hipSuccess ||
+        hipMemset(d.n_flipped, 0, 4) != hipSuccess ||
         hipMemset(d.nucl_r, 0, V * 8) != hipSuccess || hipMemset(d.nucl_ori, 0, V * 8) != hipSuccess ||
         hipMemset(d.dE, 0, V * 8) != hipSuccess || hipMemset(d.nfac, 0, V * 8) != hipSuccess) {
         sevc_destroy(s);

@@ -30,30 +30,48 @@ namespace sevc {
 
 // Loop condition of main.cu:120 (time rule), grain rule, nucleation gate
 // main.cu:137-141; clears the per-step counters.
-__global__ void k_begin(Dev d) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m == 0)
+__global__ void __launch_bounds__(1024) k_begin(Dev d) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0)
         for (int c = 0; c < C_COUNT; c++) if (c != C_ERR) d.cnt[c] = 0;
-    if (m >= d.B) return;
-    const Params& P = d.prm[m];
-    MState& M = d.ms[m];
-    M.tau_bits = 0x7FF0000000000000ull;
-    M.nucl_now = 0;
-    if (M.done) { M.active = 0; return; }
-    bool tok = (P.tmax <= 0 && P.nmax <= 0) || (M.t <= P.tmax && P.tmax > 0) || (M.n <= P.nmax && P.nmax > 0);
-    bool gok = (P.min_grains <= 0) || (M.curr_grains >= P.min_grains);
-    if (!tok || !gok) { M.active = 0; M.done = 1; return; }
-    M.active = 1;
-    M.vsteps += M.n_venabled;
-    if (P.events && P.do_nucl) {
-        int gap = P.gap > 0 ? P.gap : 1;
-        if (P.nucl_mode == SEVC_NUCL_MC) {
-            if (M.curr_grains <= 5 * P.n_grains0 / 6) M.allow_nucl = 1;
-            M.nucl_now = ((M.n % gap) == 0 && M.allow_nucl) ? 1 : 0;
-        } else {
-            M.nucl_now = ((M.n % gap) == 0) ? 1 : 0;
+    for (int m = tid; m < d.B; m += nt) {
+        const Params& P = d.prm[m];
+        MState& M = d.ms[m];
+        M.tau_bits = 0x7FF0000000000000ull;
+        M.nucl_now = 0;
+        if (M.done) { M.active = 0; continue; }
+        bool tok = (P.tmax <= 0 && P.nmax <= 0) || (M.t <= P.tmax && P.tmax > 0) || (M.n <= P.nmax && P.nmax > 0);
+        bool gok = (P.min_grains <= 0) || (M.curr_grains >= P.min_grains);
+        if (!tok || !gok) { M.active = 0; M.done = 1; continue; }
+        M.active = 1;
+        M.vsteps += M.n_venabled;
+        if (P.events && P.do_nucl) {
+            int gap = P.gap > 0 ? P.gap : 1;
+            if (P.nucl_mode == SEVC_NUCL_MC) {
+                if (M.curr_grains <= 5 * P.n_grains0 / 6) M.allow_nucl = 1;
+                M.nucl_now = ((M.n % gap) == 0 && M.allow_nucl) ? 1 : 0;
+            } else {
+                M.nucl_now = ((M.n % gap) == 0) ? 1 : 0;
+            }
         }
     }
+    // flip (topological.h:286) leaves the flipped edge's energy without the
+    // GB_scaling factor until compute_boundary_energies (calculus.h:104-114)
+    // runs at the next step's start.  Steps that skip that pass (GB_eps = 0,
+    // where every other energy is the scale itself) recompute only the
+    // edges flipped since, for the members active in this step.
+    __syncthreads();
+    const int nf = min(*d.n_flipped, d.E);
+    for (int k = tid; k < nf; k += nt) {
+        const int b = d.l_flipped[k];
+        const int m = member_of_edge(d, b);
+        const Params& P = d.prm[m];
+        if (!d.ms[m].active || !d.benab[b]) continue;
+        const double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
+        d.ben[b] = e * P.gb_scaling;
+    }
+    __syncthreads();
+    if (tid == 0) *d.n_flipped = 0;
 }
 
 // Per-block counts of enabled grains and vertices (main.cu:317), written as
@@ -1560,6 +1578,11 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
     int t = d.vb[3 * A + 1]; d.vb[3 * A + 1] = d.vb[3 * A]; d.vb[3 * A] = t;
     t = d.vb[3 * B + 1]; d.vb[3 * B + 1] = d.vb[3 * B]; d.vb[3 * B] = t;
     d.ben[bnd] = boundary_compute_energy(d, bnd, P.gb_eps);   // no GB_scaling, topological.h:286
+    {
+        // rescaled at the next step's start (k_begin) when no full pass runs
+        const int k = atomicAdd(d.n_flipped, 1);
+        if (k < d.E) d.l_flipped[k] = bnd;
+    }
     d.vel[2 * A] = 0; d.vel[2 * A + 1] = 0;
     d.vel[2 * B] = 0; d.vel[2 * B + 1] = 0;
 }

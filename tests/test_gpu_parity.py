@@ -368,3 +368,15 @@ def test_nucleation_bursts(rate):
                          extra_capacity=0.15)
     tot = _run_pair(o, mesh, [1, 1, 2])
     assert tot["nucl"] >= (1000 if rate < 0.1 else 3000), tot
+
+
+def test_boundary_energy_scale_with_flips():
+    """GB_scaling != 1 with GB_eps = 0: a flipped edge's energy is left
+    without the scale until the next step recomputes it (topological.h:286),
+    and nucleation reads those energies.  Checked after every step so the
+    exported energies of each step's flipped edges are compared too."""
+    o, mesh = _generated(3000, 17, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3, GB_scaling=1.7,
+                         do_nucleation=1, nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02,
+                         nucleation_scaling=0.9, seed=3, extra_capacity=0.1)
+    tot = _run_pair(o, mesh, [1] * 6 + [40, 60])
+    assert tot["flips"] >= 50 and tot["nucl"] >= 10, tot
