@@ -380,3 +380,43 @@ def test_boundary_energy_scale_with_flips():
                          nucleation_scaling=0.9, seed=3, extra_capacity=0.1)
     tot = _run_pair(o, mesh, [1] * 6 + [40, 60])
     assert tot["flips"] >= 50 and tot["nucl"] >= 10, tot
+
+
+def test_two_triangle_collapse():
+    """Flip threshold twice the mean edge length: flips leave pairs of
+    adjacent 3-sided grains, which the two-triangle stage collapses (about
+    20 in 60 steps here; T2 labels are then taken again after the stage).
+    Compared after every chunk."""
+    n = 8000
+    o, mesh = _generated(n, 21, dt=1e-5, flip_threshold=2.0 * 0.6 * np.sqrt(1.0 / n))
+    tot = _run_pair(o, mesh, [1] * 10 + [10] * 5)
+    assert tot["two"] >= 5, tot
+
+
+def test_batch_mc_member_after_rate_member():
+    """Metropolis nucleation in a member whose slots start past another
+    member's (free-slot search and events at non-zero offsets), next to a
+    rate-mode member; each matches its own oracle."""
+    o1, m1 = _generated(1500, 3, dt=5e-5, do_nucleation=1, nucleation_mode=sevc.NUCL_RATE,
+                        nucleation_rate=0.02, seed=5, extra_capacity=0.1)
+    o2 = ref2k_options(domain_bound=40.0, dt=1e-3, dtau=1e-3, do_nucleation=1, leave_percent=0.05,
+                       SE_eps=1.0, MC_k=1.0)
+    o2.SE_min, o2.SE_max = 3.0, 6.0
+    m2 = sevc.load_files(o2)
+    st = sevc.State([o1, o2], [m1, m2])
+    orcs = [Oracle(o1, m1), Oracle(o2, m2)]
+    nucl = 0
+    for chunk in (400, 400, 400):
+        print(f"  chunk of {chunk} steps", flush=True)
+        st.step(chunk)
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            s1 = st.member_stats(k)
+            assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
+            for key in STAT_KEYS:
+                assert getattr(s1, key) == getattr(s2, key), (k, key)
+            if k == 1:
+                nucl += s1.n_nucleations
+    st.close()
+    assert nucl >= 20, nucl
