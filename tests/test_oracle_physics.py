@@ -222,3 +222,36 @@ def test_flip_threshold_spares_zero_length_edges():
         zero_seen += int(zero.sum())
         flips += st.n_flips
     assert flips > 0 and zero_seen > 0, (flips, zero_seen)
+
+
+def test_two_triangle_collapse_keeps_invariants():
+    """remove_two_grains_configurations (topological.h:22-73, 840-985) on a
+    run that makes pairs of adjacent 3-sided grains (flip threshold twice
+    the mean edge length): after every step with such collapses the mesh
+    passes check_data, tiles the domain (total area D^2) and keeps the torus
+    Euler characteristic, and the grain count drops by two per collapse
+    plus one per T2 removal."""
+    n = 8000
+    pos, rid = sevc.voronoi(n, 21)
+    rng = np.random.default_rng(21)
+    se = np.exp(np.log(0.3) - 0.125 + 0.5 * rng.standard_normal(n))
+    o = sevc.default_options()
+    o.GB_eps = 0.0
+    o.n_grains = n
+    o.dt = 1e-5
+    o.flip_threshold = 2.0 * 0.6 * np.sqrt(1.0 / n)
+    orc = Oracle(o, sevc.build_mesh(o, pos, rid, rng.random(n), se, n))
+    prev, seen = n, 0
+    for _ in range(60):
+        rc, st = orc.step(1)
+        assert rc == 0
+        assert prev - st.n_grains == st.n_removals + 2 * st.n_two_grain
+        prev = st.n_grains
+        if st.n_two_grain:
+            seen += st.n_two_grain
+            h = orc.get_mesh()
+            bad, total = sevc.check_mesh(h)
+            assert bad == 0
+            assert abs(total - 1.0) < 1e-12
+            assert _euler(h) == 0
+    assert seen >= 5
