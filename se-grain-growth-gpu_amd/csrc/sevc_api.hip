@@ -371,6 +371,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
     DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
+    DA(d.vmem, V); DA(d.emem, E); DA(d.gmem, G);
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
     DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
     DA(d.cnt, C_COUNT);
@@ -504,6 +505,15 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     UP(d.gdadt, gdadt.data(), G * 8); UP(d.gori, gori.data(), G * 8); UP(d.gse, gse.data(), G * 8);
     UP(d.genab, genab.data(), G); UP(d.gtype, gtype.data(), G);
     UP(dprm, prm.data(), prm.size() * sizeof(Params));
+    {
+        std::vector<int> vm(V), em(E), gm(G);
+        for (int m = 0; m < B; ++m) {
+            std::fill(vm.begin() + s->voff[m], vm.begin() + s->voff[m] + s->nv[m], m);
+            std::fill(em.begin() + s->eoff[m], em.begin() + s->eoff[m] + s->ne[m], m);
+            std::fill(gm.begin() + s->goff[m], gm.begin() + s->goff[m] + s->ng[m], m);
+        }
+        UP(d.vmem, vm.data(), V * 4); UP(d.emem, em.data(), E * 4); UP(d.gmem, gm.data(), G * 4);
+    }
     UP(d.ms, s->ms_host.data(), s->ms_host.size() * sizeof(MState));
 #undef UP
     // scratch initial values

@@ -69,35 +69,49 @@ __device__ __forceinline__ void grains_symdiff(const Dev& d, int a, int b, int o
 
 __device__ __forceinline__ int* glist(const Dev& d, int g) { return d.gvx + (size_t)g * d.S; }
 
-// Member owning a grain / edge slot (members own consecutive slot ranges).
-__device__ __forceinline__ int member_of_grain(const Dev& d, int g) {
-    int m = 0;
-    while (m + 1 < d.B && g >= d.prm[m + 1].goff) m++;
-    return m;
-}
-
-__device__ __forceinline__ int member_of_edge(const Dev& d, int e) {
-    int m = 0;
-    while (m + 1 < d.B && e >= d.prm[m + 1].eoff) m++;
-    return m;
-}
+// Member owning a vertex / grain / edge slot (members own consecutive slot
+// ranges; a batch keeps the owner of every slot, one load).
+__device__ __forceinline__ int member_of_vertex(const Dev& d, int v) { return d.B == 1 ? 0 : d.vmem[v]; }
+__device__ __forceinline__ int member_of_grain(const Dev& d, int g) { return d.B == 1 ? 0 : d.gmem[g]; }
+__device__ __forceinline__ int member_of_edge(const Dev& d, int e) { return d.B == 1 ? 0 : d.emem[e]; }
 
 // geometry.h:703-710
+// The list is read eight entries at a time (independent loads), then
+// searched in order: the first index holding v, or -1.
 __device__ __forceinline__ int grain_contains_vertex(const Dev& d, int g, int v) {
     const int* L = glist(d, g);
-    int n = d.glen[g];
-    for (int i = 0; i < n; i++) if (L[i] == v) return i;
+    const int n = d.glen[g];
+    for (int i0 = 0; i0 < n; i0 += 8) {
+        int x[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) x[q] = i0 + q < n ? L[i0 + q] : -2;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (x[q] == v) return i0 + q;
+    }
     return -1;
 }
 
-// geometry.h:724-734
+// geometry.h:724-734, reading the list eight entries (and the next one) at a
+// time: the first i (in order) with (L[i], L[(i+1) % n]) = (a, b) gives 1,
+// = (b, a) gives 2; -1 if none.
 __device__ __forceinline__ int grain_contains_consecutive(const Dev& d, int g, int a, int b) {
     const int* L = glist(d, g);
-    int n = d.glen[g];
-    for (int i = 0; i < n; i++) {
-        int nx = L[(i + 1) % n];
-        if (L[i] == a && nx == b) return 1;
-        if (L[i] == b && nx == a) return 2;
+    const int n = d.glen[g];
+    const int l0 = n > 0 ? L[0] : -2;
+    for (int i0 = 0; i0 < n; i0 += 8) {
+        int x[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int i = i0 + q;
+            x[q] = i < n ? L[i] : (i == n ? l0 : -2);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (i0 + q >= n) break;
+            if (x[q] == a && x[q + 1] == b) return 1;
+            if (x[q] == b && x[q + 1] == a) return 2;
+        }
     }
     return -1;
 }

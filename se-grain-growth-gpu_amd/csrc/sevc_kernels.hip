@@ -516,9 +516,7 @@ __global__ void k_velocity_fallback(Dev d) {
     const int n = d.cnt[C_VFALL];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int v = d.l_vfall[k];
-        int m = 0;
-        while (m + 1 < d.B && v >= d.prm[m + 1].voff) m++;
-        velocity_generic(d, d.prm[m], v, false);
+        velocity_generic(d, d.prm[member_of_vertex(d, v)], v, false);
     }
 }
 

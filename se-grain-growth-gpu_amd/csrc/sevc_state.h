@@ -55,6 +55,7 @@ struct Dev {
     int* pgat; double* pgt;          // polling gathers: 25 columns of kPollFast, t_ext
     uint8_t* blab;                  // per edge, from k_label: bit 0 candidate and enabled, bit 1 enabled
     int* l_flipped; int* n_flipped;  // edges flipped since the last step start (energies to rescale)
+    int* vmem; int* emem; int* gmem;  // owning member of each vertex / edge / grain slot (batches)
     // grains
     int* gvx; int* glen; double* garea; double* gdadt; double* gori; double* gse;
     uint8_t* genab; uint8_t* gtype; uint8_t* gcand; int* gfix;
