@@ -1394,11 +1394,42 @@ __global__ void k_label3(Dev d) {
 }
 
 // remove_grain, topological.h:687-822
+// grains_intersect on vertex-grain triples already in registers.
+__device__ __forceinline__ int rg_intersect(const int* ga, const int* gb, int out[2]) {
+    int k = 0;
+    out[0] = out[1] = -1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (ga[i] == gb[j]) {
+                out[k++] = ga[i];
+                if (k == 2) return 2;
+            }
+    return k;
+}
+
+// remove_3sided_grain.  Everything the removal reads is loaded first (the
+// grain's vertices; their edges, grains and positions; those edges' ends),
+// then the reference's sequence of tests and writes runs on local copies
+// in the same order -- a later test sees an earlier write exactly as it
+// would in memory -- and the writes go out.
 __device__ void remove_grain(const Dev& d, int g, double D) {
     const int* Lg = glist(d, g);
-    int A = Lg[0], B = Lg[1], C = Lg[2];
-    double xp0 = d.pos[2 * A], xp1 = d.pos[2 * B], xp2 = d.pos[2 * C];
-    double yp0 = d.pos[2 * A + 1], yp1 = d.pos[2 * B + 1], yp2 = d.pos[2 * C + 1];
+    const int A = Lg[0], B = Lg[1], C = Lg[2];
+    int vA[3], vB[3], vC[3], gA[3], gB[3], gC[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        vA[i] = d.vb[3 * A + i]; vB[i] = d.vb[3 * B + i]; vC[i] = d.vb[3 * C + i];
+        gA[i] = d.vg[3 * A + i]; gB[i] = d.vg[3 * B + i]; gC[i] = d.vg[3 * C + i];
+    }
+    const double xp0 = d.pos[2 * A], xp1 = d.pos[2 * B], xp2 = d.pos[2 * C];
+    const double yp0 = d.pos[2 * A + 1], yp1 = d.pos[2 * B + 1], yp2 = d.pos[2 * C + 1];
+    int xA[3][2], xB[3][2], xC[3][2];              // ends of the vertices' edges
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        xA[i][0] = d.bvx[2 * vA[i]]; xA[i][1] = d.bvx[2 * vA[i] + 1];
+        xB[i][0] = d.bvx[2 * vB[i]]; xB[i][1] = d.bvx[2 * vB[i] + 1];
+        xC[i][0] = d.bvx[2 * vC[i]]; xC[i][1] = d.bvx[2 * vC[i] + 1];
+    }
     double nAB = arclen(xp0, yp0, xp1, yp1, D);
     double nBC = arclen(xp1, yp1, xp2, yp2, D);
     double nCA = arclen(xp2, yp2, xp0, yp0, D);
@@ -1412,38 +1443,55 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     d.pos[2 * A] = dom_mod(ax, D);
     d.pos[2 * A + 1] = dom_mod(ay, D);
     int gic[2], G1, G2, G3;
-    grains_intersect(d, B, C, gic); G3 = (gic[0] == g) ? gic[1] : gic[0];
-    grains_intersect(d, A, B, gic); G2 = (gic[0] == g) ? gic[1] : gic[0];
-    grains_intersect(d, A, C, gic); G1 = (gic[0] == g) ? gic[1] : gic[0];
+    rg_intersect(gB, gC, gic); G3 = (gic[0] == g) ? gic[1] : gic[0];
+    rg_intersect(gA, gB, gic); G2 = (gic[0] == g) ? gic[1] : gic[0];
+    rg_intersect(gA, gC, gic); G1 = (gic[0] == g) ? gic[1] : gic[0];
     if (G1 < 0 || G2 < 0 || G3 < 0) { set_error(d, SEVC_E_STATE); return; }
     d.vel[2 * A] = 0.0;
     d.vel[2 * A + 1] = 0.0;
     for (int i = 0; i < 3; i++) {
-        int e = d.vb[3 * C + i];
-        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
-        if ((ei == B && ee == C) || (ee == B && ei == C)) { d.benab[e] = 0; break; }
+        const int ei = xC[i][0], ee = xC[i][1];
+        if ((ei == B && ee == C) || (ee == B && ei == C)) { d.benab[vC[i]] = 0; break; }
     }
-    int BEb = -1, CF = -1;
+    int BEb = -1, CF = -1, iBE = -1, iCF = -1;
     for (int i = 0; i < 3; i++) {
-        int e = d.vb[3 * B + i];
-        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
-        if (!(ei == A || ee == A) && !(ei == C || ee == C)) { BEb = e; break; }
+        const int ei = xB[i][0], ee = xB[i][1];
+        if (!(ei == A || ee == A) && !(ei == C || ee == C)) { BEb = vB[i]; iBE = i; break; }
     }
     for (int i = 0; i < 3; i++) {
-        int e = d.vb[3 * C + i];
-        int ei = d.bvx[2 * e], ee = d.bvx[2 * e + 1];
-        if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = e; break; }
+        const int ei = xC[i][0], ee = xC[i][1];
+        if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = vC[i]; iCF = i; break; }
     }
     if (BEb < 0 || CF < 0) { set_error(d, SEVC_E_STATE); return; }
-    if (d.bvx[2 * BEb] == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
-    if (d.bvx[2 * CF] == C) d.bvx[2 * CF] = A; else d.bvx[2 * CF + 1] = A;
+    // the ends of BEb and CF after the rewiring (CF's as seen after BEb's,
+    // should they be the same edge).  Only the end that changes is written:
+    // the other end of BEb or CF can be another removal's vertex, which that
+    // removal rewires in the same launch.
+    int be0 = xB[iBE][0], be1 = xB[iBE][1];
+    if (be0 == B) { be0 = A; d.bvx[2 * BEb] = A; }
+    else { be1 = A; d.bvx[2 * BEb + 1] = A; }
+    int cf0 = xC[iCF][0], cf1 = xC[iCF][1];
+    if (CF == BEb) { cf0 = be0; cf1 = be1; }
+    if (cf0 == C) { cf0 = A; d.bvx[2 * CF] = A; }
+    else { cf1 = A; d.bvx[2 * CF + 1] = A; }
+    if (CF == BEb) { be0 = cf0; be1 = cf1; }
+    // current ends of edge e (one of A's), after the two rewirings
+    auto ends = [&](int e, int i, int& e0, int& e1) {
+        if (e == CF) { e0 = cf0; e1 = cf1; }
+        else if (e == BEb) { e0 = be0; e1 = be1; }
+        else { e0 = xA[i][0]; e1 = xA[i][1]; }
+    };
+    int curA[3] = {vA[0], vA[1], vA[2]};
     for (int i = 0; i < 3; i++) {
-        int e = d.vb[3 * A + i];
-        if (d.bvx[2 * e] == B || d.bvx[2 * e + 1] == B) { d.benab[e] = 0; d.vb[3 * A + i] = BEb; break; }
+        int e0, e1;
+        ends(curA[i], i, e0, e1);
+        if (e0 == B || e1 == B) { d.benab[curA[i]] = 0; curA[i] = BEb; d.vb[3 * A + i] = BEb; break; }
     }
     for (int i = 0; i < 3; i++) {
-        int e = d.vb[3 * A + i];
-        if (d.bvx[2 * e] == C || d.bvx[2 * e + 1] == C) { d.benab[e] = 0; d.vb[3 * A + i] = CF; break; }
+        int e0, e1;
+        if (curA[i] == BEb) { e0 = be0; e1 = be1; }
+        else ends(curA[i], i, e0, e1);
+        if (e0 == C || e1 == C) { d.benab[curA[i]] = 0; d.vb[3 * A + i] = CF; break; }
     }
     d.ven[B] = 0;
     d.ven[C] = 0;
@@ -1453,7 +1501,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     set_mod(d, B, 1, FIX_REMOVE, G3, B); flag_grain(d, G3, 1);
     set_mod(d, C, 1, FIX_REPLACE, G3, A);
     d.mprev[4 * C + 1] = C;
-    for (int i = 0; i < 3; i++) if (d.vg[3 * A + i] == g) d.vg[3 * A + i] = G3;
+    for (int i = 0; i < 3; i++) if (gA[i] == g) d.vg[3 * A + i] = G3;
 }
 
 
@@ -1518,43 +1566,82 @@ __global__ void k_st2(Dev d) {
 // T1: flips (topological.h:174-291, 1016-1026)
 // ---------------------------------------------------------------------
 
-// flip, topological.h:174-291
+// flip, topological.h:174-291.  Every datum the flip reads is loaded first
+// (the edge's ends; their edges, grains and positions; the ends of the
+// rewired edges; the lists of the four grains around; their orientations),
+// then the reference's sequence of tests and writes runs on local copies in
+// the same order and the writes go out.
 __device__ void flip(const Dev& d, int bnd, const Params& P) {
     const double D = P.D;
-    int A = d.bvx[2 * bnd], B = d.bvx[2 * bnd + 1];
+    const int A = d.bvx[2 * bnd], B = d.bvx[2 * bnd + 1];
+    int vA[3], vB[3], gA[3], gB[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        vA[k] = d.vb[3 * A + k]; vB[k] = d.vb[3 * B + k];
+        gA[k] = d.vg[3 * A + k]; gB[k] = d.vg[3 * B + k];
+    }
+    const double pax = d.pos[2 * A], pay = d.pos[2 * A + 1], pbx = d.pos[2 * B], pby = d.pos[2 * B + 1];
     int gic[2], gnic[2];
-    grains_intersect(d, A, B, gic);
-    grains_symdiff(d, A, B, gnic);
+    rg_intersect(gA, gB, gic);
+    {
+        // grains_symdiff (geometry.h:886-898) on the triples
+        int gun[4];
+        gnic[0] = gnic[1] = -1;
+        gun[3] = -1;
+        for (int k = 0; k < 3; k++) gun[k] = gA[k];
+        for (int k = 0; k < 3; k++) {
+            const int g = gB[k];
+            if (g != gun[0] && g != gun[1] && g != gun[2]) { gun[3] = g; break; }
+        }
+        int c = 0;
+        for (int k = 0; k < 4; k++)
+            if (gun[k] != gic[0] && gun[k] != gic[1]) {
+                gnic[c++] = gun[k];
+                if (c == 2) break;
+            }
+    }
     int i = -1, j = -1, AC = -1, BEb = -1;
-    for (int k = 0; k < 3; k++) if (d.vb[3 * A + k] == bnd) { i = (k + 1) % 3; AC = d.vb[3 * A + i]; break; }
-    for (int k = 0; k < 3; k++) if (d.vb[3 * B + k] == bnd) { j = (k + 1) % 3; BEb = d.vb[3 * B + j]; break; }
+    for (int k = 0; k < 3; k++) if (vA[k] == bnd) { i = (k + 1) % 3; AC = vA[i]; break; }
+    for (int k = 0; k < 3; k++) if (vB[k] == bnd) { j = (k + 1) % 3; BEb = vB[j]; break; }
     if (i < 0 || j < 0 || gic[1] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); return; }
-    d.vb[3 * A + i] = BEb;
-    d.vb[3 * B + j] = AC;
-    if (d.bvx[2 * AC] == A) d.bvx[2 * AC] = B; else d.bvx[2 * AC + 1] = B;
-    if (d.bvx[2 * BEb] == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
+    // reads that do not depend on the writes below, issued together
+    const int ac0 = d.bvx[2 * AC], ac1 = d.bvx[2 * AC + 1];
+    const int be0 = d.bvx[2 * BEb], be1 = d.bvx[2 * BEb + 1];
+    const int c2[2] = {grain_contains_consecutive(d, gic[0], A, B), grain_contains_consecutive(d, gic[1], A, B)};
+    const int c4[2] = {grain_contains_consecutive(d, gic[0], B, A), grain_contains_consecutive(d, gic[1], B, A)};
+    const int h3[2] = {grain_contains_vertex(d, gnic[0], A), grain_contains_vertex(d, gnic[1], A)};
+    const int h1[2] = {grain_contains_vertex(d, gnic[0], B), grain_contains_vertex(d, gnic[1], B)};
+    const double oi0 = d.gori[gic[0]], oi1 = d.gori[gic[1]], on0 = d.gori[gnic[0]], on1 = d.gori[gnic[1]];
+    d.vb[3 * A + i] = BEb; vA[i] = BEb;
+    d.vb[3 * B + j] = AC; vB[j] = AC;
+    if (ac0 == A) d.bvx[2 * AC] = B; else d.bvx[2 * AC + 1] = B;
+    // BEb's ends as they are now (AC and BEb are distinct edges of a valid
+    // mesh; if they coincide, the write above is seen)
+    int nb0 = be0, nb1 = be1;
+    if (BEb == AC) { nb0 = (ac0 == A) ? B : ac0; nb1 = (ac0 == A) ? ac1 : B; }
+    if (nb0 == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
     int g1, g2, g3, g4;
     for (g2 = 0; g2 < 2; g2++)
-        if (grain_contains_consecutive(d, gic[g2], A, B) == 1) {
+        if (c2[g2] == 1) {
             set_mod(d, A, 0, FIX_REMOVE, gic[g2], A);
             flag_grain(d, gic[g2], 2);
             break;
         }
     for (g4 = 0; g4 < 2; g4++)
-        if (grain_contains_consecutive(d, gic[g4], B, A) == 1) {
+        if (c4[g4] == 1) {
             set_mod(d, B, 0, FIX_REMOVE, gic[g4], B);
             flag_grain(d, gic[g4], 2);
             break;
         }
     for (g3 = 0; g3 < 2; g3++)
-        if (grain_contains_vertex(d, gnic[g3], A) >= 0) {
+        if (h3[g3] >= 0) {
             set_mod(d, A, 1, FIX_ADD, gnic[g3], B);
             d.mprev[4 * A + 1] = A;
             flag_grain(d, gnic[g3], 2);
             break;
         }
     for (g1 = 0; g1 < 2; g1++)
-        if (grain_contains_vertex(d, gnic[g1], B) >= 0) {
+        if (h1[g1] >= 0) {
             set_mod(d, B, 1, FIX_ADD, gnic[g1], A);
             d.mprev[4 * B + 1] = B;
             flag_grain(d, gnic[g1], 2);
@@ -1562,20 +1649,31 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
         }
     if (g1 > 1 || g2 > 1 || g3 > 1 || g4 > 1) { set_error(d, SEVC_E_STATE); return; }
     for (int k = 0; k < 3; k++) {
-        if (d.vg[3 * A + k] == gic[g2]) d.vg[3 * A + k] = gnic[g1];
-        if (d.vg[3 * B + k] == gic[g4]) d.vg[3 * B + k] = gnic[g3];
+        if (gA[k] == gic[g2]) { gA[k] = gnic[g1]; d.vg[3 * A + k] = gnic[g1]; }
+        if (gB[k] == gic[g4]) { gB[k] = gnic[g3]; d.vg[3 * B + k] = gnic[g3]; }
     }
     // vector2_mean, geometry.h:311-317
-    double dx = wrapd(d.pos[2 * B], d.pos[2 * A], D);
-    double dy = wrapd(d.pos[2 * B + 1], d.pos[2 * A + 1], D);
-    double cx = d.pos[2 * A] + dx / 2, cy = d.pos[2 * A + 1] + dy / 2;
+    double dx = wrapd(pbx, pax, D);
+    double dy = wrapd(pby, pay, D);
+    double cx = pax + dx / 2, cy = pay + dy / 2;
     cx = dom_mod(cx, D);
     cy = dom_mod(cy, D);
     d.pos[2 * A] = cx; d.pos[2 * A + 1] = cy;
     d.pos[2 * B] = cx; d.pos[2 * B + 1] = cy;
-    int t = d.vb[3 * A + 1]; d.vb[3 * A + 1] = d.vb[3 * A]; d.vb[3 * A] = t;
-    t = d.vb[3 * B + 1]; d.vb[3 * B + 1] = d.vb[3 * B]; d.vb[3 * B] = t;
-    d.ben[bnd] = boundary_compute_energy(d, bnd, P.gb_eps);   // no GB_scaling, topological.h:286
+    d.vb[3 * A + 1] = vA[0]; d.vb[3 * A] = vA[1];
+    d.vb[3 * B + 1] = vB[0]; d.vb[3 * B] = vB[1];
+    // boundary_compute_energy on the rewired grains (no GB_scaling, topological.h:286)
+    double e = 1.0;
+    {
+        int gn[2];
+        if (rg_intersect(gA, gB, gn) == 2) {
+            auto ori = [&](int g) {
+                return g == gic[0] ? oi0 : g == gic[1] ? oi1 : g == gnic[0] ? on0 : g == gnic[1] ? on1 : d.gori[g];
+            };
+            e = gb_energy_func(ori(gn[0]) - ori(gn[1]), P.gb_eps);
+        }
+    }
+    d.ben[bnd] = e;
     {
         // rescaled at the next step's start (k_begin) when no full pass runs
         const int k = atomicAdd(d.n_flipped, 1);
