@@ -987,6 +987,7 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
     __shared__ short src[kMaxStride];            // index of the staged records, -1: read globally
     __shared__ unsigned char mf[4 * kMaxStride];
     __shared__ int mg[4 * kMaxStride], mv[4 * kMaxStride], mp[4 * kMaxStride];
+    __shared__ unsigned short kinds[kMaxStride];   // per staged entry: 4 bits each for ADD, REMOVE, REPLACE records of this grain
     __shared__ int n_add_s, n_fin_s;
     const int n_items = d.cnt[C_FIX0 + stage];
     const int lane = threadIdx.x;
@@ -999,12 +1000,20 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
             const int v = L[i];
             Ls[i] = v;
             src[i] = (short)i;
+            unsigned kd = 0;
             for (int j = 0; j < 4; j++) {
-                mf[4 * i + j] = d.mfix[4 * v + j];
-                mg[4 * i + j] = d.mgrn[4 * v + j];
+                const int f = d.mfix[4 * v + j], gg = d.mgrn[4 * v + j];
+                mf[4 * i + j] = (unsigned char)f;
+                mg[4 * i + j] = gg;
                 mv[4 * i + j] = d.mvrt[4 * v + j];
                 mp[4 * i + j] = d.mprev[4 * v + j];
+                if (gg == g) {
+                    if (f == FIX_ADD) kd |= 1u << j;
+                    else if (f == FIX_REMOVE) kd |= 1u << (4 + j);
+                    else if (f == FIX_REPLACE) kd |= 1u << (8 + j);
+                }
             }
+            kinds[i] = (unsigned short)kd;
         }
         __syncthreads();
         if (lane == 0) {
@@ -1017,9 +1026,13 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
                 }
             };
             int n = n0;
+            // a staged entry's kind bits name the records that can apply; an
+            // entry inserted or replaced during the replay checks all four
             for (int i = 0; i < n; i++) {                            // ADD, :1040-1067
                 const int es = src[i], v = Ls[i];
+                const unsigned km = es >= 0 ? (kinds[es] & 0xFu) : 0xFu;
                 for (int j = 0; j < 4; j++) {
+                    if (!((km >> j) & 1u)) continue;
                     int f, gg, vv, pp;
                     rec(es, v, j, f, gg, vv, pp);
                     if (f != FIX_ADD || gg != g) continue;
@@ -1037,7 +1050,9 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
             uint64_t doomed[kMaxStride / 64] = {0, 0};               // REMOVE, :1086-1106
             for (int i = 0; i < n; i++) {
                 const int es = src[i], v = Ls[i];
+                const unsigned km = es >= 0 ? ((kinds[es] >> 4) & 0xFu) : 0xFu;
                 for (int j = 0; j < 4; j++) {
+                    if (!((km >> j) & 1u)) continue;
                     int f, gg, vv, pp;
                     rec(es, v, j, f, gg, vv, pp);
                     if (f != FIX_REMOVE || gg != g) continue;
@@ -1051,7 +1066,9 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
             n = w;
             for (int i = 0; i < n; i++) {                            // REPLACE, :1124-1136
                 const int es = src[i], v = Ls[i];
+                const unsigned km = es >= 0 ? ((kinds[es] >> 8) & 0xFu) : 0xFu;
                 for (int j = 0; j < 4; j++) {
+                    if (!((km >> j) & 1u)) continue;
                     int f, gg, vv, pp;
                     rec(es, v, j, f, gg, vv, pp);
                     if (f != FIX_REPLACE || gg != g) continue;
