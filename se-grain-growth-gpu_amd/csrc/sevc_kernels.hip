@@ -627,7 +627,7 @@ __global__ void k_label(Dev d) {
     bool c = is_candidate(d.bt[i], d.blen[i], dt, P.flip_thr);
     d.bcand[i] = c ? 1 : 0;
     const uint8_t en = d.benab[i];
-    d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge
+    d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge (bit 2: k_poll)
     if (c && d.benab[i]) {
         int k = atomicAdd(&d.cnt[C_CAND], 1);
         d.l_cand[k] = i;
@@ -809,6 +809,7 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
             if (k < nc) {
                 const int c = cid[k];
                 d.bcand[c] = keep[u] ? 1 : 0;
+                d.blab[c] = (uint8_t)(keep[u] ? 7 : 3);     // + the label after polling (bit 2)
                 d.conf[c] = -1;
             }
         }
@@ -894,6 +895,11 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
     __syncthreads();
     const int ncf = d.cnt[C_CONF];
     for (int k = tid; k < ncf; k += nt) d.conf[d.l_conf[k]] = -1;
+    // the label after polling of every listed candidate, bit 2 of its byte
+    for (int k = tid; k < nc; k += nt) {
+        const int c = d.l_cand[k];
+        d.blab[c] = (uint8_t)(d.bcand[c] ? 7 : 3);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -1291,9 +1297,11 @@ __global__ void k_scan3(Dev d) {
             bb[e] = d.vb[3 * vv[e / 3] + e % 3];
             gv[e] = d.vg[3 * vv[e / 3] + e % 3];
         }
-        // the edges' ends, current labels and the (label, enabled) byte
-        // k_label left (its label is is_candidate at this step's dt, before
-        // polling): three loads per edge
+        // the edges' ends and the byte k_label and k_poll left: bit 0 the
+        // label before polling (is_candidate at this step's dt) of an enabled
+        // edge, bit 1 enabled, bit 2 the label after polling (an enabled
+        // edge's current label is set only if it was listed and kept); two
+        // loads per edge
         int ini[9], end[9];
         uint8_t bc[9], lab[9], be[9];
 #pragma unroll
@@ -1301,11 +1309,13 @@ __global__ void k_scan3(Dev d) {
             const int b = bb[e];
             ini[e] = d.bvx[2 * b];
             end[e] = d.bvx[2 * b + 1];
-            bc[e] = d.bcand[b];
             lab[e] = d.blab[b];
         }
 #pragma unroll
-        for (int e = 0; e < 9; e++) be[e] = (lab[e] >> 1) & 1;
+        for (int e = 0; e < 9; e++) {
+            be[e] = (lab[e] >> 1) & 1;
+            bc[e] = (lab[e] >> 2) & 1;
+        }
         // label_3sided_grains (as k_label3)
         auto in_g = [&](int x) { return x == v0 || x == v1 || x == v2; };
         bool hit = false;
