@@ -367,7 +367,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
 #define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
     DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
     DA(d.venergy, V); DA(d.vparts, 4 * V);
-    DA(d.mfix, 4 * V); DA(d.mgrn, 4 * V); DA(d.mvrt, 4 * V); DA(d.mprev, 4 * V);
+    DA(d.mods, V);
     DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
     DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
@@ -517,12 +517,12 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     UP(d.ms, s->ms_host.data(), s->ms_host.size() * sizeof(MState));
 #undef UP
     // scratch initial values
-    std::vector<uint8_t> fix_none(4 * V, 3);
     std::vector<int> minus1(std::max(4 * V, E), -1);
-    if (hipMemcpy(d.mfix, fix_none.data(), 4 * V, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.mgrn, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.mvrt, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.mprev, minus1.data(), 4 * V * 4, hipMemcpyHostToDevice) != hipSuccess ||
+    std::vector<ModRec> mods_none(V);
+    for (ModRec& r : mods_none) {
+        for (int j = 0; j < 4; j++) { r.grn[j] = -1; r.vrt[j] = -1; r.prev[j] = -1; r.fix[j] = FIX_NONE; }
+    }
+    if (hipMemcpy(d.mods, mods_none.data(), V * sizeof(ModRec), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.conf, minus1.data(), E * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(d.gfix, 0, G * 4) != hipSuccess || hipMemset(d.gcand, 0, G) != hipSuccess ||
         hipMemset(d.vnuc, 0, V) != hipSuccess || hipMemset(d.cnt, 0, C_COUNT * 4) != hipSuccess ||

@@ -10,7 +10,6 @@
 
 namespace sevc {
 
-enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17
 enum { T_COMMON = 0, T_NUCLEATED = 1 };                                  // geometry.h:19
 constexpr unsigned kStreamMcSite = 1u, kStreamMcPick = 2u, kStreamRate = 3u;
 constexpr int kMaxStride = 128;   // largest grain-list stride supported
@@ -169,17 +168,17 @@ __device__ __forceinline__ double grain_area(const Dev& d, int g, double D) { re
 // geometry.h:369-376
 __device__ __forceinline__ void reset_mods(const Dev& d, int v) {
     for (int j = 0; j < 4; j++) {
-        d.mfix[4 * v + j] = FIX_NONE;
-        d.mgrn[4 * v + j] = -1;
-        d.mvrt[4 * v + j] = -1;
-        d.mprev[4 * v + j] = -1;
+        d.mods[v].fix[j] = FIX_NONE;
+        d.mods[v].grn[j] = -1;
+        d.mods[v].vrt[j] = -1;
+        d.mods[v].prev[j] = -1;
     }
 }
 
 __device__ __forceinline__ void set_mod(const Dev& d, int v, int slot, int fix, int grn, int vrt) {
-    d.mfix[4 * v + slot] = (uint8_t)fix;
-    d.mgrn[4 * v + slot] = grn;
-    d.mvrt[4 * v + slot] = vrt;
+    d.mods[v].fix[slot] = (uint8_t)fix;
+    d.mods[v].grn[slot] = grn;
+    d.mods[v].vrt[slot] = vrt;
 }
 
 // Flag grain g for the fix stage `stage` (grain.fix = true) and list it once.

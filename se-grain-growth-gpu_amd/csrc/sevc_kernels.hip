@@ -945,8 +945,8 @@ __device__ void fix_one_grain(const Dev& d, int g) {
     for (int i = 0; i < d.glen[g]; i++) {                       // :1040-1067
         int v = L[i];
         for (int j = 0; j < 4; j++)
-            if (d.mfix[4 * v + j] == FIX_ADD && d.mgrn[4 * v + j] == g)
-                list_add_after(d, g, d.mvrt[4 * v + j], d.mprev[4 * v + j]);
+            if (d.mods[v].fix[j] == FIX_ADD && d.mods[v].grn[j] == g)
+                list_add_after(d, g, d.mods[v].vrt[j], d.mods[v].prev[j]);
     }
     {                                                           // :1086-1106
         // The reference gathers the REMOVE commands of the list's vertices
@@ -958,8 +958,8 @@ __device__ void fix_one_grain(const Dev& d, int g) {
         for (int i = 0; i < n; i++) {
             int v = L[i];
             for (int j = 0; j < 4; j++) {
-                if (d.mfix[4 * v + j] != FIX_REMOVE || d.mgrn[4 * v + j] != g) continue;
-                int w = d.mvrt[4 * v + j];
+                if (d.mods[v].fix[j] != FIX_REMOVE || d.mods[v].grn[j] != g) continue;
+                int w = d.mods[v].vrt[j];
                 for (int q = 0; q < n; q++)
                     if (L[q] == w) { doomed[q >> 6] |= 1ull << (q & 63); break; }
             }
@@ -973,8 +973,8 @@ __device__ void fix_one_grain(const Dev& d, int g) {
     for (int i = 0; i < d.glen[g]; i++) {                       // :1124-1136
         int v = L[i];
         for (int j = 0; j < 4; j++)
-            if (d.mfix[4 * v + j] == FIX_REPLACE && d.mgrn[4 * v + j] == g)
-                list_replace(d, g, d.mprev[4 * v + j], d.mvrt[4 * v + j]);
+            if (d.mods[v].fix[j] == FIX_REPLACE && d.mods[v].grn[j] == g)
+                list_replace(d, g, d.mods[v].prev[j], d.mods[v].vrt[j]);
     }
 }
 
@@ -1008,11 +1008,11 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
             src[i] = (short)i;
             unsigned kd = 0;
             for (int j = 0; j < 4; j++) {
-                const int f = d.mfix[4 * v + j], gg = d.mgrn[4 * v + j];
+                const int f = d.mods[v].fix[j], gg = d.mods[v].grn[j];
                 mf[4 * i + j] = (unsigned char)f;
                 mg[4 * i + j] = gg;
-                mv[4 * i + j] = d.mvrt[4 * v + j];
-                mp[4 * i + j] = d.mprev[4 * v + j];
+                mv[4 * i + j] = d.mods[v].vrt[j];
+                mp[4 * i + j] = d.mods[v].prev[j];
                 if (gg == g) {
                     if (f == FIX_ADD) kd |= 1u << j;
                     else if (f == FIX_REMOVE) kd |= 1u << (4 + j);
@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
                     const int q = 4 * es + j;
                     f = mf[q]; gg = mg[q]; vv = mv[q]; pp = mp[q];
                 } else {
-                    f = d.mfix[4 * v + j]; gg = d.mgrn[4 * v + j]; vv = d.mvrt[4 * v + j]; pp = d.mprev[4 * v + j];
+                    f = d.mods[v].fix[j]; gg = d.mods[v].grn[j]; vv = d.mods[v].vrt[j]; pp = d.mods[v].prev[j];
                 }
             };
             int n = n0;
@@ -1527,7 +1527,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     set_mod(d, C, 0, FIX_REMOVE, G1, C); flag_grain(d, G1, 1);
     set_mod(d, B, 1, FIX_REMOVE, G3, B); flag_grain(d, G3, 1);
     set_mod(d, C, 1, FIX_REPLACE, G3, A);
-    d.mprev[4 * C + 1] = C;
+    d.mods[C].prev[1] = C;
     for (int i = 0; i < 3; i++) if (gA[i] == g) d.vg[3 * A + i] = G3;
 }
 
@@ -1663,14 +1663,14 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
     for (g3 = 0; g3 < 2; g3++)
         if (h3[g3] >= 0) {
             set_mod(d, A, 1, FIX_ADD, gnic[g3], B);
-            d.mprev[4 * A + 1] = A;
+            d.mods[A].prev[1] = A;
             flag_grain(d, gnic[g3], 2);
             break;
         }
     for (g1 = 0; g1 < 2; g1++)
         if (h1[g1] >= 0) {
             set_mod(d, B, 1, FIX_ADD, gnic[g1], A);
-            d.mprev[4 * B + 1] = B;
+            d.mods[B].prev[1] = B;
             flag_grain(d, gnic[g1], 2);
             break;
         }

@@ -44,11 +44,21 @@ enum Counter {
     C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_ERR, C_COUNT
 };
 
+enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17
+
+// A vertex's four deferred grain-list edits (mod_t, geometry.h:26-31), in
+// one 64-byte record so that reading or clearing them touches one line.
+struct alignas(64) ModRec {
+    int grn[4], vrt[4], prev[4];
+    uint8_t fix[4];
+};
+static_assert(sizeof(ModRec) == 64, "one line per vertex");
+
 struct Dev {
     // vertices
     double* pos; double* vel; int* vb; int* vg; uint8_t* ven;
     double* venergy; double* vparts;
-    uint8_t* mfix; int* mgrn; int* mvrt; int* mprev;
+    ModRec* mods;                        // deferred grain-list edits, one 64-B record per vertex
     double* nucl_r; double* nucl_ori; double* dE; double* nfac; uint8_t* vnuc;
     // boundaries
     int* bvx; double* ben; double* blen; double* bt; uint8_t* benab; uint8_t* bcand; int* conf;
