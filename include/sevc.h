@@ -36,6 +36,7 @@ extern "C" {
 #define SEVC_E_GRAIN_FULL   -6   /* grain list would exceed max_vrt_per_grn    */
 #define SEVC_E_STATE        -7   /* inconsistent topology met during an event  */
 #define SEVC_E_NO_DEVICE    -8   /* no HIP device                              */
+#define SEVC_E_CHECK        -9   /* check_data failed with debug_lvl >= 1      */
 
 /* Reference fixed-size limits (geometry.h:9-11). */
 #define SEVC_REF_MAX_VRT_PER_GRN 100
@@ -207,8 +208,22 @@ int sevc_voronoi_generate(int32_t n_seeds, uint64_t seed,
 void sevc_free(void* p);
 
 /* Host port of check_data (utils.h:85-336).  Returns the number of
- * violations found (0 = valid) and writes the summed grain area. */
+ * violations found (0 = valid) and writes the summed grain area.  An id
+ * outside [0, n) counts once for the entity holding it (the checks that
+ * would follow it are skipped); a grain list is read up to
+ * min(vlen, max_vrt_per_grn) entries. */
 int sevc_check_mesh(const sevc_mesh* m, double* total_area);
+
+/* The same conditions counted on the GPU (check_data, utils.h:85-336, the
+ * checks main.cu:185-196,237-248,269-280,297-308 run with debug_lvl >= 1),
+ * one lane per vertex, edge and grain.  sevc_check_state checks the live
+ * device state of every member in place (nothing is copied to the host)
+ * and writes each member's violation count to bad[0..B-1].
+ * sevc_check_mesh_device copies a host mesh (valid or not) to the device,
+ * checks it there and returns its count in *bad.  The summed grain area of
+ * check_data is a printed warning in the reference and is not formed. */
+int sevc_check_state(sevc_state* s, int32_t* bad);
+int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad);
 
 /* Snapshot in the format of export_data (io.h:280-377) to
  * <folder>/<index as %06d>.txt. */

@@ -216,10 +216,14 @@ int orc_load_files(const sevc_options* o, sevc_mesh* out) {
 }
 
 /* check_data, utils.h:85-336: counts the conditions that set
- * is_valid = false (warnings are not counted). */
+ * is_valid = false (warnings are not counted).  The reference holds
+ * pointers; here an id outside [0, n) counts once for the entity holding
+ * it and the checks that would follow it are skipped, and a grain list is
+ * read up to min(vlen, S) entries. */
 int orc_check_mesh(const sevc_mesh* m, double* total_area) {
     int bad = 0;
     const int S = m->max_vrt_per_grn;
+    const int nv = m->n_vertices, ne = m->n_boundaries, ng = m->n_grains;
 #define BINI(b) (m->b_vtx[2 * (b)])
 #define BEND(b) (m->b_vtx[2 * (b) + 1])
 #define VBND(v, i) (m->v_bnd[3 * (v) + (i)])
@@ -227,16 +231,16 @@ int orc_check_mesh(const sevc_mesh* m, double* total_area) {
     for (int b = 0; b < m->n_boundaries; b++) {
         if (!m->b_enabled[b]) continue;
         int a = BINI(b), e = BEND(b);
-        if (a < 0 || e < 0 || a >= m->n_vertices || e >= m->n_vertices) { bad++; continue; }
+        if (a < 0 || e < 0 || a >= nv || e >= nv) { bad++; continue; }
         if (a == e) bad++;
         if (!m->v_enabled[a] || !m->v_enabled[e]) bad++;
         if (VBND(a, 0) != b && VBND(a, 1) != b && VBND(a, 2) != b) bad++;
         if (VBND(e, 0) != b && VBND(e, 1) != b && VBND(e, 2) != b) bad++;
         for (int i = 0; i < 3; i++) {
-            if (VBND(e, i) < 0 || !m->b_enabled[VBND(e, i)]) { bad++; break; }
+            if (VBND(e, i) < 0 || VBND(e, i) >= ne || !m->b_enabled[VBND(e, i)]) { bad++; break; }
         }
         for (int i = 0; i < 3; i++) {
-            if (VBND(a, i) < 0 || !m->b_enabled[VBND(a, i)]) { bad++; break; }
+            if (VBND(a, i) < 0 || VBND(a, i) >= ne || !m->b_enabled[VBND(a, i)]) { bad++; break; }
         }
         int gic[2] = {-1, -1}, k = 0;
         for (int i = 0; i < 3 && k < 2; i++)
@@ -247,7 +251,8 @@ int orc_check_mesh(const sevc_mesh* m, double* total_area) {
     for (int v = 0; v < m->n_vertices; v++) {
         if (!m->v_enabled[v]) continue;
         int ok = 1;
-        for (int j = 0; j < 3; j++) if (VBND(v, j) < 0 || VGRN(v, j) < 0) ok = 0;
+        for (int j = 0; j < 3; j++)
+            if (VBND(v, j) < 0 || VBND(v, j) >= ne || VGRN(v, j) < 0 || VGRN(v, j) >= ng) ok = 0;
         if (!ok) { bad++; continue; }
         if (VBND(v, 0) == VBND(v, 1) || VBND(v, 1) == VBND(v, 2) || VBND(v, 2) == VBND(v, 0)) bad++;
         if (VGRN(v, 0) == VGRN(v, 1) || VGRN(v, 1) == VGRN(v, 2) || VGRN(v, 2) == VGRN(v, 0)) bad++;
@@ -261,7 +266,8 @@ int orc_check_mesh(const sevc_mesh* m, double* total_area) {
             if (BINI(VBND(v, j)) != v && BEND(VBND(v, j)) != v) bad++;
         for (int j = 0; j < 3; j++) {
             int g = VGRN(v, j), found = 0;
-            for (int k = 0; k < m->g_vlen[g]; k++) if (m->g_vtx[(size_t)g * S + k] == v) found = 1;
+            int n = m->g_vlen[g] < S ? m->g_vlen[g] : S;
+            for (int k = 0; k < n; k++) if (m->g_vtx[(size_t)g * S + k] == v) found = 1;
             if (!found) bad++;
         }
     }
@@ -269,13 +275,14 @@ int orc_check_mesh(const sevc_mesh* m, double* total_area) {
     for (int g = 0; g < m->n_grains; g++) {
         if (!m->g_enabled[g]) continue;
         total += m->g_area ? m->g_area[g] : 0.0;
-        for (int j = 0; j < m->g_vlen[g]; j++) {
+        const int n = m->g_vlen[g] < S ? m->g_vlen[g] : S;
+        for (int j = 0; j < n; j++) {
             int v = m->g_vtx[(size_t)g * S + j];
-            if (v < 0 || !m->v_enabled[v]) bad++;
+            if (v < 0 || v >= nv || !m->v_enabled[v]) bad++;
         }
-        for (int j = 0; j < m->g_vlen[g]; j++) {
+        for (int j = 0; j < n; j++) {
             int v = m->g_vtx[(size_t)g * S + j];
-            if (v < 0) continue;
+            if (v < 0 || v >= nv) continue;
             if (VGRN(v, 0) != g && VGRN(v, 1) != g && VGRN(v, 2) != g) { bad++; break; }
         }
         int cnt = 0;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "sevc_state.h"
+#include "sevc_check.h"
 #include "../../include/sevc.h"
 
 namespace sevc {
@@ -86,6 +87,7 @@ struct sevc_state {
     bool any_nucl = false, any_events = false, any_adaptive = false, any_analytic = false;
     bool any_mc = false, any_rate = false;   // nucleation modes present
     bool any_aniso = false;                  // a member with GB_eps != 0 (energies change with the topology)
+    bool any_debug = false;                  // a member with debug_lvl >= 1 (checks inside the step)
     bool ben_dirty = true;                   // boundary energies not computed since creation / a new scale
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
@@ -212,6 +214,30 @@ int set_tiles(sevc_state* s, int T) {
     return SEVC_OK;
 }
 
+// The device check of check_data over the state's slots; gate = 1 keeps
+// it to the active members with debug_lvl >= 1.
+ChkView check_view(const Dev& d, int gate) {
+    ChkView c{};
+    c.vb = d.vb; c.vg = d.vg; c.ven = d.ven;
+    c.bvx = d.bvx; c.benab = d.benab;
+    c.gvx = d.gvx; c.glen = d.glen; c.genab = d.genab;
+    c.S = d.S; c.V = d.V; c.E = d.E; c.G = d.G; c.B = d.B;
+    c.vmem = d.vmem; c.emem = d.emem; c.gmem = d.gmem;
+    c.prm = d.prm; c.ms = d.ms; c.gate = gate; c.bad = d.chk_bad;
+    return c;
+}
+
+int check_grid(long long slots) { return static_cast<int>(std::min<long long>((slots + 255) / 256, 8192)); }
+
+// A check inside the step (main.cu:185-196, 237-248, 269-280, 297-308):
+// members with violations stop with SEVC_E_CHECK and the phase.
+void launch_check(sevc_state* s, int phase) {
+    const Dev& d = s->d;
+    launch(s, "check", k_check_clear, 1, 256, d.chk_bad, s->B);
+    launch(s, "check", k_check, check_grid((long long)d.V + d.E + d.G), 256, check_view(d, 1));
+    launch(s, "check", k_check_verdict, 1, 256, d.prm, d.ms, static_cast<const int*>(d.chk_bad), s->B, phase);
+}
+
 // One time step of every active member (main.cu:124-389).
 void launch_step(sevc_state* s, bool diag) {
     Dev d = s->d;
@@ -232,6 +258,7 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "nucl_insert", k_nucl_insert, kNuclInsGrid, 64, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     }
+    if (s->any_debug) launch_check(s, CHK_NUCLEATION);
     if (diag) {
         launch(s, "velocity_generic", k_velocity_generic, d.nvb, kBlock, d);
     } else {
@@ -251,16 +278,19 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "poll", k_poll, 1, 1024, d);
         launch(s, "scan3", k_scan3, d.ngb, kBlock, d);
         launch(s, "two_grain", k_two_grain, 1, 64, d);
+        if (s->any_debug) launch_check(s, CHK_TWO_GRAIN);
         launch(s, "label3", k_label3, d.ngb, kBlock, d);     // only after a two-triangle event
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
         launch(s, "fix", k_fix, kFixGrid, 64, d, 1);
         launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 1, 1);
+        if (s->any_debug) launch_check(s, CHK_REMOVAL);
         launch(s, "st2", k_st2, kListGrid, 256, d);
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
         launch(s, "fix", k_fix, kFixGrid, 64, d, 2);
         launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 2, 0);
+        if (s->any_debug) launch_check(s, CHK_FLIP);
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);
@@ -353,6 +383,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         if (opts[m].adaptive_dt) s->any_adaptive = true;
         if (opts[m].velocity_model == SEVC_VEL_ANALYTIC) s->any_analytic = true;
         if (opts[m].GB_eps != 0.0) s->any_aniso = true;
+        if (opts[m].debug_lvl >= 1) s->any_debug = true;
     }
     if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
     const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
@@ -363,6 +394,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     d.V = s->V;
     d.E = s->E;
     d.G = s->G;
+    d.debug = s->any_debug ? 1 : 0;
     int rc = SEVC_OK;
 #define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
     DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
@@ -372,6 +404,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
     DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
     DA(d.vmem, V); DA(d.emem, E); DA(d.gmem, G);
+    DA(d.chk_bad, static_cast<size_t>(B));
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
     DA(d.genab, G); DA(d.gtype, G); DA(d.gcand, G); DA(d.gfix, G);
     DA(d.cnt, C_COUNT);
@@ -486,6 +519,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         P.nucl_rate = o.nucleation_rate; P.nucl_scaling = o.nucleation_scaling; P.tmax = o.tmax;
         P.seed = o.seed; P.nmax = o.nmax; P.adaptive = o.adaptive_dt; P.nucl_mode = o.nucleation_mode;
         P.do_nucl = o.do_nucleation; P.gap = o.nucleation_gap; P.events = o.events; P.vel_model = o.velocity_model;
+        P.debug = o.debug_lvl;
         P.n_grains0 = ngr;
         P.min_grains = o.leave_percent < 0 ? -1 : static_cast<int>(std::floor(o.leave_percent * ngr));
         P.voff = vo; P.nv = me.n_vertices; P.eoff = eo; P.ne = me.n_boundaries; P.goff = go; P.ng = me.n_grains;
@@ -596,6 +630,18 @@ static int finish_call(sevc_state* s) {
     int err = 0;
     HIPCHK(hipMemcpy(&err, d.cnt + C_ERR, sizeof(int), hipMemcpyDeviceToHost));
     if (err != 0) return fail(s, err, std::string("device event error: ") + sevc_strerror(err));
+    // a member whose check failed during this call (debug_lvl >= 1): the
+    // reference's "Error during ..." lines (main.cu:193, 245, 277, 305)
+    static const char* const kPhase[] = {"", "nucleation", "remove two grains configuration", "Grain Removal",
+                                         "flipping"};
+    for (int m = 0; m < s->B; ++m) {
+        const MState& a = s->ms_host[static_cast<size_t>(m)];
+        if (a.status == SEVC_E_CHECK && s->ms_prev[static_cast<size_t>(m)].status != SEVC_E_CHECK) {
+            const int ph = a.chk_phase >= 1 && a.chk_phase <= 4 ? a.chk_phase : 0;
+            return fail(s, SEVC_E_CHECK, std::string("Error during ") + kPhase[ph] + " (member " +
+                                             std::to_string(m) + ", step " + std::to_string(a.n) + ")");
+        }
+    }
     return SEVC_OK;
 }
 
@@ -745,6 +791,69 @@ int sevc_get_positions(sevc_state* s, int32_t m, double* pos) {
                           hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return SEVC_OK;
+}
+
+int sevc_check_state(sevc_state* s, int32_t* bad) {
+    if (!s || !bad) return SEVC_E_INVALID_ARG;
+    const Dev& d = s->d;
+    hipLaunchKernelGGL(k_check_clear, dim3(1), dim3(256), 0, s->stream, d.chk_bad, s->B);
+    hipLaunchKernelGGL(k_check, dim3(check_grid((long long)d.V + d.E + d.G)), dim3(256), 0, s->stream,
+                       check_view(d, 0));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(bad, d.chk_bad, sizeof(int32_t) * static_cast<size_t>(s->B), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return SEVC_OK;
+}
+
+int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad) {
+    if (!m || !bad) return SEVC_E_INVALID_ARG;
+    const long long nv = m->n_vertices, ne = m->n_boundaries, ng = m->n_grains, S = m->max_vrt_per_grn;
+    if (nv <= 0 || ne <= 0 || ng <= 0 || S <= 0 || !m->v_bnd || !m->v_grn || !m->v_enabled || !m->b_vtx ||
+        !m->b_enabled || !m->g_vtx || !m->g_vlen || !m->g_enabled)
+        return SEVC_E_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SEVC_E_NO_DEVICE;
+    // one allocation: the arrays at 256-byte aligned offsets
+    struct Part { const void* src; size_t bytes; size_t off; };
+    Part parts[] = {{m->v_bnd, 12 * (size_t)nv, 0},   {m->v_grn, 12 * (size_t)nv, 0},
+                    {m->v_enabled, (size_t)nv, 0},    {m->b_vtx, 8 * (size_t)ne, 0},
+                    {m->b_enabled, (size_t)ne, 0},    {m->g_vtx, 4 * (size_t)(ng * S), 0},
+                    {m->g_vlen, 4 * (size_t)ng, 0},   {m->g_enabled, (size_t)ng, 0},
+                    {nullptr, sizeof(Params), 0},      {nullptr, sizeof(int), 0}};
+    size_t total = 0;
+    for (Part& p : parts) {
+        p.off = total;
+        total += (p.bytes + 255) / 256 * 256;
+    }
+    char* buf = nullptr;
+    if (hipMalloc(&buf, total) != hipSuccess) return SEVC_E_NOMEM;
+    Params P{};
+    P.voff = 0; P.nv = (int)nv; P.eoff = 0; P.ne = (int)ne; P.goff = 0; P.ng = (int)ng;
+    P.debug = 1;
+    parts[8].src = &P;
+    int rc = SEVC_OK;
+    for (const Part& p : parts)
+        if (p.src && hipMemcpy(buf + p.off, p.src, p.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SEVC_E_HIP;
+    if (rc == SEVC_OK) {
+        ChkView c{};
+        c.vb = (const int*)(buf + parts[0].off); c.vg = (const int*)(buf + parts[1].off);
+        c.ven = (const uint8_t*)(buf + parts[2].off);
+        c.bvx = (const int*)(buf + parts[3].off); c.benab = (const uint8_t*)(buf + parts[4].off);
+        c.gvx = (const int*)(buf + parts[5].off); c.glen = (const int*)(buf + parts[6].off);
+        c.genab = (const uint8_t*)(buf + parts[7].off);
+        c.S = (int)S; c.V = (int)nv; c.E = (int)ne; c.G = (int)ng; c.B = 1;
+        c.prm = (const Params*)(buf + parts[8].off);
+        c.gate = 0;
+        c.bad = (int*)(buf + parts[9].off);
+        hipLaunchKernelGGL(k_check_clear, dim3(1), dim3(256), 0, nullptr, c.bad, 1);
+        hipLaunchKernelGGL(k_check, dim3(check_grid(nv + ne + ng)), dim3(256), 0, nullptr, c);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(bad, c.bad, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = SEVC_E_HIP;
+    }
+    (void)hipFree(buf);
+    return rc;
 }
 
 int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const double* g_SE, double gb_scaling,

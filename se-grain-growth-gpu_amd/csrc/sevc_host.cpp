@@ -4,6 +4,7 @@
 #include "../../include/sevc.h"
 #include "../../include/sevc_math.h"
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -340,7 +341,10 @@ int sevc_mesh_load_files(const sevc_options* o, sevc_mesh* out) {
 }
 
 // check_data, utils.h:85-336: counts the conditions that make the
-// reference report invalid data (its warnings are not counted).
+// reference report invalid data (its warnings are not counted).  Where the
+// reference holds pointers, an id outside [0, n) counts once for the entity
+// holding it and the checks that would follow it are skipped; a grain list
+// is read up to min(vlen, S) entries.
 int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
     if (!m) return -1;
     const int S = m->max_vrt_per_grn;
@@ -359,10 +363,10 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
         if (vb(a, 0) != b && vb(a, 1) != b && vb(a, 2) != b) ++bad;
         if (vb(e, 0) != b && vb(e, 1) != b && vb(e, 2) != b) ++bad;
         bool dis = false;
-        for (int i = 0; i < 3; ++i) if (vb(e, i) < 0 || !m->b_enabled[vb(e, i)]) dis = true;
+        for (int i = 0; i < 3; ++i) if (vb(e, i) < 0 || vb(e, i) >= ne || !m->b_enabled[vb(e, i)]) dis = true;
         if (dis) ++bad;
         dis = false;
-        for (int i = 0; i < 3; ++i) if (vb(a, i) < 0 || !m->b_enabled[vb(a, i)]) dis = true;
+        for (int i = 0; i < 3; ++i) if (vb(a, i) < 0 || vb(a, i) >= ne || !m->b_enabled[vb(a, i)]) dis = true;
         if (dis) ++bad;
         int gic[2] = {-1, -1}, k = 0;
         for (int i = 0; i < 3 && k < 2; ++i)
@@ -373,7 +377,8 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
     for (int v = 0; v < nv; ++v) {
         if (!m->v_enabled[v]) continue;
         bool ok = true;
-        for (int j = 0; j < 3; ++j) if (vb(v, j) < 0 || vg(v, j) < 0) ok = false;
+        for (int j = 0; j < 3; ++j)
+            if (vb(v, j) < 0 || vb(v, j) >= ne || vg(v, j) < 0 || vg(v, j) >= ng) ok = false;
         if (!ok) { ++bad; continue; }
         if (vb(v, 0) == vb(v, 1) || vb(v, 1) == vb(v, 2) || vb(v, 2) == vb(v, 0)) ++bad;
         if (vg(v, 0) == vg(v, 1) || vg(v, 1) == vg(v, 2) || vg(v, 2) == vg(v, 0)) ++bad;
@@ -387,7 +392,8 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
         for (int j = 0; j < 3; ++j) {
             const int g = vg(v, j);
             bool found = false;
-            for (int k = 0; k < m->g_vlen[g]; ++k) if (m->g_vtx[static_cast<size_t>(g) * S + k] == v) found = true;
+            const int n = std::min(m->g_vlen[g], S);
+            for (int k = 0; k < n; ++k) if (m->g_vtx[static_cast<size_t>(g) * S + k] == v) found = true;
             if (!found) ++bad;
         }
     }
@@ -396,10 +402,11 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
         if (!m->g_enabled[g]) continue;
         total += m->g_area ? m->g_area[g] : 0.0;
         const int32_t* L = &m->g_vtx[static_cast<size_t>(g) * S];
-        for (int j = 0; j < m->g_vlen[g]; ++j) if (L[j] < 0 || !m->v_enabled[L[j]]) ++bad;
-        for (int j = 0; j < m->g_vlen[g]; ++j) {
+        const int n = std::min(m->g_vlen[g], S);
+        for (int j = 0; j < n; ++j) if (L[j] < 0 || L[j] >= nv || !m->v_enabled[L[j]]) ++bad;
+        for (int j = 0; j < n; ++j) {
             const int v = L[j];
-            if (v < 0) continue;
+            if (v < 0 || v >= nv) continue;
             if (vg(v, 0) != g && vg(v, 1) != g && vg(v, 2) != g) { ++bad; break; }
         }
         int cnt = 0;
@@ -599,6 +606,7 @@ const char* sevc_strerror(int code) {
         case SEVC_E_GRAIN_FULL: return "grain has more vertices than max_vrt_per_grn";
         case SEVC_E_STATE: return "inconsistent topology met during an event";
         case SEVC_E_NO_DEVICE: return "no HIP device";
+        case SEVC_E_CHECK: return "check_data failed (debug_lvl >= 1)";
         default: return "unknown error";
     }
 }

@@ -1543,6 +1543,7 @@ __global__ void k_t2_par(Dev d) {
     const int n = d.cnt[C_T2];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int g = d.l_t2[k];
+        if (d.debug && !d.ms[member_of_grain(d, g)].active) continue;   // stopped by a failed check
         const int* L = glist(d, g);
         bool clash = false;
         for (int q = 0; q < 3 && !clash; q++) {
@@ -1572,6 +1573,7 @@ __global__ void k_t2_serial(Dev d) {
         int g = d.l_t2c[k];
         if (!(d.genab[g] && d.gcand[g])) continue;
         int m = member_of_grain(d, g);
+        if (d.debug && !d.ms[m].active) continue;
         remove_grain(d, g, d.prm[m].D);
         d.ms[m].removals += 1;
     }
@@ -1583,6 +1585,7 @@ __global__ void k_st2(Dev d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int b = d.l_cand[k];
         if (!(d.benab[b] && d.bcand[b])) continue;
+        if (d.debug && !d.ms[member_of_edge(d, b)].active) continue;
         int gic[2];
         if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) { d.bcand[b] = 0; continue; }
         if (d.glen[gic[0]] == 3 || d.glen[gic[1]] == 3) d.bcand[b] = 0;
@@ -1721,6 +1724,7 @@ __global__ void k_flip_par(Dev d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int c = d.l_cand[k];
         if (!(d.benab[c] && d.bcand[c])) continue;
+        if (d.debug && !d.ms[member_of_edge(d, c)].active) continue;
         bool clash = false;
         for (int e = 0; e < 2 && !clash; e++) {
             int v = d.bvx[2 * c + e];
@@ -1749,6 +1753,7 @@ __global__ void k_flip_serial(Dev d) {
         int c = d.l_flipc[k];
         if (!(d.benab[c] && d.bcand[c])) continue;
         int m = member_of_edge(d, c);
+        if (d.debug && !d.ms[m].active) continue;
         flip(d, c, d.prm[m]);
         d.ms[m].flips += 1;
     }

@@ -24,12 +24,14 @@ struct Params {
     double gb_eps, gb_scaling, mc_k, nucl_rate, nucl_scaling, tmax;
     unsigned long long seed;
     int nmax, adaptive, nucl_mode, do_nucl, gap, events, vel_model, min_grains, n_grains0;
+    int debug;                       // debug_lvl: >= 1 runs the checks of check_data inside the step
     int voff, nv, eoff, ne, goff, ng;
 };
 
 // Per-member loop state (main.cu:30-113), updated on the device.
 struct MState {
     int n, active, done, curr_grains, allow_nucl, nucl_now, status, n_venabled;
+    int chk_phase;                   // phase of a failed check (debug_lvl >= 1), else 0
     double t, dt;
     unsigned long long tau_bits;     // min positive t_ext (as ordered bits)
     long long flips, removals, two, nucl, vsteps, steps;
@@ -90,6 +92,8 @@ struct Dev {
     unsigned long long* bmin;             // per edge block min t_ext bits
     int* nscr;                            // nucleation scratch, 12 ints per grain slot
     int diag;                             // compute export diagnostics this step
+    int debug;                            // a member runs the checks of check_data (debug_lvl >= 1)
+    int* chk_bad;                         // violations per member found by the last check
 };
 
 }  // namespace sevc

@@ -14,6 +14,7 @@ E_NOMEM = -5
 E_GRAIN_FULL = -6
 E_STATE = -7
 E_NO_DEVICE = -8
+E_CHECK = -9
 
 NUCL_MC = 0
 NUCL_RATE = 1

@@ -22,7 +22,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-Wno-unused-function", "-Wno-unused-variable"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
-HIP_SOURCES = ["sevc_kernels.hip", "sevc_api.hip", "sevc_adjoint.hip"]
+HIP_SOURCES = ["sevc_kernels.hip", "sevc_api.hip", "sevc_adjoint.hip", "sevc_check.hip"]
 CXX_SOURCES = ["sevc_host.cpp", "sevc_voronoi.cpp"]
 
 

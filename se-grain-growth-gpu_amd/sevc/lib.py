@@ -45,6 +45,8 @@ def lib():
                                             P(C.c_int32), P(P(C.c_int32))]),
         "sevc_free": (None, [C.c_void_p]),
         "sevc_check_mesh": (C.c_int, [P(Mesh), P(C.c_double)]),
+        "sevc_check_mesh_device": (C.c_int, [P(Mesh), P(C.c_int32)]),
+        "sevc_check_state": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "sevc_export_snapshot": (C.c_int, [P(Mesh), P(Options), C.c_char_p, C.c_int32, C.c_double]),
         "sevc_load_snapshot": (C.c_int, [C.c_char_p, C.c_int32, P(Mesh), P(C.c_double)]),
         "sevc_create": (C.c_int, [P(Options), P(Mesh), P(C.c_void_p)]),
@@ -156,6 +158,13 @@ def check_mesh(h):
     return n, ta.value
 
 
+def check_mesh_device(h):
+    """check_data's violation count of a host mesh, counted on the GPU."""
+    bad = C.c_int32()
+    _check(lib().sevc_check_mesh_device(C.byref(h.struct), C.byref(bad)))
+    return bad.value
+
+
 class State:
     """A device state holding one or more members (sevc_batch_create)."""
 
@@ -177,6 +186,13 @@ class State:
         f = lib().sevc_step_graph if graph else lib().sevc_step
         _check(f(self.h, int(n), C.byref(st)), self.h)
         return st
+
+    def check(self):
+        """check_data's violation count of every member, counted on the
+        device state in place."""
+        bad = (C.c_int32 * self.n_members)()
+        _check(lib().sevc_check_state(self.h, bad), self.h)
+        return list(bad)
 
     def member_stats(self, m):
         st = StepStats()

@@ -161,3 +161,79 @@ def diff_fields(a, b, fields=STATE_FIELDS):
         if neq.any():
             bad[f] = int(neq.sum())
     return bad
+
+
+def oracle_check(m):
+    """The oracle's check_data count of a host mesh."""
+    ta = C.c_double()
+    return oracle_lib().orc_check_mesh(C.byref(m.struct), C.byref(ta))
+
+
+def planted_corruptions(m):
+    """Corrupted copies of a valid mesh whose slots are all enabled, each
+    with its violation count worked out by hand from check_data's
+    conditions (utils.h:85-336) and the out-of-range rule (an id outside
+    [0, n) counts once for the entity holding it, and the checks that would
+    follow it are skipped).  Returns (name, mesh, expected) triples."""
+    out = []
+    nv, ne = m.nv, m.ne
+    S = m.S
+
+    # edge 10 disabled: both ends point to a disabled boundary (utils.h:190)
+    # and each of the 4 other edges at those ends has a disabled neighbour
+    # edge on that side (:132-145)
+    c = m.copy()
+    c["b_enabled"][10] = 0
+    out.append(("edge disabled", c, 6))
+
+    # vertex 20 disabled: its 3 edges have a disabled end (:99), its 3
+    # grains list a disabled vertex (:280)
+    c = m.copy()
+    c["v_enabled"][20] = 0
+    out.append(("vertex disabled", c, 6))
+
+    # grain 30 with vlen one short: count mismatch (:303-316), and the last
+    # vertex no longer finds the grain's list holding it (:236-253)
+    c = m.copy()
+    c["g_vlen"][30] -= 1
+    out.append(("grain list one short", c, 2))
+
+    # vertex 40's first boundary id out of range: the vertex (1); that
+    # boundary's end no longer lists it (:107-130) and has an out-of-range
+    # neighbour (:132-145) (2); the vertex's two other edges see it (2)
+    c = m.copy()
+    c["v_bnd"][3 * 40] = ne + 5
+    out.append(("boundary id out of range at a vertex", c, 5))
+
+    # grain 50's first list entry out of range: the grain (:280), and the
+    # vertex that was there no longer finds itself in the grain (:236-253)
+    c = m.copy()
+    c["g_vtx"][50 * S] = nv + 3
+    out.append(("vertex id out of range in a grain list", c, 2))
+
+    # edge 60's first end out of range: the edge (1), and its old end is no
+    # longer referenced by one of its boundaries (:227-234)
+    c = m.copy()
+    c["b_vtx"][2 * 60] = nv + 1
+    out.append(("edge end out of range", c, 2))
+    return out
+
+
+def random_corruptions(m, n, seed):
+    """n copies of m, each with one to three random entries of the
+    topology arrays overwritten by a random id (in range or not) or flag."""
+    rng = np.random.default_rng(seed)
+    fields = ["v_bnd", "v_grn", "v_enabled", "b_vtx", "b_enabled", "g_vtx", "g_vlen", "g_enabled"]
+    hi = {"v_bnd": m.ne, "v_grn": m.ng, "b_vtx": m.nv, "g_vtx": m.nv, "g_vlen": m.S}
+    out = []
+    for _ in range(n):
+        c = m.copy()
+        for _ in range(int(rng.integers(1, 4))):
+            f = fields[int(rng.integers(len(fields)))]
+            k = int(rng.integers(c[f].size))
+            if f in hi:
+                c[f][k] = int(rng.integers(-2, hi[f] + 3))
+            else:
+                c[f][k] = 1 - c[f][k]
+        out.append(c)
+    return out
