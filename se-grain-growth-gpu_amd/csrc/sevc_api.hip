@@ -79,6 +79,7 @@ struct sevc_state {
     int B = 0;
     std::vector<sevc_options> opts;
     std::vector<int> nv, ne, ng, voff, eoff, goff;
+    std::vector<int> Sm;                     // each member's max_vrt_per_grn (lists are stored at stride S)
     int V = 0, E = 0, G = 0, S = 0;
     Dev d{};
     std::vector<void*> allocs;
@@ -187,6 +188,7 @@ void collect_timers(sevc_state* s) {
 }
 
 constexpr int kListGrid = 512;
+int count_grid(const Dev& d) { return (d.ngb + d.neb + d.nvb + kCountWaves - 1) / kCountWaves; }
 // one wave per flagged grain in the fix stages: enough workgroups that
 // every grain of a busy step has its own
 constexpr int kFixGrid = 2048;
@@ -293,7 +295,7 @@ void launch_step(sevc_state* s, bool diag) {
         if (s->any_debug) launch_check(s, CHK_FLIP);
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
-    launch(s, "count_all", k_count_all, d.ngb + d.neb + d.nvb, kBlock, d);
+    launch(s, "count_all", k_count_all, count_grid(d), 64 * kCountWaves, d);
     launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 1);
 }
 
@@ -370,6 +372,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
             return SEVC_E_TOPOLOGY;
         }
         s->S = std::max(s->S, static_cast<int>(me.max_vrt_per_grn));
+        s->Sm.push_back(me.max_vrt_per_grn);
         s->voff.push_back(s->V); s->eoff.push_back(s->E); s->goff.push_back(s->G);
         s->nv.push_back(me.n_vertices); s->ne.push_back(me.n_boundaries); s->ng.push_back(me.n_grains);
         s->V += me.n_vertices; s->E += me.n_boundaries; s->G += me.n_grains;
@@ -502,7 +505,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         for (int g = 0; g < me.n_grains; ++g) {
             const size_t I = static_cast<size_t>(go + g);
             glen[I] = me.g_vlen[g];
-            if (glen[I] > s->S || glen[I] < 0) { sevc_destroy(s); return SEVC_E_INVALID_ARG; }
+            if (glen[I] > Sm || glen[I] < 0) { sevc_destroy(s); return SEVC_E_INVALID_ARG; }
             for (int k = 0; k < glen[I]; ++k) gvx[I * S + k] = gid(me.g_vtx[static_cast<size_t>(g) * Sm + k], vo);
             genab[I] = me.g_enabled[g] ? 1 : 0;
             ngr += genab[I];
@@ -520,6 +523,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         P.seed = o.seed; P.nmax = o.nmax; P.adaptive = o.adaptive_dt; P.nucl_mode = o.nucleation_mode;
         P.do_nucl = o.do_nucleation; P.gap = o.nucleation_gap; P.events = o.events; P.vel_model = o.velocity_model;
         P.debug = o.debug_lvl;
+        P.cap = me.max_vrt_per_grn;
         P.n_grains0 = ngr;
         P.min_grains = o.leave_percent < 0 ? -1 : static_cast<int>(std::floor(o.leave_percent * ngr));
         P.voff = vo; P.nv = me.n_vertices; P.eoff = eo; P.ne = me.n_boundaries; P.goff = go; P.ng = me.n_grains;
@@ -567,7 +571,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         return SEVC_E_HIP;
     }
     // per-block (enabled, free) counts, which the first step's nucleation reads
-    hipLaunchKernelGGL(k_count_all, dim3(d.ngb + d.neb + d.nvb), dim3(kBlock), 0, s->stream, d);
+    hipLaunchKernelGGL(k_count_all, dim3(count_grid(d)), dim3(64 * kCountWaves), 0, s->stream, d);
     if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
         sevc_destroy(s);
         return SEVC_E_HIP;
@@ -705,13 +709,13 @@ int sevc_member_sizes(const sevc_state* s, int32_t m, int32_t* nv, int32_t* ne, 
     if (nv) *nv = s->nv[m];
     if (ne) *ne = s->ne[m];
     if (ng) *ng = s->ng[m];
-    if (S) *S = s->S;
+    if (S) *S = s->Sm[m];
     return SEVC_OK;
 }
 
 int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
     if (!s || !out || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
-    const int nv = s->nv[m], ne = s->ne[m], ng = s->ng[m], S = s->S;
+    const int nv = s->nv[m], ne = s->ne[m], ng = s->ng[m], S = s->Sm[m];
     if (out->n_vertices != nv || out->n_boundaries != ne || out->n_grains != ng || out->max_vrt_per_grn != S)
         return fail(s, SEVC_E_INVALID_ARG, "output mesh sizes differ from the member's");
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -736,7 +740,10 @@ int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
     if (out->b_t_ext) ok &= down(out->b_t_ext, d.bt + eo, 8 * static_cast<size_t>(ne));
     ok &= down(out->b_enabled, d.benab + eo, static_cast<size_t>(ne));
     if (out->b_candidate) ok &= down(out->b_candidate, d.bcand + eo, static_cast<size_t>(ne));
-    ok &= down(out->g_vtx, d.gvx + go * S, 4 * static_cast<size_t>(ng) * S);
+    // the member's lists at its own stride S (held at the batch's stride d.S;
+    // no list is longer than S)
+    ok &= hipMemcpy2D(out->g_vtx, 4 * static_cast<size_t>(S), d.gvx + go * d.S, 4 * static_cast<size_t>(d.S),
+                      4 * static_cast<size_t>(S), static_cast<size_t>(ng), hipMemcpyDeviceToHost) == hipSuccess;
     ok &= down(out->g_vlen, d.glen + go, 4 * static_cast<size_t>(ng));
     if (out->g_area) ok &= down(out->g_area, d.garea + go, 8 * static_cast<size_t>(ng));
     if (out->g_dAdt) ok &= down(out->g_dAdt, d.gdadt + go, 8 * static_cast<size_t>(ng));

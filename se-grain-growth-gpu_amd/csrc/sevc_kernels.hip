@@ -74,26 +74,17 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d) {
     if (tid == 0) *d.n_flipped = 0;
 }
 
-// Per-block counts of enabled grains and vertices (main.cu:317), written as
-// one partial per table block and summed per member by k_member_reduce.
-__device__ __forceinline__ int block_count(int c) {
-    __shared__ int wsum[kBlock / 64];
-    unsigned long long bal = __ballot(c);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (int)__popcll(bal);
-    __syncthreads();
-    int r = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += wsum[w];
-    return r;
-}
-
 // Per table block, for grains, edges and vertices in one launch: the
 // numbers of enabled and of free slots.  The enabled counts feed the
-// grain / vertex totals of main.cu:317; the pairs also serve the next
-// step's nucleation (free-slot search, alive-vertex draw), since nothing
-// enables or disables a slot between the end of a step and nucleation.
-__global__ void k_count_all(Dev d) {
-    int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+// grain / vertex totals of main.cu:317 (one partial per table block, summed
+// per member by k_member_reduce); the pairs also serve the next step's
+// nucleation (free-slot search, alive-vertex draw), since nothing enables
+// or disables a slot between the end of a step and nucleation.  One wave
+// per table block: each lane reads four flags 64 bytes apart (four
+// coalesced 64-byte loads per wave) and the wave counts them by ballots.
+__global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
+    const int lane = threadIdx.x & 63;
+    int lb = (int)blockIdx.x * kCountWaves + (int)(threadIdx.x >> 6);
     if (lb >= d.ngb + d.neb + d.nvb) return;
     const Blk* tbl;
     const uint8_t* en;
@@ -106,10 +97,16 @@ __global__ void k_count_all(Dev d) {
         lb -= d.ngb + d.neb; tbl = d.vblk; en = d.ven; out = d.bc_v;
     }
     const Blk bd = tbl[lb];
-    const int i = bd.b + (int)threadIdx.x;
-    const int c = (i < bd.e && en[i]) ? 1 : 0;
-    const int r = block_count(c);
-    if (threadIdx.x == 0) out[lb] = make_int2(r, (bd.e - bd.b) - r);
+    uint8_t f[kBlock / 64];
+#pragma unroll
+    for (int j = 0; j < kBlock / 64; j++) {
+        const int i = bd.b + 64 * j + lane;
+        f[j] = i < bd.e ? en[i] : 0;
+    }
+    int r = 0;
+#pragma unroll
+    for (int j = 0; j < kBlock / 64; j++) r += (int)__popcll(__ballot(f[j] != 0));
+    if (lane == 0) out[lb] = make_int2(r, (bd.e - bd.b) - r);
 }
 
 // One workgroup per member: sums / minimises the per-block partials.
@@ -553,13 +550,17 @@ __global__ void k_grain_dadt(Dev d) {
 // update_vertices_positions, calculus.h:391-404
 __global__ void k_translate(Dev d) {
     TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active || i >= bd_.e || !d.ven[i]) return;
+    // the vertex's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const double2 p = in ? reinterpret_cast<const double2*>(d.pos)[i] : make_double2(0.0, 0.0);
+    const double2 v = in ? reinterpret_cast<const double2*>(d.vel)[i] : make_double2(0.0, 0.0);
+    const uint8_t en = in ? d.ven[i] : 0;
+    if (!M.active || !en) return;
     const double dt = member_dt(P, M);
-    double x = d.pos[2 * i], y = d.pos[2 * i + 1];
-    x += d.vel[2 * i] * dt;
-    y += d.vel[2 * i + 1] * dt;
-    d.pos[2 * i] = dom_mod(x, P.D);
-    d.pos[2 * i + 1] = dom_mod(y, P.D);
+    double x = p.x, y = p.y;
+    x += v.x * dt;
+    y += v.y * dt;
+    reinterpret_cast<double2*>(d.pos)[i] = make_double2(dom_mod(x, P.D), dom_mod(y, P.D));
 }
 
 // ---------------------------------------------------------------------
@@ -912,7 +913,7 @@ __device__ __forceinline__ void list_add_after(const Dev& d, int g, int v, int p
     int n = d.glen[g], pos = -1;
     for (int i = 0; i < n; i++) if (L[i] == prev) { pos = i + 1; break; }
     if (pos < 0) { set_error(d, SEVC_E_STATE); return; }
-    if (n >= d.S) { set_error(d, SEVC_E_GRAIN_FULL); return; }
+    if (n >= d.prm[member_of_grain(d, g)].cap) { set_error(d, SEVC_E_GRAIN_FULL); return; }
     for (int i = n; i > pos; i--) L[i] = L[i - 1];
     L[pos] = v;
     d.glen[g] = n + 1;
@@ -1002,6 +1003,7 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
         if (!d.genab[g]) continue;
         int* L = glist(d, g);
         const int n0 = d.glen[g];
+        const int cap = d.prm[member_of_grain(d, g)].cap;
         for (int i = lane; i < n0; i += 64) {
             const int v = L[i];
             Ls[i] = v;
@@ -1045,7 +1047,7 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
                     int pos = -1;                                    // grain_add_vertex, geometry.h:629-645
                     for (int q = 0; q < n; q++) if (Ls[q] == pp) { pos = q + 1; break; }
                     if (pos < 0) { set_error(d, SEVC_E_STATE); continue; }
-                    if (n >= d.S) { set_error(d, SEVC_E_GRAIN_FULL); continue; }
+                    if (n >= cap) { set_error(d, SEVC_E_GRAIN_FULL); continue; }
                     for (int q = n; q > pos; q--) { Ls[q] = Ls[q - 1]; src[q] = src[q - 1]; }
                     Ls[pos] = vv;
                     src[pos] = -1;
@@ -2491,7 +2493,7 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
         if (lane == 0) { n_s[0] = n_old[0]; n_s[1] = n_old[1]; n_s[2] = n_old[2]; }
         __syncthreads();
         if (lane == 0) {
-            const int S = d.S;
+            const int S = P.cap;
             auto ladd = [&](int t, int v, int prev) {               // grain_add_vertex
                 const int n = n_s[t];
                 int pos = -1;

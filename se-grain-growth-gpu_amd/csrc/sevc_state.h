@@ -17,6 +17,7 @@ constexpr int kTile = 64;            // grains per velocity tile (pass 1)
 constexpr int kMaxHistBins = 64;     // bins of the device area histogram
 constexpr int kNuclThreads = 1024;   // workgroup of the nucleation manager (one per member)
 constexpr int kPollFast = 2048;      // candidate lists up to this length poll from LDS (two per lane)
+constexpr int kCountWaves = 4;       // table blocks per workgroup of k_count_all (one wave each)
 
 // Per-member run parameters (from sevc_options).
 struct Params {
@@ -25,6 +26,7 @@ struct Params {
     unsigned long long seed;
     int nmax, adaptive, nucl_mode, do_nucl, gap, events, vel_model, min_grains, n_grains0;
     int debug;                       // debug_lvl: >= 1 runs the checks of check_data inside the step
+    int cap;                         // max_vrt_per_grn of this member (lists are stored at the batch's stride S >= cap)
     int voff, nv, eoff, ne, goff, ng;
 };
 

@@ -168,6 +168,38 @@ def test_batch_members_match_separate_oracles():
                 assert getattr(s1, key) == getattr(s2, key), (k, key)
 
 
+def test_list_strides_reference_and_largest():
+    """Grain-list strides of 32 (default), 100 (the reference's
+    MAX_VRT_PER_GRN) and 128 (the largest supported) in one batch, with
+    flips, removals and rate nucleation editing the lists: each member
+    equals its own oracle run at its own stride."""
+    ms, os_ = [], []
+    for S, gen_seed in ((32, 21), (100, 22), (128, 23)):
+        o, m = _generated(1200, gen_seed, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3,
+                          do_nucleation=1, nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02,
+                          nucleation_scaling=0.9, seed=gen_seed, extra_capacity=0.1, max_vrt_per_grn=S)
+        os_.append(o)
+        ms.append(m)
+    assert [m.S for m in ms] == [32, 100, 128]
+    st = sevc.State(os_, ms)
+    orcs = [Oracle(o, m) for o, m in zip(os_, ms)]
+    tot = 0
+    for chunk in (30, 90):
+        print(f"  chunk of {chunk} steps", flush=True)
+        st.step(chunk)
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            s1 = st.member_stats(k)
+            assert s1.status == 0
+            assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
+            for key in STAT_KEYS:
+                assert getattr(s1, key) == getattr(s2, key), (k, key)
+            tot += s1.n_flips + s1.n_removals + s1.n_nucleations
+    assert tot >= 100, tot
+    st.close()
+
+
 def _config1(seed, dt=2e-4):
     """2,000-grain fixture with the model options of the small-scale runs:
     gamma = mu = 1, SE = U[0,1) * 0.5, rate nucleation 1e-3 * SE per grain
