@@ -451,10 +451,30 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
 // listed for the two-pass path.
 __global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
     TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active || P.vel_model != SEVC_VEL_FD || i >= bd_.e || !d.ven[i]) return;
     const int v = i;
+    const bool in = i < bd_.e;
+    const uint8_t en = in ? d.ven[i] : 0;
+    // the vertex's data (position, edges, their ends, the neighbours'
+    // positions, the stored-energy terms) is loaded together with the
+    // member's state
+    const double2* X = reinterpret_cast<const double2*>(d.pos);
+    double2 p = make_double2(0.0, 0.0), q[3], t[3];
+    int b[3] = {0, 0, 0};
+    if (en) {
+        p = X[v];
+        int2 ends[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) b[j] = d.vb[3 * v + j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) ends[j] = reinterpret_cast<const int2*>(d.bvx)[b[j]];
+#pragma unroll
+        for (int j = 0; j < 3; j++) q[j] = X[ends[j].x == v ? ends[j].y : ends[j].x];
+#pragma unroll
+        for (int j = 0; j < 3; j++) t[j] = reinterpret_cast<const double2*>(d.vterm)[3 * (size_t)v + j];
+    }
+    if (!M.active || P.vel_model != SEVC_VEL_FD || !en) return;
     const double D = P.D;
-    const double x = d.pos[2 * v], y = d.pos[2 * v + 1];
+    const double x = p.x, y = p.y;
     const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
     const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
     if (!((pyy == y) && (qx == x))) {
@@ -464,24 +484,22 @@ __global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
     }
     const bool uni = (P.gb_eps == 0.0);
     double bx = 0.0, by = 0.0, gx = 0.0, gy = 0.0;
+#pragma unroll
     for (int j = 0; j < 3; j++) {
-        const int b = d.vb[3 * v + j];
-        const int nb = other_end(d, b, v);
-        const double nbx = d.pos[2 * nb], nby = d.pos[2 * nb + 1];
+        const double nbx = q[j].x, nby = q[j].y;
         const double lx = arclen(px, pyy, nbx, nby, D);
         const double ly = arclen(qx, qy, nbx, nby, D);
         // the stored arclength of b (compute_boundary_arclengths at the step's
         // positions, refreshed for edges moved by nucleation): wrap_dist is odd
         // and vector2_mag sign-symmetric, so |wrap(p_v - p_nb)| has its bits
         const double L0 = arclen(x, y, nbx, nby, D);
-        const double gam = uni ? P.gb_scaling : d.ben[b];
+        const double gam = uni ? P.gb_scaling : d.ben[b[j]];
         bx += (gam * (lx - L0));
         by += (gam * (ly - L0));
     }
-    const double* t = d.vterm + 6 * (size_t)v;
-    gx += t[0]; gy += t[1];
-    gx += t[2]; gy += t[3];
-    gx += t[4]; gy += t[5];
+    gx += t[0].x; gy += t[0].y;
+    gx += t[1].x; gy += t[1].y;
+    gx += t[2].x; gy += t[2].y;
     d.vel[2 * v] = (-(bx + gx) / SEVC_GRAD_EPS) * P.mobility;
     d.vel[2 * v + 1] = (-(by + gy) / SEVC_GRAD_EPS) * P.mobility;
 }
@@ -581,13 +599,23 @@ __global__ void k_text(Dev d) {
     const MState& M = d.ms[bd_.m];
     const int i = bd_.b + (int)threadIdx.x;
     unsigned long long v = 0x7FF0000000000000ull;
-    if (M.active && i < bd_.e && d.benab[i]) {
+    // the edge's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const uint8_t en = in ? d.benab[i] : 0;
+    const int2 ab = in ? reinterpret_cast<const int2*>(d.bvx)[i] : make_int2(0, 0);
+    double2 pa = make_double2(0.0, 0.0), pb = pa, va = pa, vb = pa;
+    if (en) {
+        pa = reinterpret_cast<const double2*>(d.pos)[ab.x];
+        pb = reinterpret_cast<const double2*>(d.pos)[ab.y];
+        va = reinterpret_cast<const double2*>(d.vel)[ab.x];
+        vb = reinterpret_cast<const double2*>(d.vel)[ab.y];
+    }
+    if (M.active && en) {
         const double D = P.D;
         d.bcand[i] = 0;
-        int a = d.bvx[2 * i], b = d.bvx[2 * i + 1];
         // adjust_origin_for_points on (ini, end): U0 = 0, U1 = 0 + wrap(end - ini)
-        double x1 = 0 + wrapd(d.pos[2 * b], d.pos[2 * a], D);
-        double y1 = 0 + wrapd(d.pos[2 * b + 1], d.pos[2 * a + 1], D);
+        double x1 = 0 + wrapd(pb.x, pa.x, D);
+        double y1 = 0 + wrapd(pb.y, pa.y, D);
         double dx = x1 - 0, dy = y1 - 0;
         double norm = mag(dx, dy);
         // compute_boundary_arclengths (calculus.h:83-93): the same |wrap(end - ini)|
@@ -599,8 +627,8 @@ __global__ void k_text(Dev d) {
             tx = sevc_cos(ang);
             ty = sevc_sin(ang);
         }
-        double px = d.vel[2 * b] - d.vel[2 * a];
-        double py = d.vel[2 * b + 1] - d.vel[2 * a + 1];
+        double px = vb.x - va.x;
+        double py = vb.y - va.y;
         double aux = tx * px + ty * py;
         double te = (aux == 0) ? 100.0 : -norm / aux;
         d.bt[i] = te;
@@ -623,13 +651,16 @@ __global__ void k_text(Dev d) {
 // compaction of enabled candidates.
 __global__ void k_label(Dev d) {
     TABLE_BLOCK(d.eblk, d.neb)
-    if (!M.active || !P.events || i >= bd_.e) return;
+    // the edge's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const double te = in ? d.bt[i] : 0.0, len = in ? d.blen[i] : 0.0;
+    const uint8_t en = in ? d.benab[i] : 0;
+    if (!M.active || !P.events || !in) return;
     const double dt = member_dt(P, M);
-    bool c = is_candidate(d.bt[i], d.blen[i], dt, P.flip_thr);
+    bool c = is_candidate(te, len, dt, P.flip_thr);
     d.bcand[i] = c ? 1 : 0;
-    const uint8_t en = d.benab[i];
     d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge (bit 2: k_poll)
-    if (c && d.benab[i]) {
+    if (c && en) {
         int k = atomicAdd(&d.cnt[C_CAND], 1);
         d.l_cand[k] = i;
         d.conf[i] = k;              // list position, for k_poll_gather
