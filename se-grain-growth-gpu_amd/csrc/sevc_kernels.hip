@@ -1317,10 +1317,14 @@ __global__ void k_two_grain(Dev d) {
 // redoes them if that stage changes anything.
 __global__ void k_scan3(Dev d) {
     TABLE_BLOCK(d.gblk, d.ngb)
-    if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
+    // the grain's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const uint8_t en = in ? d.genab[i] : 0;
+    const int len = in ? d.glen[i] : 0;
+    if (!M.active || !P.events || !en) return;
     const int g = i;
     uint8_t cand = 0;
-    if (d.glen[g] == 3) {
+    if (len == 3) {
         const int* L = glist(d, g);
         const int v0 = L[0], v1 = L[1], v2 = L[2];
         const int vv[3] = {v0, v1, v2};
@@ -1403,10 +1407,14 @@ __global__ void k_scan3(Dev d) {
 __global__ void k_label3(Dev d) {
     if (d.cnt[C_RELABEL] == 0) return;
     TABLE_BLOCK(d.gblk, d.ngb)
-    if (!M.active || !P.events || i >= bd_.e || !d.genab[i]) return;
+    // the grain's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const uint8_t en = in ? d.genab[i] : 0;
+    const int len = in ? d.glen[i] : 0;
+    if (!M.active || !P.events || !en) return;
     const int g = i;
     uint8_t cand = 0;
-    if (d.glen[g] == 3) {
+    if (len == 3) {
         // every datum of the 9 (vertex, edge) visits is loaded up front (three
         // rounds of independent loads), then the visits run in the order of
         // label_3sided_grains; "in g" is membership in the 3-entry list
@@ -1947,13 +1955,17 @@ __global__ void k_nucl_factor(Dev d) {
 // site's three grains and that shortest length.
 __global__ void k_rate_cand(Dev d) {
     TABLE_BLOCK(d.gblk, d.ngb)
+    // the grain's data is loaded together with the member's state
+    const bool in = i < bd_.e;
+    const uint8_t en = in ? d.genab[i] : 0;
+    const int len = in ? d.glen[i] : 0;
+    const double se = in ? d.gse[i] : 0.0;
     if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_RATE) return;
-    if (i >= bd_.e || !d.genab[i] || d.glen[i] < 1) return;
-    double p = P.nucl_rate * d.gse[i];
+    if (!en || len < 1) return;
+    double p = P.nucl_rate * se;
     double u0, u1;
     sevc_uniform2(P.seed, (uint32_t)M.n, (uint32_t)(i - P.goff), kStreamRate, 0u, &u0, &u1);
     if (!(u0 < p)) return;
-    const int len = d.glen[i];
     int k = (int)(u1 * (double)len);
     if (k >= len) k = len - 1;
     const int A = glist(d, i)[k];
