@@ -30,8 +30,8 @@ B_VELOCITY = 98.0
 B_TRANSLATE = 48.0
 # HBM bytes per enabled vertex of the two velocity passes at the 1M-grain
 # workload, from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy of known
-# size (profiles/step_1M_hbm_traffic.md): 517.7 MB for 2.0M vertices.
-TRAFFIC_VELOCITY_B_PER_VERTEX = 259.0
+# size (profiles/step_1M_hbm_traffic.md): 515.0 MB for 2.0M vertices.
+TRAFFIC_VELOCITY_B_PER_VERTEX = 258.0
 
 
 def log(*a):
