@@ -305,17 +305,22 @@ def bench_calibration(sevc, members, iters, steps, ckpt, dt, seed0=2000):
                         "20-bin area histograms; Adam on (log gamma, log mu) from (1, 1)"}
 
 
-def cpu_baseline(opts, mesh, steps):
+def cpu_baseline(opts, mesh, steps, omp=False):
+    """The oracle timed on the host: serial (the checker itself), or its
+    OpenMP build (per-entity physics loops on all threads OpenMP is given,
+    topology serial; the same results), a multi-core baseline only."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from _util import Oracle  # the oracle is the CPU comparator
-    orc = Oracle(opts, mesh)
+    orc = Oracle(opts, mesh, omp=omp)
     t0 = time.perf_counter()
     rc, st = orc.step(steps)
     dt = time.perf_counter() - t0
     orc.close()
-    return {"value": st.n_vertex_steps / dt, "unit": "vertex-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same 1M-grain workload on the serial FP64 oracle "
-                      f"(gcc -O2 -ffp-contract=off, 1 thread), {dt:.1f} s"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if omp else 1
+    how = f"OpenMP build, {threads} threads" if omp else "1 thread"
+    return {"value": st.n_vertex_steps / dt, "unit": "vertex-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same 1M-grain workload on the FP64 oracle "
+                      f"(gcc -O2 -ffp-contract=off, {how}), {dt:.1f} s"}
 
 
 def copy_calibration(mib, reps=10):
@@ -448,6 +453,7 @@ def main():
                                                 5e-7)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
+        line["cpu_baseline_openmp"] = cpu_baseline(opts, mesh, 2 * args.cpu_steps, omp=True)
     print(json.dumps(line), flush=True)
 
 

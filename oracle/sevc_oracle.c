@@ -73,6 +73,16 @@ struct orc {
     long long n_flips, n_removals, n_two, n_nucl;
 };
 
+/* The per-entity loops of the physics kernels below (each iteration writes
+ * only its own entity and reads nothing the loop writes) run in parallel in
+ * the OpenMP build, liboracle_omp.so, which serves as a multi-core timing
+ * baseline only.  The serial build is the checker; both give the same bits. */
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PARALLEL_FOR
+#endif
+
 /* ------------------------------------------------------------------ */
 /* geometry.h primitives                                               */
 /* ------------------------------------------------------------------ */
@@ -382,6 +392,7 @@ static double vertex_compute_energy(const orc* s, int v) {
 
 /* calculus.h:83-93 */
 static void compute_boundary_arclengths(orc* s) {
+    ORC_PARALLEL_FOR
     for (int b = 0; b < s->ne; b++)
         if (s->benab[b])
             s->blen[b] = arclen(PX(s, BI(s, b)), PY(s, BI(s, b)), PX(s, BE(s, b)), PY(s, BE(s, b)), s->D);
@@ -397,12 +408,14 @@ static double grain_area(const orc* s, int g) {
 
 /* calculus.h:224-234 */
 static void compute_grain_areas(orc* s) {
+    ORC_PARALLEL_FOR
     for (int g = 0; g < s->ng; g++)
         if (s->genab[g]) s->garea[g] = grain_area(s, g);
 }
 
 /* calculus.h:104-114 */
 static void compute_boundary_energies(orc* s) {
+    ORC_PARALLEL_FOR
     for (int b = 0; b < s->ne; b++)
         if (s->benab[b]) s->ben[b] = boundary_compute_energy(s, b, s->o.GB_eps) * s->o.GB_scaling;
 }
@@ -410,6 +423,7 @@ static void compute_boundary_energies(orc* s) {
 /* calculus.h:123-133 */
 static void compute_vertex_energies(orc* s) {
     if (!s->venergy) return;
+    ORC_PARALLEL_FOR
     for (int v = 0; v < s->nv; v++)
         if (s->ven[v]) s->venergy[v] = vertex_compute_energy(s, v);
 }
@@ -496,6 +510,7 @@ static void analytic_force(const orc* s, int v, double* fx, double* fy) {
 /* calculus.h:143-154 (+ mobility, + analytic model) */
 static void compute_vertex_velocities(orc* s) {
     double mu = s->o.mobility;
+    ORC_PARALLEL_FOR
     for (int v = 0; v < s->nv; v++) {
         if (!s->ven[v]) continue;
         if (s->o.velocity_model == SEVC_VEL_ANALYTIC) {
@@ -513,6 +528,7 @@ static void compute_vertex_velocities(orc* s) {
 /* calculus.h:243-253 */
 static void compute_grain_dadts(orc* s) {
     if (!s->gdadt) return;
+    ORC_PARALLEL_FOR
     for (int g = 0; g < s->ng; g++) {
         if (!s->genab[g]) continue;
         double X[SEVC_REF_MAX_VRT_PER_GRN * 4], Y[SEVC_REF_MAX_VRT_PER_GRN * 4];
@@ -529,6 +545,7 @@ static void compute_grain_dadts(orc* s) {
 
 /* calculus.h:391-404 */
 static void update_vertices_positions(orc* s, double dt) {
+    ORC_PARALLEL_FOR
     for (int v = 0; v < s->nv; v++) {
         if (!s->ven[v]) continue;
         PX(s, v) += VX(s, v) * dt;
@@ -544,6 +561,7 @@ static void update_vertices_positions(orc* s, double dt) {
 
 /* topological.h:303-334 */
 static void compute_t_ext(orc* s) {
+    ORC_PARALLEL_FOR
     for (int b = 0; b < s->ne; b++) {
         if (!s->benab[b]) continue;
         s->bcand[b] = 0;
@@ -582,6 +600,7 @@ static double step_dt(const orc* s) {
 
 /* topological.h:343-353 (all boundaries, enabled or not: quirk A-6) */
 static void label_boundaries(orc* s, double dt) {
+    ORC_PARALLEL_FOR
     for (int b = 0; b < s->ne; b++) {
         s->bcand[b] = (uint8_t)boundary_is_candidate(s, b, dt);
         s->conf[b] = -1;

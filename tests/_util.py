@@ -10,19 +10,22 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(REPO, "tests", "golden")
 REF2K = os.path.join(GOLDEN, "ref2k")
 ORACLE_SO = os.path.join(REPO, "oracle", "build", "liboracle.so")
+ORACLE_OMP_SO = os.path.join(REPO, "oracle", "build", "liboracle_omp.so")
 
 import sevc  # noqa: E402
 from sevc._abi import HostMesh, Mesh, Options, StepStats  # noqa: E402
 
-_orc = None
+_orc = {}
 
 
-def oracle_lib():
-    global _orc
-    if _orc is None:
-        if not os.path.exists(ORACLE_SO):
+def oracle_lib(omp=False):
+    """The serial oracle, or (omp=True) its OpenMP build, a timing baseline
+    that gives the same results."""
+    path = ORACLE_OMP_SO if omp else ORACLE_SO
+    if path not in _orc:
+        if not os.path.exists(path):
             subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
-        L = C.CDLL(ORACLE_SO)
+        L = C.CDLL(path)
         P = C.POINTER
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [P(Options), P(Mesh)]
@@ -55,15 +58,15 @@ def oracle_lib():
         L.orc_window_forward.argtypes = [C.c_void_p, C.c_int32]
         L.orc_adjoint_window.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]
-        _orc = L
-    return _orc
+        _orc[path] = L
+    return _orc[path]
 
 
 class Oracle:
     """The serial FP64 restatement of the reference step."""
 
-    def __init__(self, opts, mesh):
-        self.L = oracle_lib()
+    def __init__(self, opts, mesh, omp=False):
+        self.L = oracle_lib(omp)
         self.template = mesh
         self.h = self.L.orc_create(C.byref(opts), C.byref(mesh.struct))
         if not self.h:

@@ -31,3 +31,22 @@ def test_random_corruptions_oracle_equals_host():
         assert a == b
         counts.append(a)
     assert sum(1 for a in counts if a > 0) > 150      # most single edits are caught
+
+
+def test_openmp_oracle_build_gives_the_same_bits():
+    """The OpenMP build of the oracle (a timing baseline) runs the physics
+    loops on threads; its states equal the serial build's bit for bit."""
+    from _util import Oracle, diff_fields
+    import sevc as _s
+    o = ref2k_options(dt=2e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                      nucleation_mode=_s.NUCL_RATE, nucleation_rate=1e-3, nucleation_scaling=0.9, seed=4,
+                      extra_capacity=0.05)
+    m = _ref2k()
+    a, b = Oracle(o, m), Oracle(o, m, omp=True)
+    for n in (10, 60):
+        ra, sa = a.step(n)
+        rb, sb = b.step(n)
+        assert ra == 0 and rb == 0
+        assert diff_fields(a.get_mesh(), b.get_mesh()) == {}
+        assert (sa.n_flips, sa.n_removals, sa.n_nucleations) == (sb.n_flips, sb.n_removals, sb.n_nucleations)
+    assert sa.n_flips > 0
