@@ -1316,15 +1316,27 @@ __global__ void k_two_grain(Dev d) {
 // labels (as k_label3) are those before the two-triangle stage; k_label3
 // redoes them if that stage changes anything.
 __global__ void k_scan3(Dev d) {
+    __shared__ int q_s[kBlock];       // the block's enabled 3-sided grains
+    __shared__ int nq;
     TABLE_BLOCK(d.gblk, d.ngb)
     // the grain's data is loaded together with the member's state
     const bool in = i < bd_.e;
     const uint8_t en = in ? d.genab[i] : 0;
     const int len = in ? d.glen[i] : 0;
-    if (!M.active || !P.events || !en) return;
-    const int g = i;
+    if (threadIdx.x == 0) nq = 0;
+    __syncthreads();
+    const bool run = M.active && P.events;          // the same for the whole workgroup
+    if (run && en) {
+        if (len == 3) q_s[atomicAdd(&nq, 1)] = i;
+        else d.gcand[i] = 0;
+    }
+    __syncthreads();
+    // the 3-sided grains go to the first lanes, so that one wave carries
+    // their chains of loads and the others finish at once
+    if (!run || (int)threadIdx.x >= nq) return;
+    const int g = q_s[threadIdx.x];
     uint8_t cand = 0;
-    if (len == 3) {
+    {
         const int* L = glist(d, g);
         const int v0 = L[0], v1 = L[1], v2 = L[2];
         const int vv[3] = {v0, v1, v2};
