@@ -1,0 +1,52 @@
+"""Input and argument validation of the C ABI on a device: the codes the
+library returns where the reference calls exit() or would read invalid
+pointers (utils.h:17-44, check_data at load).  A state that rejected a
+call keeps working."""
+import pytest
+
+import sevc
+from sevc._abi import HostMesh
+from _util import planted_corruptions, ref2k_options
+
+pytestmark = pytest.mark.gpu
+
+
+def _code(fn):
+    with pytest.raises(sevc.SevcError) as e:
+        fn()
+    return e.value.code
+
+
+def test_create_rejects_invalid_meshes_and_options():
+    o = ref2k_options()
+    m = sevc.load_files(o)
+    print("  empty mesh", flush=True)
+    assert _code(lambda: sevc.State(o, HostMesh(0, 0, 0, 32))) == sevc.E_INVALID_ARG
+    assert _code(lambda: sevc.State([], [])) == sevc.E_INVALID_ARG
+    for name, c, _ in planted_corruptions(m):
+        print("  corrupted:", name, flush=True)
+        assert _code(lambda: sevc.State(o, c)) == sevc.E_TOPOLOGY, name
+    print("  list stride above 128", flush=True)
+    o129 = ref2k_options(max_vrt_per_grn=129)
+    assert _code(lambda: sevc.State(o129, sevc.load_files(o129))) == sevc.E_INVALID_ARG
+    print("  dt = 0", flush=True)
+    o0 = ref2k_options(dt=0.0)
+    assert _code(lambda: sevc.State(o0, m)) == sevc.E_INVALID_ARG
+    print("  a corrupted member rejects the whole batch", flush=True)
+    bad = planted_corruptions(m)[0][1]
+    assert _code(lambda: sevc.State([o, o], [m, bad])) == sevc.E_TOPOLOGY
+
+
+def test_calls_reject_bad_arguments_and_state_keeps_working():
+    o = ref2k_options(dt=2e-4)
+    m = sevc.load_files(o)
+    st = sevc.State(o, m)
+    assert _code(lambda: st.step(-1)) == sevc.E_INVALID_ARG
+    assert _code(lambda: st.get_mesh(1)) == sevc.E_INVALID_ARG
+    assert _code(lambda: st.member_stats(3)) == sevc.E_INVALID_ARG
+    assert _code(lambda: st.positions(2)) == sevc.E_INVALID_ARG
+    assert _code(lambda: sevc.check_mesh_device(HostMesh(0, 0, 0, 32))) == sevc.E_INVALID_ARG
+    s = st.step(150)
+    assert s.status == 0 and s.steps_done == 150 and s.n_flips > 0
+    assert st.check() == [0]
+    st.close()
