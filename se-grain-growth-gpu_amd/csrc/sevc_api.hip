@@ -18,7 +18,7 @@ namespace sevc {
 // kernels of sevc_kernels.hip
 __global__ void k_begin(Dev d);
 __global__ void k_count_all(Dev d);
-__global__ void k_member_reduce(Dev d, int what);
+__global__ void k_member_reduce(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
@@ -271,9 +271,6 @@ void launch_step(sevc_state* s, bool diag) {
     if (s->any_analytic) launch(s, "velocity_analytic", k_velocity_analytic, d.nvb, kBlock, d);
     if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
     launch(s, "text", k_text, d.neb, kBlock, d);
-    if (s->any_adaptive) {
-        launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 0);
-    }
     if (s->any_events) {
         launch(s, "label", k_label, d.neb, kBlock, d);
         launch(s, "poll_gather", k_poll_gather, kPollFast / 64, 64, d);
@@ -296,7 +293,7 @@ void launch_step(sevc_state* s, bool diag) {
     }
     launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, count_grid(d), 64 * kCountWaves, d);
-    launch(s, "member_reduce", k_member_reduce, s->B, 1024, d, 1);
+    launch(s, "member_reduce", k_member_reduce, s->B, 1024, d);
 }
 
 void build_table(std::vector<Blk>& tbl, std::vector<int>& first, const std::vector<int>& off,
@@ -431,7 +428,6 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.vblk_first, vf.size()); DA(d.eblk_first, ef.size()); DA(d.gblk_first, gf.size());
     DA(d.bc_v, vt.size()); DA(d.bc_e, et.size()); DA(d.bc_g, gt.size());
     DA(d.partial, 2 * gt.size() + 2);
-    DA(d.bmin, et.size());
     d.vblk = dvt; d.eblk = det; d.gblk = dgt;
     {
         // room for the smallest tile size the autotuner tries

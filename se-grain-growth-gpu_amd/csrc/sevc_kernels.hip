@@ -109,10 +109,9 @@ __global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
     if (lane == 0) out[lb] = make_int2(r, (bd.e - bd.b) - r);
 }
 
-// One workgroup per member: sums / minimises the per-block partials.
-// what = 0: tau (adaptive dt); 1: grain and vertex counts.
-__global__ void __launch_bounds__(1024) k_member_reduce(Dev d, int what) {
-    __shared__ unsigned long long sm[16];
+// One workgroup per member: sums the per-block partials of the enabled
+// grain and vertex counts.
+__global__ void __launch_bounds__(1024) k_member_reduce(Dev d) {
     __shared__ long long sa[16], sb[16];
     const int m = blockIdx.x;
     if (m >= d.B) return;
@@ -121,30 +120,9 @@ __global__ void __launch_bounds__(1024) k_member_reduce(Dev d, int what) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     // each lane reads its share of the partials 16 at a time (independent
     // loads), then the waves reduce with shuffles and wave 0 reduces the
-    // wave results: min and integer sums, so any order gives the same value
+    // wave results: integer sums, so any order gives the same value
     constexpr int kBatch = 16;
-    if (what == 0) {
-        unsigned long long v = 0x7FF0000000000000ull;
-        const int lo = d.eblk_first[m], hi = d.eblk_first[m + 1];
-        for (int b0 = lo + tid; b0 < hi; b0 += kBatch * nt) {
-            unsigned long long t[kBatch];
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < hi ? d.bmin[b0 + q * nt] : 0x7FF0000000000000ull;
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) v = t[q] < v ? t[q] : v;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long u = __shfl_xor(v, o, 64);
-            v = u < v ? u : v;
-        }
-        if (lane == 0) sm[wave] = v;
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long r = sm[0];
-            for (int w = 1; w < nw; w++) r = sm[w] < r ? sm[w] : r;
-            M.tau_bits = r;
-        }
-    } else {
+    {
         long long a = 0, b2 = 0;
         const int glo = d.gblk_first[m], ghi = d.gblk_first[m + 1];
         const int vlo = d.vblk_first[m], vhi = d.vblk_first[m + 1];
@@ -585,18 +563,19 @@ __global__ void k_translate(Dev d) {
 // topological.h: extinction times, labels, votes
 // ---------------------------------------------------------------------
 
-// compute_t_ext, topological.h:303-334, fused with the adaptive-dt partial:
-// the smallest positive t_ext of the block's enabled edges not shorter than
-// the flip threshold, one value per table block (no contended atomics),
-// reduced per member by k_member_reduce.  The minimum does not depend on
-// the order, so the result equals a serial scan.
+// compute_t_ext, topological.h:303-334, fused with the adaptive-dt minimum:
+// the smallest positive t_ext of the member's enabled edges not shorter than
+// the flip threshold, reduced per block and then into the member's value by
+// one atomic minimum per block (skipped when the block's is not smaller).
+// The minimum does not depend on the order, so the result equals a serial
+// scan.
 __global__ void k_text(Dev d) {
     __shared__ unsigned long long wmin[kBlock / 64];
     const int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (lb_ >= d.neb) return;
     const Blk bd_ = d.eblk[lb_];
     const Params& P = d.prm[bd_.m];
-    const MState& M = d.ms[bd_.m];
+    MState& M = d.ms[bd_.m];
     const int i = bd_.b + (int)threadIdx.x;
     unsigned long long v = 0x7FF0000000000000ull;
     // the edge's data is loaded together with the member's state
@@ -643,7 +622,10 @@ __global__ void k_text(Dev d) {
     if (threadIdx.x == 0) {
         unsigned long long r = wmin[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = wmin[w] < r ? wmin[w] : r;
-        d.bmin[lb_] = r;
+        // the member's minimum (k_begin set it to +inf): the smallest value
+        // whatever the order, so the result equals a serial scan; a block
+        // whose minimum is not below the value already there skips the atomic
+        if (r < M.tau_bits) atomicMin(&M.tau_bits, r);
     }
 }
 

@@ -91,7 +91,6 @@ struct Dev {
     int* l_nev;                           // nucleation events of the step: (member, A, B, C, AB, BC, CA, g)
     int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
     double* partial;                      // per grain block partial sums (2 levels)
-    unsigned long long* bmin;             // per edge block min t_ext bits
     int* nscr;                            // nucleation scratch, 12 ints per grain slot
     int diag;                             // compute export diagnostics this step
     int debug;                            // a member runs the checks of check_data (debug_lvl >= 1)
