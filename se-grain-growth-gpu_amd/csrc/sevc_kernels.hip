@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
         __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p], b0 = off[g], n = off[g + 1] - b0, k = p - b0;
-            const int v = Ls[p], gg = gs + g;
+            const int v = Ls[p];
             const double x = Ps[p].x, y = Ps[p].y;
             const double px = dom_mod(x + SEVC_GRAD_EPS, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
             double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
@@ -1579,15 +1579,24 @@ __global__ void k_t2_par(Dev d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int g = d.l_t2[k];
         if (d.debug && !d.ms[member_of_grain(d, g)].active) continue;   // stopped by a failed check
+        // the grains of the three vertices and their flags, in three rounds
+        // of independent loads
         const int* L = glist(d, g);
-        bool clash = false;
-        for (int q = 0; q < 3 && !clash; q++) {
-            int v = L[q];
-            for (int r = 0; r < 3; r++) {
-                int h = d.vg[3 * v + r];
-                if (h != g && h >= 0 && d.genab[h] && d.gcand[h]) { clash = true; break; }
-            }
+        int vv[3], h[9];
+        uint8_t he[9], hc[9];
+#pragma unroll
+        for (int q = 0; q < 3; q++) vv[q] = L[q];
+#pragma unroll
+        for (int e = 0; e < 9; e++) h[e] = d.vg[3 * vv[e / 3] + e % 3];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const bool other = h[e] != g && h[e] >= 0;
+            he[e] = other ? d.genab[h[e]] : 0;
+            hc[e] = other ? d.gcand[h[e]] : 0;
         }
+        bool clash = false;
+#pragma unroll
+        for (int e = 0; e < 9; e++) clash = clash || (he[e] && hc[e]);
         if (clash) {
             int s = atomicAdd(&d.cnt[C_T2C], 1);
             d.l_t2c[s] = g;
@@ -1670,8 +1679,8 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
     for (int k = 0; k < 3; k++) if (vB[k] == bnd) { j = (k + 1) % 3; BEb = vB[j]; break; }
     if (i < 0 || j < 0 || gic[1] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); return; }
     // reads that do not depend on the writes below, issued together
-    const int ac0 = d.bvx[2 * AC], ac1 = d.bvx[2 * AC + 1];
-    const int be0 = d.bvx[2 * BEb], be1 = d.bvx[2 * BEb + 1];
+    const int ac0 = d.bvx[2 * AC];
+    const int be0 = d.bvx[2 * BEb];
     const int c2[2] = {grain_contains_consecutive(d, gic[0], A, B), grain_contains_consecutive(d, gic[1], A, B)};
     const int c4[2] = {grain_contains_consecutive(d, gic[0], B, A), grain_contains_consecutive(d, gic[1], B, A)};
     const int h3[2] = {grain_contains_vertex(d, gnic[0], A), grain_contains_vertex(d, gnic[1], A)};
@@ -1682,8 +1691,8 @@ __device__ void flip(const Dev& d, int bnd, const Params& P) {
     if (ac0 == A) d.bvx[2 * AC] = B; else d.bvx[2 * AC + 1] = B;
     // BEb's ends as they are now (AC and BEb are distinct edges of a valid
     // mesh; if they coincide, the write above is seen)
-    int nb0 = be0, nb1 = be1;
-    if (BEb == AC) { nb0 = (ac0 == A) ? B : ac0; nb1 = (ac0 == A) ? ac1 : B; }
+    int nb0 = be0;
+    if (BEb == AC) nb0 = (ac0 == A) ? B : ac0;
     if (nb0 == B) d.bvx[2 * BEb] = A; else d.bvx[2 * BEb + 1] = A;
     int g1, g2, g3, g4;
     for (g2 = 0; g2 < 2; g2++)
@@ -1758,16 +1767,24 @@ __global__ void k_flip_par(Dev d) {
     const int n = d.cnt[C_CAND];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int c = d.l_cand[k];
-        if (!(d.benab[c] && d.bcand[c])) continue;
+        // the candidate's flags and ends, its ends' edges, their flags: three
+        // rounds of independent loads
+        const uint8_t ce = d.benab[c], cc = d.bcand[c];
+        const int2 ab = reinterpret_cast<const int2*>(d.bvx)[c];
+        if (!(ce && cc)) continue;
         if (d.debug && !d.ms[member_of_edge(d, c)].active) continue;
-        bool clash = false;
-        for (int e = 0; e < 2 && !clash; e++) {
-            int v = d.bvx[2 * c + e];
-            for (int q = 0; q < 3; q++) {
-                int x = d.vb[3 * v + q];
-                if (x != c && d.benab[x] && d.bcand[x]) { clash = true; break; }
-            }
+        int x[6];
+        uint8_t xe[6], xc[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) x[q] = d.vb[3 * (q < 3 ? ab.x : ab.y) + q % 3];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            xe[q] = x[q] != c ? d.benab[x[q]] : 0;
+            xc[q] = x[q] != c ? d.bcand[x[q]] : 0;
         }
+        bool clash = false;
+#pragma unroll
+        for (int q = 0; q < 6; q++) clash = clash || (xe[q] && xc[q]);
         if (clash) {
             int s = atomicAdd(&d.cnt[C_FLIPC], 1);
             d.l_flipc[s] = c;
@@ -2024,7 +2041,7 @@ struct RunScan {
     int total;      // wanted slots of the member
 };
 
-__device__ RunScan run_scan(const Blk* tbl, int first, int nblk, const int2* bc, int want, int* sh) {
+__device__ RunScan run_scan(int first, int nblk, const int2* bc, int want, int* sh) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int C = (nblk + nt - 1) / nt;
     RunScan r;
@@ -2287,7 +2304,7 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     __syncthreads();
     if (P.nucl_mode == SEVC_NUCL_MC) {
         const int first = d.vblk_first[m], nb = d.vblk_first[m + 1] - first;
-        const RunScan rs = run_scan(d.vblk, first, nb, d.bc_v, 1, sh);
+        const RunScan rs = run_scan(first, nb, d.bc_v, 1, sh);
         const int n_alive = rs.total;
         if (n_alive == 0) return;
         double u0, u1;
