@@ -1640,21 +1640,44 @@ __global__ void k_st2(Dev d) {
 // T1: flips (topological.h:174-291, 1016-1026)
 // ---------------------------------------------------------------------
 
+// What a flip reads first about its edge (A, B): the ends' edges, grains
+// and positions.  The parallel flips load it together with their conflict
+// check.
+struct FlipIn {
+    int A, B;
+    int vA[3], vB[3], gA[3], gB[3];
+    double pax, pay, pbx, pby;
+};
+
+__device__ __forceinline__ FlipIn flip_in(const Dev& d, int A, int B) {
+    FlipIn f;
+    f.A = A;
+    f.B = B;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        f.vA[k] = d.vb[3 * A + k]; f.vB[k] = d.vb[3 * B + k];
+        f.gA[k] = d.vg[3 * A + k]; f.gB[k] = d.vg[3 * B + k];
+    }
+    const double2 pa = reinterpret_cast<const double2*>(d.pos)[A], pb = reinterpret_cast<const double2*>(d.pos)[B];
+    f.pax = pa.x; f.pay = pa.y; f.pbx = pb.x; f.pby = pb.y;
+    return f;
+}
+
 // flip, topological.h:174-291.  Every datum the flip reads is loaded first
-// (the edge's ends; their edges, grains and positions; the ends of the
-// rewired edges; the lists of the four grains around; their orientations),
-// then the reference's sequence of tests and writes runs on local copies in
-// the same order and the writes go out.
-__device__ void flip(const Dev& d, int bnd, const Params& P) {
+// (the edge's ends; their edges, grains and positions, given in `in`; the
+// ends of the rewired edges; the lists of the four grains around; their
+// orientations), then the reference's sequence of tests and writes runs on
+// local copies in the same order and the writes go out.
+__device__ __forceinline__ void flip(const Dev& d, int bnd, const Params& P, const FlipIn& in) {
     const double D = P.D;
-    const int A = d.bvx[2 * bnd], B = d.bvx[2 * bnd + 1];
+    const int A = in.A, B = in.B;
     int vA[3], vB[3], gA[3], gB[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        vA[k] = d.vb[3 * A + k]; vB[k] = d.vb[3 * B + k];
-        gA[k] = d.vg[3 * A + k]; gB[k] = d.vg[3 * B + k];
+        vA[k] = in.vA[k]; vB[k] = in.vB[k];
+        gA[k] = in.gA[k]; gB[k] = in.gB[k];
     }
-    const double pax = d.pos[2 * A], pay = d.pos[2 * A + 1], pbx = d.pos[2 * B], pby = d.pos[2 * B + 1];
+    const double pax = in.pax, pay = in.pay, pbx = in.pbx, pby = in.pby;
     int gic[2], gnic[2];
     rg_intersect(gA, gB, gic);
     {
@@ -1767,16 +1790,18 @@ __global__ void k_flip_par(Dev d) {
     const int n = d.cnt[C_CAND];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int c = d.l_cand[k];
-        // the candidate's flags and ends, its ends' edges, their flags: three
+        // the candidate's flags and ends; its ends' edges, grains and
+        // positions (what the flip reads first); those edges' flags: three
         // rounds of independent loads
         const uint8_t ce = d.benab[c], cc = d.bcand[c];
         const int2 ab = reinterpret_cast<const int2*>(d.bvx)[c];
         if (!(ce && cc)) continue;
         if (d.debug && !d.ms[member_of_edge(d, c)].active) continue;
+        const FlipIn fi = flip_in(d, ab.x, ab.y);
         int x[6];
         uint8_t xe[6], xc[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) x[q] = d.vb[3 * (q < 3 ? ab.x : ab.y) + q % 3];
+        for (int q = 0; q < 6; q++) x[q] = q < 3 ? fi.vA[q] : fi.vB[q - 3];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
             xe[q] = x[q] != c ? d.benab[x[q]] : 0;
@@ -1790,7 +1815,7 @@ __global__ void k_flip_par(Dev d) {
             d.l_flipc[s] = c;
         } else {
             int m = member_of_edge(d, c);
-            flip(d, c, d.prm[m]);
+            flip(d, c, d.prm[m], fi);
             atomicAdd((unsigned long long*)&d.ms[m].flips, 1ull);
         }
     }
@@ -1806,7 +1831,7 @@ __global__ void k_flip_serial(Dev d) {
         if (!(d.benab[c] && d.bcand[c])) continue;
         int m = member_of_edge(d, c);
         if (d.debug && !d.ms[m].active) continue;
-        flip(d, c, d.prm[m]);
+        flip(d, c, d.prm[m], flip_in(d, d.bvx[2 * c], d.bvx[2 * c + 1]));
         d.ms[m].flips += 1;
     }
 }
