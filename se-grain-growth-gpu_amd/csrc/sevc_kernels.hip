@@ -2085,45 +2085,6 @@ __device__ RunScan run_scan(int first, int nblk, const int2* bc, int want, int* 
     return r;
 }
 
-// Exclusive prefix sums of three ints per thread over the workgroup, in one
-// pass (wg_exclusive_scan for each of them, sharing its two barriers).
-// `sh` holds >= 3 * 65 ints.
-__device__ __forceinline__ void wg_exclusive_scan3(const int x[3], int* sh, int pre[3], int tot[3]) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
-    int inc[3] = {x[0], x[1], x[2]};
-    for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const int v = __shfl_up(inc[t], o, 64);
-            if (lane >= o) inc[t] += v;
-        }
-    }
-    if (lane == 63)
-#pragma unroll
-        for (int t = 0; t < 3; t++) sh[65 * t + wave] = inc[t];
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const int v = lane < nw ? sh[65 * t + lane] : 0;
-            int vi = v;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int u = __shfl_up(vi, o, 64);
-                if (lane >= o) vi += u;
-            }
-            if (lane < nw) sh[65 * t + lane] = vi - v;
-            if (lane == nw - 1) sh[65 * t + 64] = vi;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        pre[t] = sh[65 * t + wave] + inc[t] - x[t];
-        tot[t] = sh[65 * t + 64];
-    }
-    __syncthreads();
-}
-
 // One kind of slot of a member to collect: its blocks [first, first+nblk)
 // of the block table, which cover slots [off, off+n) kBlock at a time, the
 // per-block (enabled, free) counts, the enable flags, how many free slots
@@ -2137,54 +2098,48 @@ struct SlotKind {
 };
 
 // The first R free slots (enable flag 0) of each of three kinds, in index
-// order, cooperatively in one workgroup: each lane takes one run of
-// consecutive blocks of every kind; a scan of the runs' free counts tells
-// each lane which ranks its run holds; lanes whose run holds wanted ranks
-// list (kind, block, first rank) for its blocks with free slots in `bl`
-// (LDS when the caller's bound on the entries fits there, else global
-// scratch); the waves then sweep the listed blocks, kSweep at a time, and
-// write each free slot at its rank.  got[t] = min(R_t, free slots of kind
-// t).  `sh` holds >= 3 * 65 ints; `bl` holds at least R_0 + R_1 + R_2
-// entries (a listed block holds a wanted rank).
+// order, cooperatively in one workgroup.  Each kind's blocks are read front
+// to back in rounds of 4 consecutive blocks per lane (coalesced free
+// counts); a workgroup scan gives each lane the first rank of its blocks,
+// lanes list (kind, block, first rank) for their blocks with free slots and
+// a wanted rank in `bl` (LDS when the caller's bound on the entries fits
+// there, else global scratch), and the rounds stop once R ranks are found,
+// which after T2 removals have left holes is usually the first round.  The
+// waves then sweep the listed blocks, kSweep at a time, and write each free
+// slot at its rank.  got[t] = min(R_t, free slots of kind t).  `sh` holds
+// >= 65 ints; `bl` holds at least R_0 + R_1 + R_2 entries (a listed block
+// holds a wanted rank).
 __device__ __forceinline__ void collect_free3(const SlotKind K0, const SlotKind K1, const SlotKind K2, int* got,
                                               int* sh, int4* bl) {
     const SlotKind K[3] = {K0, K1, K2};    // by value: kept in registers
     __shared__ int nl;
     const int tid = threadIdx.x, nt = blockDim.x;
-    int lo[3], hi[3], c[3];
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        const int C = (K[t].nblk + nt - 1) / nt;
-        lo[t] = K[t].first + min(K[t].nblk, tid * C);
-        hi[t] = K[t].first + min(K[t].nblk, (tid + 1) * C);
-        int n = 0;
-        for (int b = lo[t]; b < hi[t]; b += 8) {     // 8 independent loads per round
-            int v[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = b + q < hi[t] ? K[t].bc[b + q].y : 0;
-#pragma unroll
-            for (int q = 0; q < 8; q++) n += v[q];
-        }
-        c[t] = n;
-    }
+    constexpr int kPer = 4;                 // consecutive blocks per lane and round
     if (tid == 0) nl = 0;
-    int pre[3], tot[3];
-    wg_exclusive_scan3(c, sh, pre, tot);
+    int tot[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-        if (c[t] == 0 || pre[t] >= K[t].R) continue;
-        int rank = pre[t];
-        for (int b0 = lo[t]; b0 < hi[t] && rank < K[t].R; b0 += 8) {
-            int v[8];                               // 8 independent loads, then in order
+        const int first = K[t].first, nblk = K[t].nblk, R = K[t].R;
+        const int2* bc = K[t].bc;
+        int off = 0;                        // free slots in the blocks of earlier rounds
+        for (int r0 = 0; r0 < nblk && off < R; r0 += kPer * nt) {   // the same for every lane
+            const int l0 = r0 + kPer * tid;                          // this lane's blocks [l0, l0 + kPer)
+            int v[kPer], c = 0;
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = (b0 + q < hi[t]) ? K[t].bc[b0 + q].y : 0;
+            for (int q = 0; q < kPer; q++) v[q] = l0 + q < nblk ? bc[first + l0 + q].y : 0;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                if (v[q] > 0 && rank < K[t].R) {
-                    bl[atomicAdd(&nl, 1)] = make_int4(t, b0 + q, rank, 0);
+            for (int q = 0; q < kPer; q++) c += v[q];
+            int total;
+            int rank = off + wg_exclusive_scan(c, sh, &total);
+#pragma unroll
+            for (int q = 0; q < kPer; q++)
+                if (v[q] > 0 && rank < R) {
+                    bl[atomicAdd(&nl, 1)] = make_int4(t, first + l0 + q, rank, 0);
                     rank += v[q];
                 }
+            off += total;
         }
+        tot[t] = off;
     }
     __syncthreads();
     const int n = nl;
