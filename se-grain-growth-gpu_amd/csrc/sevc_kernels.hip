@@ -642,10 +642,19 @@ __global__ void k_label(Dev d) {
     bool c = is_candidate(te, len, dt, P.flip_thr);
     d.bcand[i] = c ? 1 : 0;
     d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge (bit 2: k_poll)
-    if (c && en) {
-        int k = atomicAdd(&d.cnt[C_CAND], 1);
-        d.l_cand[k] = i;
-        d.conf[i] = k;              // list position, for k_poll_gather
+    // the wave's candidates take consecutive list positions with one atomic
+    const bool cand = c && en;
+    const unsigned long long bal = __ballot(cand);
+    if (bal) {
+        const int lane = threadIdx.x & 63, leader = __ffsll((long long)bal) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&d.cnt[C_CAND], (int)__popcll(bal));
+        base = __shfl(base, leader, 64);
+        if (cand) {
+            const int k = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+            d.l_cand[k] = i;
+            d.conf[i] = k;          // list position, for k_poll_gather
+        }
     }
 }
 
