@@ -452,3 +452,19 @@ def test_batch_mc_member_after_rate_member():
                 nucl += s1.n_nucleations
     st.close()
     assert nucl >= 20, nucl
+
+
+def test_full_size_two_steps():
+    """The benchmark workload itself (periodic Voronoi of 10^6 seeds,
+    lognormal SE, rate nucleation, adaptive dt, flip threshold 1e-4): two
+    steps bit for bit against the oracle, with flips, removals and
+    nucleations in both.  In these first steps the free slots lie only at the
+    ends of the arrays, so the free-slot search runs over several rounds of
+    blocks (3M edge slots)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    o, mesh = bench.make_mesh(sevc, 1000000, 1)
+    tot = _run_pair(o, mesh, [1, 1])
+    assert tot["flips"] > 50 and tot["removals"] > 10 and tot["nucl"] > 20, tot
