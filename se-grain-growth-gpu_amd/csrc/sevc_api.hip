@@ -620,8 +620,8 @@ int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], dou
 
 static int finish_call(sevc_state* s) {
     Dev d = s->d;
-    launch(s, "nucl_area_l1", k_nucl_area_l1, (d.ngb + 255) / 256, 256, d);
-    launch(s, "nucl_area_up", k_nucl_area_up, s->B, 256, d);
+    launch(s, "nucl_area_l1", k_nucl_area_l1, d.ngb, 64, d);
+    launch(s, "nucl_area_up", k_nucl_area_up, s->B, 1024, d);
     s->ms_prev = s->ms_host;
     HIPCHK(hipMemcpyAsync(s->ms_host.data(), d.ms, s->ms_host.size() * sizeof(MState), hipMemcpyDeviceToHost,
                           s->stream));

@@ -1849,45 +1849,78 @@ __global__ void k_flip_serial(Dev d) {
 // deterministic sums (nucleated area, main.cu:321)
 // ---------------------------------------------------------------------
 
-// Level 1: one lane per 256-grain table block sums its nucleated areas
-// left to right (the chunking of the oracle's sum).
-__global__ void k_nucl_area_l1(Dev d) {
-    int lb = blockIdx.x * blockDim.x + threadIdx.x;
+// Level 1: one wave per 256-grain table block.  The lanes load the block's
+// flags and areas (four coalesced rounds) into LDS, then lane 0 sums the
+// nucleated areas left to right (the chunking of the oracle's sum).
+__global__ void __launch_bounds__(64) k_nucl_area_l1(Dev d) {
+    __shared__ double a_s[kBlock];
+    __shared__ uint8_t ok_s[kBlock];
+    const int lb = blockIdx.x;
     if (lb >= d.ngb) return;
-    Blk bd = d.gblk[lb];
-    double s = 0.0;
-    for (int g0 = bd.b; g0 < bd.e; g0 += 8) {       // 8 grains' loads at a time, summed in order
-        double a[8];
-        bool ok[8];
+    const Blk bd = d.gblk[lb];
+    const int lane = threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int g = g0 + q;
-            ok[q] = g < bd.e && d.genab[g] && d.gtype[g] == T_NUCLEATED;
-            a[q] = ok[q] ? d.garea[g] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (ok[q]) s += a[q];
+    for (int q = 0; q < kBlock / 64; q++) {
+        const int g = bd.b + 64 * q + lane;
+        const bool ok = g < bd.e && d.genab[g] && d.gtype[g] == T_NUCLEATED;
+        ok_s[64 * q + lane] = ok ? 1 : 0;
+        a_s[64 * q + lane] = ok ? d.garea[g] : 0.0;
     }
-    d.partial[lb] = s;
+    __syncthreads();
+    if (lane == 0) {
+        double s = 0.0;
+        const int n = bd.e - bd.b;
+        for (int i = 0; i < n; i++)
+            if (ok_s[i]) s += a_s[i];
+        d.partial[lb] = s;
+    }
 }
 
 // Upper levels for each member: chunks of 256 partials left to right,
-// repeated until one value remains.  One workgroup per member; levels
-// alternate between the two halves of `partial` so no lane overwrites a
-// value another lane still reads.
-__global__ void k_nucl_area_up(Dev d) {
+// repeated until one value remains.  One workgroup per member; a level of
+// up to kAreaLds partials is first copied to LDS with coalesced loads and
+// each lane then sums its chunk there; levels alternate between the two
+// halves of `partial` so no lane overwrites a value another lane still
+// reads.
+constexpr int kAreaLds = 8192;
+__global__ void __launch_bounds__(1024) k_nucl_area_up(Dev d) {
+    __shared__ double v_s[kAreaLds];
     int m = blockIdx.x;
     if (m >= d.B) return;
     int first = d.gblk_first[m], cnt = d.gblk_first[m + 1] - first;
     double* src = d.partial + first;
     double* dst = d.partial + d.ngb + first;
     while (cnt > 1) {
-        int nn = (cnt + 255) / 256;
+        const int nn = (cnt + 255) / 256;
+        const bool lds = cnt <= kAreaLds;                // the same for the whole workgroup
+        if (lds) {
+            for (int i0 = threadIdx.x; i0 < cnt; i0 += 4 * blockDim.x) {   // 4 loads per lane and round
+                double t[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = i0 + q * (int)blockDim.x;
+                    t[q] = i < cnt ? src[i] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = i0 + q * (int)blockDim.x;
+                    if (i < cnt) v_s[i] = t[q];
+                }
+            }
+            __syncthreads();
+        }
+        const double* from = lds ? v_s : src;
         for (int c = threadIdx.x; c < nn; c += blockDim.x) {
             double s = 0.0;
             int e = (c + 1) * 256 < cnt ? (c + 1) * 256 : cnt;
-            for (int q = c * 256; q < e; q++) s += src[q];
+            for (int q0 = c * 256; q0 < e; q0 += 8) {    // 8 values read ahead, added in order
+                double t[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) t[q] = q0 + q < e ? from[q0 + q] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    if (q0 + q < e) s += t[q];
+            }
             dst[c] = s;
         }
         __syncthreads();
