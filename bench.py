@@ -368,8 +368,9 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how the step would shard.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only: there is no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Ensembles would shard by whole members, one process per GPU; a single domain by spatial strips "
+              "with a halo exchange per step.", file=sys.stderr)
         sys.exit(2)
 
     import sevc

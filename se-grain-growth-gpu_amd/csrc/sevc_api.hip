@@ -237,7 +237,7 @@ void launch_check(sevc_state* s, int phase) {
     const Dev& d = s->d;
     launch(s, "check", k_check_clear, 1, 256, d.chk_bad, s->B);
     launch(s, "check", k_check, check_grid((long long)d.V + d.E + d.G), 256, check_view(d, 1));
-    launch(s, "check", k_check_verdict, 1, 256, d.prm, d.ms, static_cast<const int*>(d.chk_bad), s->B, phase);
+    launch(s, "check", k_check_stop, 1, 256, d.prm, d.ms, static_cast<const int*>(d.chk_bad), s->B, phase);
 }
 
 // One time step of every active member (main.cu:124-389).

@@ -27,6 +27,6 @@ enum { CHK_NUCLEATION = 1, CHK_TWO_GRAIN = 2, CHK_REMOVAL = 3, CHK_FLIP = 4 };
 
 __global__ void k_check_clear(int* bad, int B);
 __global__ void k_check(ChkView c);
-__global__ void k_check_verdict(const Params* prm, MState* ms, const int* bad, int B, int phase);
+__global__ void k_check_stop(const Params* prm, MState* ms, const int* bad, int B, int phase);
 
 }  // namespace sevc

@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) k_check(ChkView c) {
 // violation stops here, as the reference leaves its loop (main.cu:192-195
 // and the like): status SEVC_E_CHECK, the phase recorded, inactive for
 // the rest of the step.
-__global__ void k_check_verdict(const Params* prm, MState* ms, const int* bad, int B, int phase) {
+__global__ void k_check_stop(const Params* prm, MState* ms, const int* bad, int B, int phase) {
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < B; m += gridDim.x * blockDim.x) {
         MState& M = ms[m];
         if (prm[m].debug < 1 || !M.active || bad[m] == 0) continue;
