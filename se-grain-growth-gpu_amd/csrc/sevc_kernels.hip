@@ -1941,6 +1941,7 @@ __global__ void __launch_bounds__(1024) k_nucl_area_up(Dev d) {
 
 struct NuclGeom {
     double Apos[2], Bpos[2], Cpos[2];
+    double Dpos[2], Epos[2], Fpos[2];   // positions of D, E, F as read
     int D, E, F, AD, AE, AF, g1, g2, g3;
     double dE;
 };
@@ -1967,6 +1968,9 @@ __device__ __forceinline__ int generate_grain(const Dev& d, const Params& P, int
     const double ax = d.pos[2 * A], ay = d.pos[2 * A + 1], rr = d.nucl_r[A];
     const int nb[3] = {r->D, r->E, r->F};
     double* out[3] = {r->Apos, r->Bpos, r->Cpos};
+    r->Dpos[0] = d.pos[2 * nb[0]]; r->Dpos[1] = d.pos[2 * nb[0] + 1];
+    r->Epos[0] = d.pos[2 * nb[1]]; r->Epos[1] = d.pos[2 * nb[1] + 1];
+    r->Fpos[0] = d.pos[2 * nb[2]]; r->Fpos[1] = d.pos[2 * nb[2] + 1];
     for (int k = 0; k < 3; k++) {
         double tx = wrapd(d.pos[2 * nb[k]], ax, Dm);
         double ty = wrapd(d.pos[2 * nb[k] + 1], ay, Dm);
@@ -2566,6 +2570,12 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
             const int* L = glist(d, gl[t]);
             for (int i = lane; i < n_old[t]; i += 64) Ls[t][i] = L[i];
         }
+        // what the serial edits below read, loaded with the lists (nothing
+        // writes these entries before they are read)
+        int vgA[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) vgA[k] = d.vg[3 * A + k];
+        const int ae0 = d.bvx[2 * r.AE], ae1 = d.bvx[2 * r.AE + 1], af0 = d.bvx[2 * r.AF];
         if (lane == 0) { n_s[0] = n_old[0]; n_s[1] = n_old[1]; n_s[2] = n_old[2]; }
         __syncthreads();
         if (lane == 0) {
@@ -2631,19 +2641,18 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
             d.gfix[g] = 0;
             d.gtype[g] = T_NUCLEATED;
             d.gori[g] = d.nucl_ori[A];
-            for (int k = 0; k < 3; k++) if (d.vg[3 * A + k] == r.g3) { d.vg[3 * A + k] = g; break; }
+            for (int k = 0; k < 3; k++) if (vgA[k] == r.g3) { d.vg[3 * A + k] = g; break; }
             d.vg[3 * B] = g; d.vg[3 * B + 1] = r.g2; d.vg[3 * B + 2] = r.g3;
             d.vg[3 * C] = g; d.vg[3 * C + 1] = r.g3; d.vg[3 * C + 2] = r.g1;
             ladd(sl[0], C, A);
-            const int e0 = d.bvx[2 * r.AE], e1 = d.bvx[2 * r.AE + 1];
-            if (e0 == A) ladd(sl[1], B, e1);
-            else if (e1 == A) ladd(sl[1], B, e0);
+            if (ae0 == A) ladd(sl[1], B, ae1);
+            else if (ae1 == A) ladd(sl[1], B, ae0);
             ladd(sl[2], C, A);
             ladd(sl[2], B, C);
             lremove(sl[2], A);
-            if (e0 == A) d.bvx[2 * r.AE] = B;
-            else if (e1 == A) d.bvx[2 * r.AE + 1] = B;
-            if (d.bvx[2 * r.AF] == A) d.bvx[2 * r.AF] = C;
+            if (ae0 == A) d.bvx[2 * r.AE] = B;
+            else if (ae1 == A) d.bvx[2 * r.AE + 1] = B;
+            if (af0 == A) d.bvx[2 * r.AF] = C;
             else d.bvx[2 * r.AF + 1] = C;
             atomicAdd((unsigned long long*)&d.ms[m].nucl, 1ull);
         }
@@ -2668,13 +2677,21 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
         // order by lanes 0-3 as grain_area does
         const double Dm = P.D;
         if (lane < 6) {
+            // the six edges and their ends' positions after the event: AB, BC,
+            // CA between the new vertices, AD at A, AE rewired to B, AF to C
+            // (arclen gives the same bits in either end order)
             const int b = lane == 0 ? AB : lane == 1 ? BC : lane == 2 ? CA : lane == 3 ? r.AD : lane == 4 ? r.AE : r.AF;
-            if (d.benab[b]) {
-                const int a0 = d.bvx[2 * b], a1 = d.bvx[2 * b + 1];
-                d.blen[b] = arclen(d.pos[2 * a0], d.pos[2 * a0 + 1], d.pos[2 * a1], d.pos[2 * a1 + 1], Dm);
-                const double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
-                d.ben[b] = e * P.gb_scaling;
-            }
+            // by selection (no pointer into r, which would put it in scratch)
+            const bool a0 = lane == 0 || lane == 3, b0 = lane == 1 || lane == 4;
+            const double x0 = a0 ? r.Apos[0] : (b0 ? r.Bpos[0] : r.Cpos[0]);
+            const double y0 = a0 ? r.Apos[1] : (b0 ? r.Bpos[1] : r.Cpos[1]);
+            const double x1 = lane == 0 ? r.Bpos[0] : lane == 1 ? r.Cpos[0] : lane == 2 ? r.Apos[0]
+                            : lane == 3 ? r.Dpos[0] : lane == 4 ? r.Epos[0] : r.Fpos[0];
+            const double y1 = lane == 0 ? r.Bpos[1] : lane == 1 ? r.Cpos[1] : lane == 2 ? r.Apos[1]
+                            : lane == 3 ? r.Dpos[1] : lane == 4 ? r.Epos[1] : r.Fpos[1];
+            d.blen[b] = arclen(x0, y0, x1, y1, Dm);
+            const double e = (P.gb_eps == 0.0) ? 1.0 : boundary_compute_energy(d, b, P.gb_eps);
+            d.ben[b] = e * P.gb_scaling;
         }
 #pragma unroll
         for (int t = 0; t < 3; t++) {
