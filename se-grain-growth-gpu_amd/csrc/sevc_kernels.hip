@@ -1022,12 +1022,17 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
     const int lane = threadIdx.x;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int g = d.l_fix[stage][item];
-        if (!d.genab[g]) continue;
+        // the grain's flag, length and whole list row in one round of loads
         int* L = glist(d, g);
+        const uint8_t en = d.genab[g];
         const int n0 = d.glen[g];
+        int row[kMaxStride / 64];
+#pragma unroll
+        for (int q = 0; q < kMaxStride / 64; q++) row[q] = lane + 64 * q < d.S ? L[lane + 64 * q] : -1;
+        if (!en) continue;
         const int cap = d.prm[member_of_grain(d, g)].cap;
         for (int i = lane; i < n0; i += 64) {
-            const int v = L[i];
+            const int v = row[i >> 6];
             Ls[i] = v;
             src[i] = (short)i;
             unsigned kd = 0;
