@@ -107,6 +107,16 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
     if bad:
         raise SystemExit(f"ensemble members {bad[:8]} failed (status {per[bad[0]].status})")
     batched = sum(p.n_vertex_steps for p in per) / (t2 - t1)
+    # the same batch replayed as a captured HIP graph of one step
+    gsteps = max(1, steps // 5)
+    st.step(1, graph=True)
+    g1 = time.perf_counter()
+    sg = st.step(gsteps, graph=True)
+    g2 = time.perf_counter()
+    perg = [st.member_stats(k) for k in range(members)]
+    if any(p.status != 0 for p in perg):
+        raise SystemExit("ensemble graph replay: a member failed")
+    graph_rate = sum(p.n_vertex_steps for p in perg) / (g2 - g1)
     st.close()
     vs, el = 0, 0.0
     for o, m in zip(opts, meshes):
@@ -119,11 +129,13 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
         vs += r.n_vertex_steps
         one.close()
     b2b = vs / el
-    log(f"ensemble: batched {batched:.3e} vertex-steps/s over {steps} steps; back-to-back {b2b:.3e} "
-        f"over {b2b_steps} steps per member")
+    log(f"ensemble: batched {batched:.3e} vertex-steps/s over {steps} steps (graph replay {graph_rate:.3e}); "
+        f"back-to-back {b2b:.3e} over {b2b_steps} steps per member")
     return {"value": batched, "unit": "vertex-steps/s", "members": members, "grains": grains, "steps": steps,
             "tile_grains": tile,
             "ms_per_step": (t2 - t1) / max(1, s.steps_done) * 1e3,
+            "graph_replay": {"value": graph_rate, "steps": gsteps,
+                             "ms_per_step": (g2 - g1) / max(1, sg.steps_done) * 1e3},
             "back_to_back": {"value": b2b, "steps_per_member": b2b_steps, "seconds": el},
             "speedup": batched / b2b,
             "workload": f"{members} members, G log-uniform in [1000, 50000] (Voronoi seed 1000+i), "
