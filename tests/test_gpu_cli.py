@@ -1,7 +1,7 @@
 """The command-line driver (`sevc -c config.ini`, in place of main.cu) on
 the GPU: snapshots at the reference's schedule (main.cu:328), in the
 export_data format, whose vertex coordinates equal the oracle's after the
-same steps."""
+same steps, with and without the in-step checks of debug_lvl = 1."""
 import os
 import subprocess
 
@@ -14,7 +14,8 @@ from _util import REF2K, Oracle
 pytestmark = pytest.mark.gpu
 
 
-def test_cli_snapshots_match_oracle(tmp_path):
+@pytest.mark.parametrize("debug_lvl", [0, 1])
+def test_cli_snapshots_match_oracle(tmp_path, debug_lvl):
     cli = sevc.CLI_PATH
     assert os.path.exists(cli), "CLI not built"
     out = tmp_path / "run"
@@ -25,6 +26,7 @@ def test_cli_snapshots_match_oracle(tmp_path):
         f"vertex_fname={REF2K}/vertices.txt", f"boundary_fname={REF2K}/ridges.txt",
         f"orientation_fname={REF2K}/oris.txt", f"SE_fname={REF2K}/SE/uniform.txt",
         "GB_eps=0.0", "GB_scaling=1.0", "SE_eps=0.5", "domain_bound=1.0", "do_nucleation=false",
+        f"debug_lvl={debug_lvl}",
         "[Output]", f"folder={out}", ""]))
     print("  running the CLI for 61 steps", flush=True)
     r = subprocess.run([cli, "-c", str(ini)], capture_output=True, text=True, timeout=120)
