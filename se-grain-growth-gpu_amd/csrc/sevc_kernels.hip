@@ -326,7 +326,10 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             }
             const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
             const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
-            if (tid < cnt) off[tid] = inc - n;
+            if (tid < cnt) {
+                off[tid] = inc - n;
+                for (int q = inc - n; q < inc; q++) pgl[q] = (unsigned char)tid;   // the grain of each point
+            }
             if (tid == cnt - 1) off[cnt] = inc;
             if (tid == 0) {
                 gend_s = gs + cnt;
@@ -335,9 +338,6 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
         }
         __syncthreads();
         const int gend = gend_s, ngr = gend - gs, npts = off[ngr];
-        for (int g = tid; g < ngr; g += nt)
-            for (int q = off[g]; q < off[g + 1]; q++) pgl[q] = (unsigned char)g;
-        __syncthreads();
         for (int p = tid; p < npts; p += nt) {
             const int g = pgl[p];
             const int w = glist(d, gs + g)[p - off[g]];
