@@ -226,7 +226,9 @@ int sevc_check_state(sevc_state* s, int32_t* bad);
 int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad);
 
 /* Snapshot in the format of export_data (io.h:280-377) to
- * <folder>/<index as %06d>.txt. */
+ * <folder>/<index as %06d>.txt.  The format holds 100 vertices per grain
+ * (MAX_VRT_PER_GRN, geometry.h:9): a grain with a longer list returns
+ * SEVC_E_GRAIN_FULL and no file is written. */
 int sevc_export_snapshot(const sevc_mesh* m, const sevc_options* o, const char* folder,
                          int32_t index, double time);
 
@@ -248,13 +250,20 @@ int sevc_load_snapshot(const char* path, int32_t max_vrt_per_grn, sevc_mesh* out
 int sevc_create(const sevc_options* o, const sevc_mesh* m, sevc_state** out);
 
 /* Upload n_members independent meshes as one batched state: every kernel
- * of a step covers all members in one launch. */
+ * of a step covers all members in one launch.  velocity_model and
+ * nucleation_mode must be known values and domain_bound positive
+ * (SEVC_E_INVALID_ARG, checked before any device call). */
 int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t n_members,
                       sevc_state** out);
 
 /* Run up to n_steps steps of main.cu:124-389 on every active member;
  * a member stops at its own stop rule (main.cu:120,391).  stats may be
- * NULL; for batches it receives member 0's statistics. */
+ * NULL; for batches it receives member 0's statistics.  An event that meets
+ * an inconsistent topology or a full grain list stops only its own member
+ * (status SEVC_E_STATE / SEVC_E_GRAIN_FULL in sevc_member_stats); the call
+ * in which a member stops so returns that code and names the member, the
+ * other members keep stepping.  A one-member state returns the code again
+ * on every later call. */
 int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* stats);
 
 /* Same, capturing one step into a HIP graph on first use and replaying

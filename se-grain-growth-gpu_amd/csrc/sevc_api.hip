@@ -347,6 +347,15 @@ int32_t sevc_n_members(const sevc_state* s) { return s ? s->B : 0; }
 int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t B, sevc_state** out) {
     if (!opts || !meshes || B <= 0 || !out) return SEVC_E_INVALID_ARG;
     *out = nullptr;
+    // enumerated options and the domain: an unknown velocity model would
+    // leave the velocities unwritten, an unknown nucleation mode would run
+    // as another one
+    for (int m = 0; m < B; ++m) {
+        const sevc_options& o = opts[m];
+        if (o.velocity_model != SEVC_VEL_FD && o.velocity_model != SEVC_VEL_ANALYTIC) return SEVC_E_INVALID_ARG;
+        if (o.nucleation_mode != SEVC_NUCL_MC && o.nucleation_mode != SEVC_NUCL_RATE) return SEVC_E_INVALID_ARG;
+        if (!(o.domain_bound > 0) || !(meshes[m].domain_bound > 0)) return SEVC_E_INVALID_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SEVC_E_NO_DEVICE;
     sevc_state* s = new sevc_state();
@@ -627,9 +636,15 @@ static int finish_call(sevc_state* s) {
                           s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->timing) collect_timers(s);
-    int err = 0;
-    HIPCHK(hipMemcpy(&err, d.cnt + C_ERR, sizeof(int), hipMemcpyDeviceToHost));
-    if (err != 0) return fail(s, err, std::string("device event error: ") + sevc_strerror(err));
+    // a member whose event met an inconsistent state or a full grain list
+    // during this call stops alone; the call reports the first such member
+    for (int m = 0; m < s->B; ++m) {
+        const MState& a = s->ms_host[static_cast<size_t>(m)];
+        const int was = s->ms_prev[static_cast<size_t>(m)].status;
+        if (a.status != 0 && a.status != SEVC_E_CHECK && (was == 0 || s->B == 1))
+            return fail(s, a.status, std::string("device event error: ") + sevc_strerror(a.status) +
+                                         " (member " + std::to_string(m) + ", step " + std::to_string(a.n) + ")");
+    }
     // a member whose check failed during this call (debug_lvl >= 1): the
     // reference's "Error during ..." lines (main.cu:193, 245, 277, 305)
     static const char* const kPhase[] = {"", "nucleation", "remove two grains configuration", "Grain Removal",

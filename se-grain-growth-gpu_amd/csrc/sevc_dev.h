@@ -26,8 +26,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-__device__ __forceinline__ void set_error(const Dev& d, int code) {
-    atomicCAS(&d.cnt[C_ERR], 0, code);
+// An event of member m met an inconsistent state or a full grain list: the
+// member records the first such code and stops (its later event entries of
+// the step are skipped, as after a failed check); the other members of a
+// batch go on.
+__device__ __forceinline__ void set_error(const Dev& d, int m, int code) {
+    MState& M = d.ms[m];
+    if (atomicCAS(&M.status, 0, code) == 0) {
+        M.active = 0;
+        M.done = 1;
+    }
 }
 
 __device__ __forceinline__ int other_end(const Dev& d, int b, int v) {

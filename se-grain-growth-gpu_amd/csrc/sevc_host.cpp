@@ -419,10 +419,14 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area) {
 
 // export_data, io.h:280-377: same header lines, tables and number formats.
 // The grain table always has SEVC_REF_MAX_VRT_PER_GRN vertex columns padded
-// with -1, as the reference's fixed-size grain struct does.
+// with -1, as the reference's fixed-size grain struct does; a grain with a
+// longer list (possible with max_vrt_per_grn above 100) does not fit the
+// format and is an error, nothing is written.
 int sevc_export_snapshot(const sevc_mesh* m, const sevc_options* o, const char* folder,
                          int32_t index, double time) {
     if (!m || !o || !folder) return SEVC_E_INVALID_ARG;
+    for (int i = 0; i < m->n_grains; ++i)
+        if (m->g_vlen[i] > SEVC_REF_MAX_VRT_PER_GRN || m->g_vlen[i] > m->max_vrt_per_grn) return SEVC_E_GRAIN_FULL;
     char fname[2 * SEVC_FNAME + 16];
     std::snprintf(fname, sizeof fname, "%s/%06d.txt", folder, index);
     FILE* f = std::fopen(fname, "w");

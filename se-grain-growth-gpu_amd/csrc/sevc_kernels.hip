@@ -33,7 +33,7 @@ namespace sevc {
 __global__ void __launch_bounds__(1024) k_begin(Dev d) {
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0)
-        for (int c = 0; c < C_COUNT; c++) if (c != C_ERR) d.cnt[c] = 0;
+        for (int c = 0; c < C_COUNT; c++) d.cnt[c] = 0;
     for (int m = tid; m < d.B; m += nt) {
         const Params& P = d.prm[m];
         MState& M = d.ms[m];
@@ -158,7 +158,6 @@ __global__ void __launch_bounds__(1024) k_member_reduce(Dev d) {
             M.steps += 1;
             if (P.adaptive) M.t += M.dt;
             else M.t = M.n * P.dt_max;
-            if (d.cnt[C_ERR] != 0 && M.status == 0) M.status = d.cnt[C_ERR];
             if (M.status != 0) M.done = 1;
             if (M.curr_grains <= P.min_grains && P.min_grains > 0) M.done = 1;
         }
@@ -934,8 +933,8 @@ __device__ __forceinline__ void list_add_after(const Dev& d, int g, int v, int p
     int* L = glist(d, g);
     int n = d.glen[g], pos = -1;
     for (int i = 0; i < n; i++) if (L[i] == prev) { pos = i + 1; break; }
-    if (pos < 0) { set_error(d, SEVC_E_STATE); return; }
-    if (n >= d.prm[member_of_grain(d, g)].cap) { set_error(d, SEVC_E_GRAIN_FULL); return; }
+    if (pos < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); return; }
+    if (n >= d.prm[member_of_grain(d, g)].cap) { set_error(d, member_of_grain(d, g), SEVC_E_GRAIN_FULL); return; }
     for (int i = n; i > pos; i--) L[i] = L[i - 1];
     L[pos] = v;
     d.glen[g] = n + 1;
@@ -1073,8 +1072,8 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
                     if (f != FIX_ADD || gg != g) continue;
                     int pos = -1;                                    // grain_add_vertex, geometry.h:629-645
                     for (int q = 0; q < n; q++) if (Ls[q] == pp) { pos = q + 1; break; }
-                    if (pos < 0) { set_error(d, SEVC_E_STATE); continue; }
-                    if (n >= cap) { set_error(d, SEVC_E_GRAIN_FULL); continue; }
+                    if (pos < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); continue; }
+                    if (n >= cap) { set_error(d, member_of_grain(d, g), SEVC_E_GRAIN_FULL); continue; }
                     for (int q = n; q > pos; q--) { Ls[q] = Ls[q - 1]; src[q] = src[q - 1]; }
                     Ls[pos] = vv;
                     src[pos] = -1;
@@ -1189,12 +1188,12 @@ __device__ int remove_two_grains(const Dev& d, int bnd, const int gnic[2]) {
     int AC = -1, BD = -1, CE = -1, DF = -1;
     for (int i = 0; i < 3; i++) if (d.vb[3 * A + i] == bnd) { AC = d.vb[3 * A + (i + 1) % 3]; break; }
     for (int j = 0; j < 3; j++) if (d.vb[3 * B + j] == bnd) { BD = d.vb[3 * B + (j + 1) % 3]; break; }
-    if (AC < 0 || BD < 0) { set_error(d, SEVC_E_STATE); return -1; }
+    if (AC < 0 || BD < 0) { set_error(d, member_of_edge(d, bnd), SEVC_E_STATE); return -1; }
     int C = other_end(d, AC, A);
     int Dv = other_end(d, BD, B);
     for (int k = 0; k < 3; k++) if (d.vb[3 * C + k] == AC) { CE = d.vb[3 * C + (k + 2) % 3]; break; }
     for (int l = 0; l < 3; l++) if (d.vb[3 * Dv + l] == BD) { DF = d.vb[3 * Dv + (l + 2) % 3]; break; }
-    if (CE < 0 || DF < 0) { set_error(d, SEVC_E_STATE); return -1; }
+    if (CE < 0 || DF < 0) { set_error(d, member_of_edge(d, bnd), SEVC_E_STATE); return -1; }
     int F = other_end(d, DF, Dv);
     if (d.bvx[2 * CE] == C) d.bvx[2 * CE] = F;
     else d.bvx[2 * CE + 1] = F;
@@ -1246,7 +1245,7 @@ __global__ void k_two_grain(Dev d) {
             int gic[2], gnic[2];
             grains_intersect(d, d.bvx[2 * e], d.bvx[2 * e + 1], gic);
             grains_symdiff(d, d.bvx[2 * e], d.bvx[2 * e + 1], gnic);
-            if (gnic[0] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); continue; }
+            if (gnic[0] < 0 || gnic[1] < 0) { set_error(d, member_of_edge(d, e), SEVC_E_STATE); continue; }
             int CE = remove_two_grains(d, e, gnic);
             d.genab[gic[0]] = 0;
             d.genab[gic[1]] = 0;
@@ -1274,7 +1273,7 @@ __global__ void k_two_grain(Dev d) {
             d.gfix[g] = 0;
             int s = atomicAdd(&d.cnt[C_DIRTY], 1);
             if (s < d.G) d.l_dirty[s] = g;
-            else set_error(d, SEVC_E_STATE);
+            else set_error(d, member_of_grain(d, g), SEVC_E_STATE);
             if (!d.genab[g]) continue;
             const int* L = glist(d, g);
             for (int i = 0; i < d.glen[g]; i++) reset_mods(d, L[i]);
@@ -1522,7 +1521,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     rg_intersect(gB, gC, gic); G3 = (gic[0] == g) ? gic[1] : gic[0];
     rg_intersect(gA, gB, gic); G2 = (gic[0] == g) ? gic[1] : gic[0];
     rg_intersect(gA, gC, gic); G1 = (gic[0] == g) ? gic[1] : gic[0];
-    if (G1 < 0 || G2 < 0 || G3 < 0) { set_error(d, SEVC_E_STATE); return; }
+    if (G1 < 0 || G2 < 0 || G3 < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); return; }
     d.vel[2 * A] = 0.0;
     d.vel[2 * A + 1] = 0.0;
     for (int i = 0; i < 3; i++) {
@@ -1538,7 +1537,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
         const int ei = xC[i][0], ee = xC[i][1];
         if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = vC[i]; iCF = i; break; }
     }
-    if (BEb < 0 || CF < 0) { set_error(d, SEVC_E_STATE); return; }
+    if (BEb < 0 || CF < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); return; }
     // the ends of BEb and CF after the rewiring (CF's as seen after BEb's,
     // should they be the same edge).  Only the end that changes is written:
     // the other end of BEb or CF can be another removal's vertex, which that
@@ -1592,7 +1591,7 @@ __global__ void k_t2_par(Dev d) {
     const int n = d.cnt[C_T2];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int g = d.l_t2[k];
-        if (d.debug && !d.ms[member_of_grain(d, g)].active) continue;   // stopped by a failed check
+        if (!d.ms[member_of_grain(d, g)].active) continue;   // stopped by a failed check or an error
         // the grains of the three vertices and their flags, in three rounds
         // of independent loads
         const int* L = glist(d, g);
@@ -1631,7 +1630,7 @@ __global__ void k_t2_serial(Dev d) {
         int g = d.l_t2c[k];
         if (!(d.genab[g] && d.gcand[g])) continue;
         int m = member_of_grain(d, g);
-        if (d.debug && !d.ms[m].active) continue;
+        if (!d.ms[m].active) continue;
         remove_grain(d, g, d.prm[m].D);
         d.ms[m].removals += 1;
     }
@@ -1643,7 +1642,7 @@ __global__ void k_st2(Dev d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int b = d.l_cand[k];
         if (!(d.benab[b] && d.bcand[b])) continue;
-        if (d.debug && !d.ms[member_of_edge(d, b)].active) continue;
+        if (!d.ms[member_of_edge(d, b)].active) continue;
         int gic[2];
         if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) { d.bcand[b] = 0; continue; }
         if (d.glen[gic[0]] == 3 || d.glen[gic[1]] == 3) d.bcand[b] = 0;
@@ -1714,7 +1713,7 @@ __device__ __forceinline__ void flip(const Dev& d, int bnd, const Params& P, con
     int i = -1, j = -1, AC = -1, BEb = -1;
     for (int k = 0; k < 3; k++) if (vA[k] == bnd) { i = (k + 1) % 3; AC = vA[i]; break; }
     for (int k = 0; k < 3; k++) if (vB[k] == bnd) { j = (k + 1) % 3; BEb = vB[j]; break; }
-    if (i < 0 || j < 0 || gic[1] < 0 || gnic[1] < 0) { set_error(d, SEVC_E_STATE); return; }
+    if (i < 0 || j < 0 || gic[1] < 0 || gnic[1] < 0) { set_error(d, member_of_edge(d, bnd), SEVC_E_STATE); return; }
     // reads that do not depend on the writes below, issued together
     const int ac0 = d.bvx[2 * AC];
     const int be0 = d.bvx[2 * BEb];
@@ -1758,7 +1757,7 @@ __device__ __forceinline__ void flip(const Dev& d, int bnd, const Params& P, con
             flag_grain(d, gnic[g1], 2);
             break;
         }
-    if (g1 > 1 || g2 > 1 || g3 > 1 || g4 > 1) { set_error(d, SEVC_E_STATE); return; }
+    if (g1 > 1 || g2 > 1 || g3 > 1 || g4 > 1) { set_error(d, member_of_edge(d, bnd), SEVC_E_STATE); return; }
     for (int k = 0; k < 3; k++) {
         if (gA[k] == gic[g2]) { gA[k] = gnic[g1]; d.vg[3 * A + k] = gnic[g1]; }
         if (gB[k] == gic[g4]) { gB[k] = gnic[g3]; d.vg[3 * B + k] = gnic[g3]; }
@@ -1810,7 +1809,7 @@ __global__ void k_flip_par(Dev d) {
         const uint8_t ce = d.benab[c], cc = d.bcand[c];
         const int2 ab = reinterpret_cast<const int2*>(d.bvx)[c];
         if (!(ce && cc)) continue;
-        if (d.debug && !d.ms[member_of_edge(d, c)].active) continue;
+        if (!d.ms[member_of_edge(d, c)].active) continue;
         const FlipIn fi = flip_in(d, ab.x, ab.y);
         int x[6];
         uint8_t xe[6], xc[6];
@@ -1844,7 +1843,7 @@ __global__ void k_flip_serial(Dev d) {
         int c = d.l_flipc[k];
         if (!(d.benab[c] && d.bcand[c])) continue;
         int m = member_of_edge(d, c);
-        if (d.debug && !d.ms[m].active) continue;
+        if (!d.ms[m].active) continue;
         flip(d, c, d.prm[m], flip_in(d, d.bvx[2 * c], d.bvx[2 * c + 1]));
         d.ms[m].flips += 1;
     }
@@ -2589,8 +2588,8 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
                 const int n = n_s[t];
                 int pos = -1;
                 for (int i = 0; i < n; i++) if (Ls[t][i] == prev) { pos = i + 1; break; }
-                if (pos < 0) { set_error(d, SEVC_E_STATE); return; }
-                if (n >= S) { set_error(d, SEVC_E_GRAIN_FULL); return; }
+                if (pos < 0) { set_error(d, m, SEVC_E_STATE); return; }
+                if (n >= S) { set_error(d, m, SEVC_E_GRAIN_FULL); return; }
                 for (int q = n; q > pos; q--) Ls[t][q] = Ls[t][q - 1];
                 Ls[t][pos] = v;
                 n_s[t] = n + 1;

@@ -45,7 +45,7 @@ struct Blk { int m, b, e; };
 
 enum Counter {
     C_CAND = 0, C_TWO, C_T2, C_T2C, C_FLIPC,
-    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_ERR, C_COUNT
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_COUNT
 };
 
 enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17

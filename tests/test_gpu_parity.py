@@ -334,6 +334,49 @@ def test_grain_list_overflow_is_reported():
     st.close()
 
 
+def test_member_error_stops_only_that_member():
+    """In a batch, a member whose grain list overflows (capacity equal to its
+    initial largest grain) stops with SEVC_E_GRAIN_FULL in the same chunk of
+    steps as its own oracle, and the call that meets it names the member;
+    the other members keep stepping and stay bit-identical to their own
+    oracle runs, before and after."""
+    o0 = ref2k_options(dt=2e-4)
+    o0.max_vrt_per_grn = int(sevc.load_files(o0)["g_vlen"].max())
+    m0 = sevc.load_files(o0)
+    o1, m1 = _generated(1500, 3, dt=1e-4)
+    o2 = ref2k_options(dt=1e-4)
+    m2 = sevc.load_files(o2)
+    st = sevc.State([o1, o0, o2], [m1, m0, m2])
+    orcs = [Oracle(o1, m1), Oracle(o0, m0), Oracle(o2, m2)]
+    gpu_fail = orc_fail = None
+    for chunk in range(40):
+        print(f"  chunk {chunk} of 10 steps", flush=True)
+        try:
+            st.step(10)
+        except sevc.SevcError as e:
+            assert gpu_fail is None
+            gpu_fail = (chunk, e.code)
+            assert "member 1" in str(e)
+        for k, orc in enumerate(orcs):
+            if k == 1 and orc_fail is not None:
+                continue
+            rc, _ = orc.step(10)
+            if k == 1 and rc != 0:
+                orc_fail = (chunk, rc)
+                continue
+            assert rc == 0, k
+            if k != 1:
+                assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, (chunk, k)
+                assert st.member_stats(k).status == 0
+        if gpu_fail and chunk >= gpu_fail[0] + 3:
+            break
+    assert gpu_fail is not None and gpu_fail == orc_fail, (gpu_fail, orc_fail)
+    assert gpu_fail[1] == sevc.E_GRAIN_FULL
+    s = st.member_stats(1)
+    assert s.status == sevc.E_GRAIN_FULL and s.done
+    st.close()
+
+
 @pytest.mark.parametrize("n,gen_seed", [(12, 3), (30, 13)])
 def test_degenerate_mesh_error_matches_oracle(n, gen_seed):
     """Run a tiny tessellation down to a handful of grains, where a flip

@@ -215,3 +215,33 @@ def test_snapshot_round_trip_for_resume(tmp_path):
     # the restored mesh is a valid state
     bad, total = sevc.check_mesh(r)
     assert bad == 0 and abs(total - 1.0) < 1e-12
+
+
+def test_snapshot_rejects_grain_longer_than_the_format(tmp_path):
+    """The format has 100 vertex columns per grain (MAX_VRT_PER_GRN,
+    geometry.h:9): a mesh stored at a larger stride with a 101-vertex grain
+    is refused with SEVC_E_GRAIN_FULL and no file is written, instead of a
+    snapshot cut short that the reader then rejects."""
+    h = sevc.HostMesh(4, 2, 1, 128)
+    h["g_vlen"][0] = 101
+    h["g_vtx"][:101] = 0
+    o = ref2k_options()
+    L = sevc.lib()
+    rc = L.sevc_export_snapshot(C.byref(h.struct), C.byref(o), str(tmp_path).encode(), 1, C.c_double(0.0))
+    assert rc == sevc.E_GRAIN_FULL
+    assert not (tmp_path / "000001.txt").exists()
+    h["g_vlen"][0] = 100
+    assert L.sevc_export_snapshot(C.byref(h.struct), C.byref(o), str(tmp_path).encode(), 1, C.c_double(0.0)) == 0
+
+
+@pytest.mark.parametrize("field,value", [("velocity_model", 7), ("nucleation_mode", -1), ("domain_bound", 0.0)])
+def test_create_rejects_unknown_enums_before_any_device_call(field, value):
+    """Unknown velocity / nucleation models and a non-positive domain are
+    refused with SEVC_E_INVALID_ARG before the device is touched (so also
+    on a machine without a GPU)."""
+    o = ref2k_options()
+    m = sevc.load_files(o)
+    setattr(o, field, value)
+    with pytest.raises(sevc.SevcError) as e:
+        sevc.State(o, m)
+    assert e.value.code == sevc.E_INVALID_ARG
