@@ -25,6 +25,11 @@
  *    writes past the array, geometry.h:615-619);
  *  - the hard-coded min_grains = 5000 (main.cu:49) is not reproduced: the
  *    stop rule uses leave_percent.
+ *
+ * ORC_MUTANT (0 by default) builds a deliberately broken copy of the oracle
+ * for the tests that show its pins catch plausible mistakes: each value
+ * plants one mistake (see the `#if ORC_MUTANT == k` sites).  The normal
+ * builds never define it.
  */
 #include "sevc_oracle.h"
 #include "../include/sevc_math.h"
@@ -36,6 +41,9 @@
 
 enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   /* geometry.h:17 */
 enum { T_COMMON = 0, T_NUCLEATED = 1 };                                  /* geometry.h:19 */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 #define STREAM_MC_SITE 1u
 #define STREAM_MC_PICK 2u
 #define STREAM_RATE    3u
@@ -619,7 +627,11 @@ static void polling_system(orc* s) {
             int arg = 0;
             for (int i = 0; i < 3; i++) {
                 int b = VB(s, v, i);
+#if ORC_MUTANT == 1
+                if (s->bt[b] < min_t && s->bcand[b]) { min_t = s->bt[b]; arg = i; }
+#else
                 if (s->bt[b] <= min_t && s->bcand[b]) { min_t = s->bt[b]; arg = i; }
+#endif
             }
             s->voted[v] = VB(s, v, arg);
         }
@@ -643,6 +655,9 @@ static void polling_system(orc* s) {
         int changed = 0;
         for (int b = 0; b < s->ne; b++)
             if (s->benab[b] && s->bcand[b] != s->cand_prev[b]) { changed = 1; break; }
+#if ORC_MUTANT == 2
+        changed = 0;
+#endif
         if (!changed) break;
     }
 }
@@ -662,10 +677,22 @@ static void check_conflictive_boundaries(orc* s) {
             if (x != b && s->bcand[x]) { bc_end = x; break; }
         }
         if (bc_ini >= 0 && bc_end >= 0) {
+#if ORC_MUTANT == 3
+            if (s->bt[bc_ini] == s->bt[bc_end])
+                s->conf[b] = bc_ini < bc_end ? bc_ini : bc_end;
+            else
+                s->conf[b] = s->bt[bc_ini] > s->bt[bc_end] ? bc_ini : bc_end;
+#elif ORC_MUTANT == 4
+            if (s->bt[bc_ini] == s->bt[bc_end])
+                s->conf[b] = bc_ini > bc_end ? bc_ini : bc_end;
+            else
+                s->conf[b] = s->bt[bc_ini] < s->bt[bc_end] ? bc_ini : bc_end;
+#else
             if (s->bt[bc_ini] == s->bt[bc_end])
                 s->conf[b] = bc_ini < bc_end ? bc_ini : bc_end;
             else
                 s->conf[b] = s->bt[bc_ini] < s->bt[bc_end] ? bc_ini : bc_end;
+#endif
         }
     }
 }
@@ -1064,10 +1091,21 @@ static int generate_grain_for_nucleation(const orc* s, int A, double eps, double
         if (do_scale) E0 += 0.99 * rr * s->ben[VB(s, A, i)];
         else E0 += rr * s->ben[VB(s, A, i)];
     }
+#if ORC_MUTANT == 5
+    E0 += A1 * s->gse[r->g2] + A2 * s->gse[r->g1] + A3 * s->gse[r->g3];
+#elif ORC_MUTANT == 6
+    E0 -= A1 * s->gse[r->g1] + A2 * s->gse[r->g2] + A3 * s->gse[r->g3];
+#else
     E0 += A1 * s->gse[r->g1] + A2 * s->gse[r->g2] + A3 * s->gse[r->g3];
+#endif
     double ori = s->nucl_ori[A];
+#if ORC_MUTANT == 11
+    E1 = gb_energy_func(ori - s->gori[r->g1], eps) * scaling * LAB;
+    E1 += gb_energy_func(ori - s->gori[r->g2], eps) * scaling * LCA;
+#else
     E1 = gb_energy_func(ori - s->gori[r->g1], eps) * scaling * LCA;
     E1 += gb_energy_func(ori - s->gori[r->g2], eps) * scaling * LAB;
+#endif
     E1 += gb_energy_func(ori - s->gori[r->g3], eps) * scaling * LBC;
     r->dE = E1 - E0;
     return 0;
@@ -1101,8 +1139,13 @@ static int nucleate_at(orc* s, int A, int B, int C, int AB, int BC, int CA, int 
     nucl_geom r;
     /* nucleation2.cu:526 passes (scaling, eps) swapped; only dE differs */
     if (generate_grain_for_nucleation(s, A, s->o.GB_scaling, s->o.GB_eps, 1, &r) != 0) return -1;
+#if ORC_MUTANT == 9
+    init_vertex(s, B, r.Cpos[0], r.Cpos[1]);
+    init_vertex(s, C, r.Bpos[0], r.Bpos[1]);
+#else
     init_vertex(s, B, r.Bpos[0], r.Bpos[1]);
     init_vertex(s, C, r.Cpos[0], r.Cpos[1]);
+#endif
     PX(s, A) = r.Apos[0]; PY(s, A) = r.Apos[1];
     for (int i = 0; i < 3; i++) VB(s, A, i) = -1;
     init_boundary(s, AB, A, B);
@@ -1114,8 +1157,13 @@ static int nucleate_at(orc* s, int A, int B, int C, int AB, int BC, int CA, int 
     grain_push_vertex(s, g, A);
     grain_push_vertex(s, g, B);
     grain_push_vertex(s, g, C);
+#if ORC_MUTANT == 8
+    for (int i = 0; i < 3; i++)
+        if (VG(s, A, i) == r.g1) { VG(s, A, i) = g; break; }
+#else
     for (int i = 0; i < 3; i++)
         if (VG(s, A, i) == r.g3) { VG(s, A, i) = g; break; }
+#endif
     vertex_add_grain(s, B, g); vertex_add_grain(s, B, r.g2); vertex_add_grain(s, B, r.g3);
     vertex_add_grain(s, C, g); vertex_add_grain(s, C, r.g3); vertex_add_grain(s, C, r.g1);
     grain_add_vertex_after(s, r.g1, C, A);
@@ -1142,12 +1190,14 @@ static int kth_free(const uint8_t* en, int n, int rank) {
     return -1;
 }
 
-/* nucleation2.cu:292-443 + :33-72 + :474 (reference MC mode). */
-static int nucleation_mc(orc* s) {
-    compute_nucleation_factor(s);
+/* nucleation2.cu:292-440: the random alive vertex and the Metropolis rule
+ * (accept if dE < 0, or if d2E < 0 and u < exp(-MC_k dE)), after
+ * compute_nucleation_factor.  Returns the site or -1; vnuc[site] holds the
+ * decision. */
+static int mc_pick_and_accept(orc* s) {
     int n_alive = 0;
     for (int v = 0; v < s->nv; v++) n_alive += s->ven[v] ? 1 : 0;
-    if (n_alive == 0) return 0;
+    if (n_alive == 0) return -1;
     double u0, u1;
     sevc_uniform2(s->o.seed, (uint32_t)s->n, 0u, STREAM_MC_PICK, 0u, &u0, &u1);
     int k = (int)(u0 * (double)n_alive);
@@ -1155,13 +1205,25 @@ static int nucleation_mc(orc* s) {
     int A = -1;
     for (int v = 0; v < s->nv; v++)
         if (s->ven[v]) { if (k == 0) { A = v; break; } k--; }
-    if (A < 0) return 0;
+    if (A < 0) return -1;
     s->vnuc[A] = 0;
     if (s->dE[A] < 0) s->vnuc[A] = 1;
     else if (s->nfac[A] < 0) {
         double thr = sevc_exp(-s->o.MC_k * s->dE[A]);
+#if ORC_MUTANT == 7
+        if (u1 > thr) s->vnuc[A] = 1;
+#else
         if (u1 < thr) s->vnuc[A] = 1;
+#endif
     }
+    return A;
+}
+
+/* nucleation2.cu:292-443 + :33-72 + :474 (reference MC mode). */
+static int nucleation_mc(orc* s) {
+    compute_nucleation_factor(s);
+    int A = mc_pick_and_accept(s);
+    if (A < 0) return 0;
     if (!s->vnuc[A]) return 0;
     int g = kth_free(s->genab, s->ng, 0);
     int b0 = kth_free(s->benab, s->ne, 0), b1 = kth_free(s->benab, s->ne, 1), b2 = kth_free(s->benab, s->ne, 2);
@@ -1176,17 +1238,19 @@ static int nucleation_mc(orc* s) {
  * nucleation_scaling * min(L) * u, orientation 2 pi u.  An event is kept
  * when no earlier candidate (lower grain index) touches any of its three
  * grains; kept events take the free slots in rank order. */
-static int nucleation_rate(orc* s) {
-    int cap = 0;
-    for (int g = 0; g < s->ng; g++) cap += s->genab[g] ? 1 : 0;
-    int* ev_v = (int*)malloc(sizeof(int) * (size_t)(cap + 1));
-    double* ev_r = (double*)malloc(sizeof(double) * (size_t)(cap + 1));
-    double* ev_o = (double*)malloc(sizeof(double) * (size_t)(cap + 1));
-    int* ev_g = (int*)malloc(sizeof(int) * (size_t)(cap + 1) * 3);
+/* The draws of the rate mode: every enabled grain draws with probability
+ * rate * SE_g; a drawing grain names one of its vertices, the radius and
+ * the orientation.  Returns the number of candidate sites; ev_src[e] is the
+ * grain that drew candidate e (may be NULL). */
+static int rate_draws(orc* s, int* ev_v, double* ev_r, double* ev_o, int* ev_g, int* ev_src) {
     int ne = 0;
     for (int g = 0; g < s->ng; g++) {
         if (!s->genab[g] || s->glen[g] < 1) continue;
+#if ORC_MUTANT == 10
+        double p = s->o.nucleation_rate;
+#else
         double p = s->o.nucleation_rate * s->gse[g];
+#endif
         double u0, u1, u2, u3;
         sevc_uniform2(s->o.seed, (uint32_t)s->n, (uint32_t)g, STREAM_RATE, 0u, &u0, &u1);
         if (!(u0 < p)) continue;
@@ -1203,8 +1267,20 @@ static int nucleation_rate(orc* s) {
         ev_r[ne] = s->o.nucleation_scaling * minAL * u2;
         ev_o[ne] = 2 * SEVC_PI * u3;
         for (int i = 0; i < 3; i++) ev_g[3 * ne + i] = VG(s, A, i);
+        if (ev_src) ev_src[ne] = g;
         ne++;
     }
+    return ne;
+}
+
+static int nucleation_rate(orc* s) {
+    int cap = 0;
+    for (int g = 0; g < s->ng; g++) cap += s->genab[g] ? 1 : 0;
+    int* ev_v = (int*)malloc(sizeof(int) * (size_t)(cap + 1));
+    double* ev_r = (double*)malloc(sizeof(double) * (size_t)(cap + 1));
+    double* ev_o = (double*)malloc(sizeof(double) * (size_t)(cap + 1));
+    int* ev_g = (int*)malloc(sizeof(int) * (size_t)(cap + 1) * 3);
+    int ne = rate_draws(s, ev_v, ev_r, ev_o, ev_g, NULL);
     int kept = 0, made = 0;
     /* free slots ranked before any insertion of this step */
     int nfg = 0, nfb = 0, nfv = 0;
@@ -1599,4 +1675,108 @@ int orc_adjoint_window(orc* s, const double* dL_dpos, double* d_pos, double* d_S
         }
     free(lam); free(lam2); free(acc_mu); free(acc_s); free(acc_se);
     return SEVC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* test hooks: single stages of the step on a state, for the tests that  */
+/* pin the oracle (brute force, closed forms, statistics)               */
+/* ------------------------------------------------------------------ */
+
+/* The vote rounds (stages & 1: polling_system, topological.h:549-578) and
+ * the conflict filter (stages & 2: topological.h:83-155) on given
+ * extinction times and candidate flags (one per boundary slot); cand
+ * receives the flags after the stages, conf the conflict records. */
+int orc_poll(orc* s, const double* t_ext, uint8_t* cand, int32_t* conf, int32_t stages) {
+    for (int b = 0; b < s->ne; b++) {
+        s->bt[b] = t_ext[b];
+        s->bcand[b] = cand[b] ? 1 : 0;
+        s->conf[b] = -1;
+    }
+    if (stages & 1) polling_system(s);
+    if (stages & 2) {
+        check_conflictive_boundaries(s);
+        unlabel_conflictive_boundaries(s);
+    }
+    for (int b = 0; b < s->ne; b++) {
+        cand[b] = s->bcand[b];
+        if (conf) conf[b] = s->conf[b];
+    }
+    return 0;
+}
+
+/* generate_grain_for_nucleation (nucleation2.cu:116-212) at vertex A with
+ * radius r and orientation ori, on the current lengths and energies:
+ * xy = (A', B, C) positions, *dE, ids = (D, E, F, AD, AE, AF, g1, g2, g3). */
+int orc_nucl_geometry(orc* s, int32_t A, double r, double ori, int32_t do_scale, double* xy, double* dE,
+                      int32_t* ids) {
+    if (A < 0 || A >= s->nv || !s->ven[A]) return SEVC_E_INVALID_ARG;
+    compute_boundary_arclengths(s);
+    compute_boundary_energies(s);
+    s->nucl_r[A] = r;
+    s->nucl_ori[A] = ori;
+    nucl_geom g;
+    if (generate_grain_for_nucleation(s, A, s->o.GB_eps, s->o.GB_scaling, do_scale, &g) != 0) return SEVC_E_STATE;
+    for (int k = 0; k < 2; k++) { xy[k] = g.Apos[k]; xy[2 + k] = g.Bpos[k]; xy[4 + k] = g.Cpos[k]; }
+    *dE = g.dE;
+    const int v[9] = {g.D, g.E, g.F, g.AD, g.AE, g.AF, g.g1, g.g2, g.g3};
+    for (int k = 0; k < 9; k++) ids[k] = v[k];
+    return 0;
+}
+
+/* nucleate (nucleation2.cu:474-676) at vertex A with radius r and
+ * orientation ori into the first free slots (one grain, three boundaries,
+ * two vertices); slots = (g, AB, BC, CA, B, C) used. */
+int orc_nucleate_vertex(orc* s, int32_t A, double r, double ori, int32_t* slots) {
+    if (A < 0 || A >= s->nv || !s->ven[A]) return SEVC_E_INVALID_ARG;
+    compute_boundary_arclengths(s);
+    compute_boundary_energies(s);
+    s->nucl_r[A] = r;
+    s->nucl_ori[A] = ori;
+    int g = kth_free(s->genab, s->ng, 0);
+    int b0 = kth_free(s->benab, s->ne, 0), b1 = kth_free(s->benab, s->ne, 1), b2 = kth_free(s->benab, s->ne, 2);
+    int v0 = kth_free(s->ven, s->nv, 0), v1 = kth_free(s->ven, s->nv, 1);
+    if (g < 0 || b0 < 0 || b1 < 0 || b2 < 0 || v0 < 0 || v1 < 0) return SEVC_E_GRAIN_FULL;
+    if (nucleate_at(s, A, v0, v1, b0, b1, b2, g) != 0) return SEVC_E_STATE;
+    compute_boundary_arclengths(s);
+    compute_grain_areas(s);
+    const int u[6] = {g, b0, b1, b2, v0, v1};
+    if (slots) for (int k = 0; k < 6; k++) slots[k] = u[k];
+    return s->status;
+}
+
+/* One Metropolis trial of the MC mode at step counter `step`
+ * (compute_nucleation_factor, the random alive vertex, the acceptance
+ * rule): returns 1 if accepted, 0 if not; *site, *dE and *nfac (Delta E and
+ * Delta^2 E of the site) are written.  Nothing is inserted. */
+int orc_mc_trial(orc* s, int32_t step, int32_t* site, double* dE, double* nfac) {
+    const int n0 = s->n;
+    s->n = step;
+    compute_boundary_arclengths(s);
+    compute_boundary_energies(s);
+    compute_nucleation_factor(s);
+    const int A = mc_pick_and_accept(s);
+    s->n = n0;
+    *site = A;
+    if (A < 0) return 0;
+    *dE = s->dE[A];
+    *nfac = s->nfac[A];
+    return s->vnuc[A] ? 1 : 0;
+}
+
+/* The rate mode's draws at step counter `step`: the grain that drew each
+ * candidate site and the site (at most `max`); returns their number. */
+int orc_rate_draws(orc* s, int32_t step, int32_t* grains, int32_t* sites, int32_t max) {
+    const int n0 = s->n;
+    s->n = step;
+    compute_boundary_arclengths(s);
+    int* ev_v = (int*)malloc(sizeof(int) * (size_t)(s->ng + 1));
+    double* ev_r = (double*)malloc(sizeof(double) * (size_t)(s->ng + 1));
+    double* ev_o = (double*)malloc(sizeof(double) * (size_t)(s->ng + 1));
+    int* ev_g = (int*)malloc(sizeof(int) * (size_t)(s->ng + 1) * 3);
+    int* ev_s = (int*)malloc(sizeof(int) * (size_t)(s->ng + 1));
+    const int ne = rate_draws(s, ev_v, ev_r, ev_o, ev_g, ev_s);
+    for (int e = 0; e < ne && e < max; e++) { grains[e] = ev_s[e]; sites[e] = ev_v[e]; }
+    free(ev_v); free(ev_r); free(ev_o); free(ev_g); free(ev_s);
+    s->n = n0;
+    return ne;
 }

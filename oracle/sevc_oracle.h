@@ -58,6 +58,14 @@ void orc_mesh_free(sevc_mesh* m);
 /* check_data restatement (utils.h:85-336): number of violations. */
 int orc_check_mesh(const sevc_mesh* m, double* total_area);
 
+/* Single stages of the step, for the tests that pin the oracle. */
+int orc_poll(orc* s, const double* t_ext, uint8_t* cand, int32_t* conf, int32_t stages);
+int orc_nucl_geometry(orc* s, int32_t A, double r, double ori, int32_t do_scale, double* xy, double* dE,
+                      int32_t* ids);
+int orc_nucleate_vertex(orc* s, int32_t A, double r, double ori, int32_t* slots);
+int orc_mc_trial(orc* s, int32_t step, int32_t* site, double* dE, double* nfac);
+int orc_rate_draws(orc* s, int32_t step, int32_t* grains, int32_t* sites, int32_t max);
+
 /* Geometry primitives exposed for closed-form tests. */
 double orc_wrap_dist(double dest, double orig, double D);     /* geometry.h:147 */
 double orc_dom_mod(double a, double D);                       /* geometry.h:133 */
