@@ -225,6 +225,20 @@ int sevc_check_mesh(const sevc_mesh* m, double* total_area);
 int sevc_check_state(sevc_state* s, int32_t* bad);
 int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): the count of
+ * sevc_check_mesh_device for n_meshes meshes whose topology arrays (v_bnd,
+ * v_grn, v_enabled, b_vtx, b_enabled, g_vtx, g_vlen, g_enabled) are already
+ * in device memory; the other array pointers are not read.  bad[m] (device
+ * memory, n_meshes ints) receives mesh m's violation count.  The call only
+ * enqueues kernels on `stream` (a hipStream_t, NULL for the default
+ * stream) and returns without waiting; it calls no hipMalloc, memset or
+ * copy.  Its scratch (a parameter record and a counter per mesh) is a
+ * static device array of 11.5 KB per GPU, held until the process ends:
+ * at most 64 meshes per call (more: SEVC_E_INVALID_ARG, the reason in
+ * sevc_last_error(NULL)), and calls that overlap on one GPU race. */
+int sevc_check_meshes_device_noalloc(const sevc_mesh* meshes, int32_t n_meshes, int32_t* bad, void* stream);
+
 /* Snapshot in the format of export_data (io.h:280-377) to
  * <folder>/<index as %06d>.txt.  The format holds 100 vertices per grain
  * (MAX_VRT_PER_GRN, geometry.h:9): a grain with a longer list returns
@@ -355,6 +369,8 @@ int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
 int sevc_selftest_math(int32_t op, int32_t n, const double* x, const double* y, double* out);
 
 void sevc_destroy(sevc_state* s);
+/* Message of the last failure of state s; with s == NULL, of the last
+ * failed call on this thread that takes no state. */
 const char* sevc_last_error(const sevc_state* s);
 const char* sevc_strerror(int code);
 int sevc_device_count(void);

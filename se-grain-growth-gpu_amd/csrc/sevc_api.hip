@@ -340,7 +340,7 @@ int sevc_device_count(void) {
     return n;
 }
 
-const char* sevc_last_error(const sevc_state* s) { return s ? s->err.c_str() : "null state"; }
+const char* sevc_last_error(const sevc_state* s) { return s ? s->err.c_str() : stateless_error(); }
 
 int32_t sevc_n_members(const sevc_state* s) { return s ? s->B : 0; }
 

@@ -29,4 +29,7 @@ __global__ void k_check_clear(int* bad, int B);
 __global__ void k_check(ChkView c);
 __global__ void k_check_stop(const Params* prm, MState* ms, const int* bad, int B, int phase);
 
+// Message of the last failed call made without a state on this thread.
+const char* stateless_error();
+
 }  // namespace sevc

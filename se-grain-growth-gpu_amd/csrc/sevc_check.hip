@@ -13,6 +13,9 @@
 // the input holds.  The host checker (sevc_check_mesh) counts the same way.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "sevc_state.h"
 #include "sevc_check.h"
 #include "../../include/sevc.h"
@@ -157,4 +160,118 @@ __global__ void k_check_stop(const Params* prm, MState* ms, const int* bad, int 
     }
 }
 
+
+// ---------------------------------------------------------------------
+// check_data of meshes already in device memory, allocating nothing
+// ---------------------------------------------------------------------
+
+// The scratch of sevc_check_meshes_device_noalloc: one parameter record
+// (the slot ranges k_check reads) and one violation counter per mesh, for
+// at most kNoAllocMeshes meshes per call.  Each GPU holds its own copy for
+// the life of the process; calls that overlap on one GPU share it.
+constexpr int kNoAllocMeshes = 64;
+constexpr size_t kNoAllocPrmBytes = (sizeof(Params) * kNoAllocMeshes + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocPrmBytes + sizeof(int) * kNoAllocMeshes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Slot counts of the meshes of one call, passed by value.
+struct NoAllocSizes {
+    int n;
+    int nv[kNoAllocMeshes], ne[kNoAllocMeshes], ng[kNoAllocMeshes];
+};
+
+// Fill the parameter records and clear the counters (a kernel, not a
+// memset or a copy, so that the call can be captured into a graph).
+__global__ void k_noalloc_setup(unsigned char* scr, NoAllocSizes z) {
+    Params* prm = reinterpret_cast<Params*>(scr);
+    int* cnt = reinterpret_cast<int*>(scr + kNoAllocPrmBytes);
+    const int m = threadIdx.x;
+    if (m >= z.n) return;
+    Params P{};
+    P.voff = 0; P.nv = z.nv[m]; P.eoff = 0; P.ne = z.ne[m]; P.goff = 0; P.ng = z.ng[m];
+    P.debug = 1;
+    prm[m] = P;
+    cnt[m] = 0;
+}
+
+__global__ void k_noalloc_out(const unsigned char* scr, int n, int* bad) {
+    const int* cnt = reinterpret_cast<const int*>(scr + kNoAllocPrmBytes);
+    const int m = threadIdx.x;
+    if (m < n) bad[m] = cnt[m];
+}
+
+namespace {
+thread_local std::string g_stateless_err;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[256] = {};        // device address of `scratch` per device id
+
+int stateless_fail(int code, const std::string& msg) {
+    g_stateless_err = msg;
+    return code;
+}
+}  // namespace
+
+const char* stateless_error() { return g_stateless_err.c_str(); }
+
 }  // namespace sevc
+
+using namespace sevc;
+
+extern "C" int sevc_check_meshes_device_noalloc(const sevc_mesh* meshes, int32_t n_meshes, int32_t* bad,
+                                                void* stream) {
+    if (!meshes || !bad) return stateless_fail(SEVC_E_INVALID_ARG, "null meshes or output");
+    if (n_meshes <= 0 || n_meshes > kNoAllocMeshes)
+        return stateless_fail(SEVC_E_INVALID_ARG, "n_meshes = " + std::to_string(n_meshes) +
+                                                      " outside [1, " + std::to_string(kNoAllocMeshes) +
+                                                      "], the meshes one call's static scratch holds");
+    NoAllocSizes z{};
+    z.n = n_meshes;
+    for (int m = 0; m < n_meshes; ++m) {
+        const sevc_mesh& h = meshes[m];
+        if (h.n_vertices <= 0 || h.n_boundaries <= 0 || h.n_grains <= 0 || h.max_vrt_per_grn <= 0 || !h.v_bnd ||
+            !h.v_grn || !h.v_enabled || !h.b_vtx || !h.b_enabled || !h.g_vtx || !h.g_vlen || !h.g_enabled)
+            return stateless_fail(SEVC_E_INVALID_ARG, "mesh " + std::to_string(m) + ": empty or missing arrays");
+        z.nv[m] = h.n_vertices; z.ne[m] = h.n_boundaries; z.ng[m] = h.n_grains;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256)
+        return stateless_fail(SEVC_E_NO_DEVICE, "no current HIP device");
+    unsigned char* scr = nullptr;
+    {
+        // the scratch's device address on this device, looked up once (the
+        // symbol's host-side name is no device address)
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return stateless_fail(SEVC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                                      hipGetErrorString(e));
+            g_scratch_ptr[dev] = p;
+        }
+        scr = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(kNoAllocMeshes), 0, st, scr, z);
+    const Params* prm = reinterpret_cast<const Params*>(scr);
+    int* cnt = reinterpret_cast<int*>(scr + kNoAllocPrmBytes);
+    for (int m = 0; m < n_meshes; ++m) {
+        const sevc_mesh& h = meshes[m];
+        ChkView c{};
+        c.vb = h.v_bnd; c.vg = h.v_grn; c.ven = h.v_enabled;
+        c.bvx = h.b_vtx; c.benab = h.b_enabled;
+        c.gvx = h.g_vtx; c.glen = h.g_vlen; c.genab = h.g_enabled;
+        c.S = h.max_vrt_per_grn; c.V = h.n_vertices; c.E = h.n_boundaries; c.G = h.n_grains; c.B = 1;
+        c.prm = prm + m;
+        c.gate = 0;
+        c.bad = cnt + m;
+        const long long slots = (long long)h.n_vertices + h.n_boundaries + h.n_grains;
+        const int grid = static_cast<int>(slots / 256 + 1 < 8192 ? slots / 256 + 1 : 8192);
+        hipLaunchKernelGGL(k_check, dim3(grid), dim3(256), 0, st, c);
+    }
+    hipLaunchKernelGGL(k_noalloc_out, dim3(1), dim3(kNoAllocMeshes), 0, st, static_cast<const unsigned char*>(scr),
+                       static_cast<int>(n_meshes), bad);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return stateless_fail(SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
+    return SEVC_OK;
+}

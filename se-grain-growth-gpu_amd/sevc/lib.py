@@ -47,6 +47,7 @@ def lib():
         "sevc_check_mesh": (C.c_int, [P(Mesh), P(C.c_double)]),
         "sevc_check_mesh_device": (C.c_int, [P(Mesh), P(C.c_int32)]),
         "sevc_check_state": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+        "sevc_check_meshes_device_noalloc": (C.c_int, [P(Mesh), C.c_int32, C.c_void_p, C.c_void_p]),
         "sevc_export_snapshot": (C.c_int, [P(Mesh), P(Options), C.c_char_p, C.c_int32, C.c_double]),
         "sevc_load_snapshot": (C.c_int, [C.c_char_p, C.c_int32, P(Mesh), P(C.c_double)]),
         "sevc_create": (C.c_int, [P(Options), P(Mesh), P(C.c_void_p)]),
