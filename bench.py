@@ -28,10 +28,13 @@ HBM_PEAK_GBS = 8000.0
 # each datum once): velocity 98 B, translation 48 B.
 B_VELOCITY = 98.0
 B_TRANSLATE = 48.0
-# HBM bytes per enabled vertex of the two velocity passes at the 1M-grain
-# workload, from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy of known
-# size (profiles/step_1M_hbm_traffic.md): 515.0 MB for 2.0M vertices.
-TRAFFIC_VELOCITY_B_PER_VERTEX = 258.0
+# HBM bytes per enabled vertex of the FD velocity at the 1M-grain workload,
+# by path, from rocprofv3 FETCH_SIZE / WRITE_SIZE of `python bench.py`
+# scaled by a plain copy of known size (the summaries under profiles/
+# named here); reported with their source, not measured in this run.
+TRAFFIC_VELOCITY = {
+    0: (258.0, "profiles/step_1M_hbm_traffic.md (two passes: k_vel_grain + k_vel_vertex)"),
+}
 
 
 def log(*a):
@@ -70,20 +73,52 @@ def make_mesh(sevc, n_grains, seed):
     return o, sevc.build_mesh(o, pos, rid, ori, se, n_grains)
 
 
+def make_mc_mesh(sevc, n_grains, seed, mc_k=5.0):
+    """The reference's Monte Carlo nucleation mode at scale
+    (nucleation2.cu:239-443): one Metropolis site per step, opened from the
+    first step (the reference waits until 1/6 of the grains are gone,
+    main.cu:137-139), in a domain of side 1000 so that edges are O(1) long as
+    in the reference's example (DOMAIN BOUND 40 for 2,000 grains), stored
+    energy uniform in [3, 6] (SE_min 3, SE_max 6 of config/example.ini),
+    GB_eps = 0.3, adaptive dt, flip threshold 0.1."""
+    pos, rid = sevc.voronoi(n_grains, seed)
+    rng = np.random.default_rng(seed)
+    o = make_options(sevc, n_grains)
+    o.domain_bound = 1000.0
+    o.dt = 1.0
+    o.flip_threshold = 0.1
+    o.GB_eps = 0.3
+    o.nucleation_mode = sevc.NUCL_MC
+    o.MC_k = mc_k
+    o.nucleation_gate = n_grains
+    o.nucleation_gap = 1
+    o.seed = seed
+    mesh = sevc.build_mesh(o, pos, rid, rng.random(n_grains), rng.random(n_grains), n_grains)
+    mesh["g_SE"][:n_grains] = 3.0 + 3.0 * rng.random(n_grains)
+    return o, mesh
+
+
+def make_member(sevc, i, seed0=1000, g_lo=1000, g_hi=50000):
+    """Member i of the ensemble workload: G_i = round(exp(ln g_lo + U_i
+    ln(g_hi/g_lo))) grains (log-uniform), Voronoi seed seed0+i, SE = s_i *
+    U[0,1) with s_i ~ U[0,1], Philox seed seed0+i, and the model options of
+    the single-domain workload."""
+    rng = np.random.default_rng(seed0 + i)
+    g = int(round(np.exp(np.log(g_lo) + rng.random() * np.log(g_hi / g_lo))))
+    s_i = rng.random()
+    pos, rid = sevc.voronoi(g, seed0 + i)
+    o = make_options(sevc, g)
+    o.seed = seed0 + i
+    return o, sevc.build_mesh(o, pos, rid, rng.random(g), s_i * rng.random(g), g)
+
+
 def make_ensemble(sevc, members, seed0=1000, g_lo=1000, g_hi=50000):
-    """Ensemble workload: member i has G_i = round(exp(ln g_lo + U_i ln(g_hi/g_lo)))
-    grains (log-uniform), Voronoi seed seed0+i, SE = s_i * U[0,1) with
-    s_i ~ U[0,1], and the model options of the single-domain workload."""
+    """The first `members` members of the ensemble workload (make_member)."""
     opts, meshes = [], []
     for i in range(members):
-        rng = np.random.default_rng(seed0 + i)
-        g = int(round(np.exp(np.log(g_lo) + rng.random() * np.log(g_hi / g_lo))))
-        s_i = rng.random()
-        pos, rid = sevc.voronoi(g, seed0 + i)
-        o = make_options(sevc, g)
-        o.seed = seed0 + i
-        meshes.append(sevc.build_mesh(o, pos, rid, rng.random(g), s_i * rng.random(g), g))
+        o, m = make_member(sevc, i, seed0, g_lo, g_hi)
         opts.append(o)
+        meshes.append(m)
     return opts, meshes
 
 
@@ -118,9 +153,12 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
         raise SystemExit("ensemble graph replay: a member failed")
     graph_rate = sum(p.n_vertex_steps for p in perg) / (g2 - g1)
     st.close()
+    # the same members one state at a time, each autotuned like the batch,
+    # from the same warm-up over the same steps
     vs, el = 0, 0.0
     for o, m in zip(opts, meshes):
         one = sevc.State(o, m)
+        one.autotune()
         one.step(warmup)
         one.step(0)
         a = time.perf_counter()
@@ -136,10 +174,44 @@ def bench_ensemble(sevc, members, steps, b2b_steps, warmup=5):
             "ms_per_step": (t2 - t1) / max(1, s.steps_done) * 1e3,
             "graph_replay": {"value": graph_rate, "steps": gsteps,
                              "ms_per_step": (g2 - g1) / max(1, sg.steps_done) * 1e3},
-            "back_to_back": {"value": b2b, "steps_per_member": b2b_steps, "seconds": el},
+            "back_to_back": {"value": b2b, "steps_per_member": b2b_steps, "warmup": warmup, "seconds": el,
+                             "note": "each member alone, autotuned, after the same warm-up as the batch"},
             "speedup": batched / b2b,
             "workload": f"{members} members, G log-uniform in [1000, 50000] (Voronoi seed 1000+i), "
                         "SE = s_i*U[0,1) with s_i ~ U[0,1]; model options of the single-domain workload"}
+
+
+def bench_at_scale(sevc, grains, steps, warmup=5):
+    """The plain step at full size with anisotropic boundary energy (the
+    benchmark workload with GB_eps = 0.3) and in the reference's Monte
+    Carlo nucleation mode (make_mc_mesh): vertex-steps/s over `steps`."""
+    out = {}
+    for name, make in (("anisotropic_gb_eps_0.3", lambda: _aniso(sevc, grains)),
+                       ("mc_nucleation", lambda: make_mc_mesh(sevc, grains, 1))):
+        o, mesh = make()
+        st = sevc.State(o, mesh)
+        st.autotune()
+        st.step(warmup)
+        st.step(0)
+        a = time.perf_counter()
+        s = st.step(steps)
+        b = time.perf_counter()
+        if s.status != 0:
+            raise SystemExit(f"{name}: step failed with status {s.status}")
+        out[name] = {"value": s.n_vertex_steps / (b - a), "unit": "vertex-steps/s", "steps": s.steps_done,
+                     "ms_per_step": (b - a) / max(1, s.steps_done) * 1e3, "flips": s.n_flips,
+                     "removals": s.n_removals, "nucleations": s.n_nucleations,
+                     "velocity_path": st.velocity_config()[0]}
+        log(f"at scale, {name}: {out[name]['value']:.3e} vertex-steps/s, {out[name]['ms_per_step']:.3f} ms/step, "
+            f"{s.n_nucleations} nucleations")
+        st.close()
+    return out
+
+
+def _aniso(sevc, grains):
+    o, mesh = make_mesh(sevc, grains, 1)
+    o.GB_eps = 0.3
+    return o, mesh
 
 
 def window_options(sevc, n_grains, dt):
@@ -156,165 +228,265 @@ def window_options(sevc, n_grains, dt):
     return o
 
 
-def bench_adjoint(sevc, grains, seed, steps, ckpt, dt, n_fd=20, reps=3):
-    """Adjoint over a topology-fixed window: a smooth 20-bin histogram of
-    grain areas after `steps` steps, against the histogram the same window
-    gives with (1.1 gamma, 0.9 mu).  Times the window forward (with its
-    checkpoints) and the reverse pass (with the segment recomputation), and
-    checks the reverse pass against differences of the forward."""
-    from sevc.calib import AreaHistogram, grain_areas
+def bench_adjoint(sevc, grains, seed, steps, ckpt, dt, n_se=18, reps=3):
+    """Adjoint over a topology-fixed window: the loss is the squared
+    difference of the smooth 20-bin histogram of grain areas after `steps`
+    steps (sevc_area_histogram, on the device) from the histogram the same
+    window gives with (1.1 gamma, 0.9 mu).  Times the window forward (with
+    its checkpoints) against the reverse pass (the loss's dL/dx seeding it
+    on the device, and the segment recomputation), and checks dL/dgamma,
+    dL/dmu and dL/dSE_g of n_se random grains against central differences
+    of the same loss (Richardson: steps h and h/2).  The step is h = 1e-6;
+    where the rounding of the loss puts more noise than that on a
+    difference quotient (the two quotients disagree by more than 1e-6 of
+    their size), the smallest of 1e-5, 1e-4, 1e-3, 1e-2 that clears it is
+    used and reported."""
+    from sevc.calib import grain_areas
     pos, rid = sevc.voronoi(grains, seed)
     rng = np.random.default_rng(seed)
     o = window_options(sevc, grains, dt)
     mesh = sevc.build_mesh(o, pos, rid, rng.random(grains), 0.3 * rng.random(grains), grains)
     am = float(np.abs(grain_areas(mesh)).mean())
-    hist = AreaHistogram(20, 0.0, 3.0 * am, 0.1 * am)
+    ranges = np.array([[0.0, 3.0 * am, 0.1 * am]])
     st = sevc.State(o, mesh)
     st.set_params(0, gb_scaling=1.1, mobility=0.9)
     st.window_forward(steps, ckpt)
-    target = hist(mesh, st.positions(0))
-    x0 = mesh["pos"].copy()
+    target, _ = st.area_histogram(20, ranges, np.zeros((1, 20)))
+    x0, se0 = mesh["pos"].copy(), mesh["g_SE"].copy()
 
-    def forward(pos=x0, se=None, gb=1.0, mu=1.0):
-        st.set_params(0, pos=pos, g_SE=mesh["g_SE"] if se is None else se, gb_scaling=gb, mobility=mu)
+    def loss(se=se0, gb=1.0, mu=1.0):
+        st.set_params(0, pos=x0, g_SE=se, gb_scaling=gb, mobility=mu)
         st.window_forward(steps, ckpt)
-        return hist.loss_and_grad(mesh, st.positions(0), target)
+        return float(st.area_histogram(20, ranges, target)[1][0])
 
-    L, gN = forward()
     t_f, t_a = [], []
     for _ in range(reps):
-        st.set_params(0, pos=x0, g_SE=mesh["g_SE"], gb_scaling=1.0, mobility=1.0)
+        st.set_params(0, pos=x0, g_SE=se0, gb_scaling=1.0, mobility=1.0)
         a = time.perf_counter()
         st.window_forward(steps, ckpt)
         b = time.perf_counter()
-        (dp, dse, dgb, dmu), = st.adjoint(gN)
+        _, lv = st.area_histogram(20, ranges, target, seed=True)   # dL/dx seeds the reverse pass on the device
+        (dp, dse, dgb, dmu), = st.adjoint(None)
         c = time.perf_counter()
         t_f.append(b - a)
         t_a.append(c - b)
-    # accuracy: a seeded linear functional w . x_N, whose reverse pass is
-    # compared with Richardson central differences of the window forward on
-    # n_fd - 2 seeded position coordinates (h = 1e-6), on GB_scaling and
-    # mobility (h = 1e-3), and along one seeded direction in SE (h = 3e-3;
-    # a single SE_g moves x_N by less than the rounding of a 20k-term sum).
-    # The rounding of w . x_N over a 500-step window is ~2e-13, so a
-    # difference quotient carries ~3e-13 / h of noise: parameter steps are
-    # taken large enough to keep that below 1e-6 of the derivative, which
-    # the O(h^4) Richardson truncation allows.
+    L0 = float(lv[0])
     pick = np.random.default_rng(seed + 1)
+    coords = [("gamma", -1), ("mu", -1)] + [("SE_g", int(g)) for g in
+                                            pick.choice(np.nonzero(mesh["g_enabled"])[0], n_se, replace=False)]
+
+    def f(kind, g, e):
+        if kind == "gamma":
+            return loss(gb=1.0 + e)
+        if kind == "mu":
+            return loss(mu=1.0 + e)
+        se = se0.copy()
+        se[g] += e
+        return loss(se=se)
+
+    checks, worst = [], 0.0
+    for kind, g in coords:
+        ad = {"gamma": dgb, "mu": dmu}.get(kind, dse[g] if g >= 0 else 0.0)
+        used, noisy_at_1e6 = None, False
+        for h in (1e-6, 1e-5, 1e-4, 1e-3, 1e-2):
+            c1 = (f(kind, g, h) - f(kind, g, -h)) / (2 * h)
+            c2 = (f(kind, g, h / 2) - f(kind, g, -h / 2)) / h
+            if abs(c1 - c2) <= 1e-6 * max(abs(c2), 1e-300) or h == 1e-2:
+                used = (h, (4 * c2 - c1) / 3)
+                break
+            if h == 1e-6:
+                noisy_at_1e6 = True
+        h, fd = used
+        rel = abs(ad - fd) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        checks.append({"coord": kind if g < 0 else f"SE_{g}", "adjoint": ad, "fd": fd, "rel_err": rel, "h": h,
+                       "h_1e-6_noise_limited": noisy_at_1e6})
+    # extra: a seeded linear functional w . x_N of the window's end
+    # positions, against Richardson differences in start positions (h = 1e-6)
+    # and in gamma, mu (h = 1e-3)
     w = pick.standard_normal(2 * mesh.nv) * np.repeat(mesh["v_enabled"], 2)
-    u = pick.standard_normal(mesh.ng) * mesh["g_enabled"]
-    st.set_params(0, pos=x0, g_SE=mesh["g_SE"], gb_scaling=1.0, mobility=1.0)
+    st.set_params(0, pos=x0, g_SE=se0, gb_scaling=1.0, mobility=1.0)
     st.window_forward(steps, ckpt)
     xN = st.positions(0).reshape(-1).copy()
-    (lp, ls, lgb, lmu), = st.adjoint(w)
+    (lp, _, lgb, lmu), = st.adjoint(w)
     D = mesh.D
 
-    def lin(p, s, gbv, muv):
-        st.set_params(0, pos=p, g_SE=s, gb_scaling=gbv, mobility=muv)
+    def lin(p=x0, gbv=1.0, muv=1.0):
+        st.set_params(0, pos=p, g_SE=se0, gb_scaling=gbv, mobility=muv)
         st.window_forward(steps, ckpt)
         d = st.positions(0).reshape(-1) - xN
         return float(np.dot(w, d - D * np.round(d / D)))
 
-    checks = [("pos", int(c), 1e-6) for c in pick.choice(np.nonzero(np.repeat(mesh["v_enabled"], 2))[0],
-                                                          n_fd - 2, replace=False)]
-    checks += [("gb", 0, 1e-3), ("mu", 0, 1e-3), ("SE-dir", 0, 3e-3)]
-    worst, detail = 0.0, []
-    for kind, i, h in checks:
-        def f(e):
-            p, s, gbv, muv = x0.copy(), mesh["g_SE"].copy(), 1.0, 1.0
+    lin_worst = 0.0
+    for kind, i, h in [("pos", int(c), 1e-6) for c in pick.choice(np.nonzero(w)[0], 6, replace=False)] + \
+                      [("gamma", -1, 1e-3), ("mu", -1, 1e-3)]:
+        def fl(e):
             if kind == "pos":
+                p = x0.copy()
                 p[i] += e
-            elif kind == "SE-dir":
-                s = s + e * u
-            elif kind == "gb":
-                gbv += e
-            else:
-                muv += e
-            return lin(p, s, gbv, muv)
-        c1 = (f(h) - f(-h)) / (2 * h)
-        c2 = (f(h / 2) - f(-h / 2)) / h
+                return lin(p=p)
+            return lin(gbv=1.0 + e) if kind == "gamma" else lin(muv=1.0 + e)
+        c1 = (fl(h) - fl(-h)) / (2 * h)
+        c2 = (fl(h / 2) - fl(-h / 2)) / h
         fd = (4 * c2 - c1) / 3
-        ad = {"pos": lambda: lp[i], "SE-dir": lambda: float(np.dot(u, ls)), "gb": lambda: lgb,
-              "mu": lambda: lmu}[kind]()
-        rel = abs(ad - fd) / max(abs(fd), 1e-300)
-        worst = max(worst, rel)
-        detail.append([kind, i, ad, fd, rel])
-    coords = checks
+        ad = lp[i] if kind == "pos" else (lgb if kind == "gamma" else lmu)
+        lin_worst = max(lin_worst, abs(ad - fd) / max(abs(fd), 1e-300))
     st.close()
     tf, ta = min(t_f), min(t_a)
+    n_noisy = sum(c["h_1e-6_noise_limited"] for c in checks)
     log(f"adjoint: {grains} grains, {steps} steps (checkpoint every {ckpt}): forward {tf * 1e3:.1f} ms, "
-        f"reverse {ta * 1e3:.1f} ms, ratio {ta / tf:.2f}; worst FD rel. error {worst:.2e}")
+        f"reverse {ta * 1e3:.1f} ms, ratio {ta / tf:.2f}; loss {L0:.3e}; worst FD rel. error {worst:.2e} over "
+        f"{len(checks)} coordinates ({n_noisy} needed h > 1e-6)")
     return {"ratio": ta / tf, "forward_ms": tf * 1e3, "adjoint_ms": ta * 1e3, "grains": grains, "steps": steps,
-            "ckpt_every": ckpt, "dt": dt, "loss": L, "fd_max_rel_err": worst, "fd_coords": len(coords),
-            "fd_checks": detail,
+            "ckpt_every": ckpt, "dt": dt, "loss": L0, "fd_max_rel_err": worst, "fd_coords": len(checks),
+            "fd_checks": checks,
+            "linear_functional_fd_max_rel_err": lin_worst,
+            "fd_note": "central differences of the histogram loss, Richardson over h and h/2; h = 1e-6 unless "
+                       "the two quotients disagree by more than 1e-6 (rounding noise of the loss), then the "
+                       "smallest of 1e-5..1e-2 that clears it (h_1e-6_noise_limited)",
             "workload": f"periodic Voronoi of {grains} seeds (seed {seed}), SE = 0.3*U[0,1), 20-bin smooth "
                         "area histogram vs the window's histogram at (1.1 gamma, 0.9 mu)"}
 
 
-def bench_calibration(sevc, members, iters, steps, ckpt, dt, seed0=2000):
-    """Batched calibration: members with G ~ U[5000, 20000] grains and
-    lognormal SE; each member's target is its own 20-bin area histogram
-    after the window at (gamma, mu) = (1.1, 0.9).  Starting from (1, 1),
-    Adam on (log gamma, log mu) with the summed loss; every iteration is
-    one batched window forward, the device histogram loss (its gradient
-    seeds the reverse pass on the device) and one batched reverse pass."""
+def horizon_forward(st, n_members, horizon, window, ckpt, keep=False):
+    """A horizon of `horizon` steps run as topology-fixed windows of
+    window - 1 steps (sevc_window_forward, analytic model, fixed dt, no
+    events), each followed by one step of the model with T1/T2 events
+    (sevc_step).  With keep, each window's starting meshes are returned
+    (one host mesh per member), the checkpoints of the reverse pass."""
+    starts = []
+    for _ in range(horizon // window):
+        if keep:
+            starts.append([st.get_mesh(m) for m in range(n_members)])
+        st.window_forward(window - 1, ckpt)
+        st.step(1)
+    return starts
+
+
+def horizon_adjoint(st, n_members, starts, window, ckpt, lam_end):
+    """Reverse pass over a horizon: from the last window back, each window's
+    start meshes are restored, the window is recomputed and its adjoint run
+    from lam (dL/dx at the window's end).  Within a window the gradient is
+    exact; across the event step between windows dL/dx is passed on by
+    vertex slot unchanged (the step's own Jacobian is not formed) and is
+    zero for vertices the step removed.  Returns per member (dL/dSE_g,
+    dL/dgamma, dL/dmu) summed over the windows."""
+    lam = [np.asarray(x, dtype=np.float64).copy() for x in lam_end]
+    d_se = None
+    d_gb = np.zeros(n_members)
+    d_mu = np.zeros(n_members)
+    for k in reversed(range(len(starts))):
+        for m in range(n_members):
+            st.set_mesh(m, starts[k][m])
+        st.window_forward(window - 1, ckpt)
+        res = st.adjoint(lam)
+        if d_se is None:
+            d_se = [r[1].copy() for r in res]
+        else:
+            for m in range(n_members):
+                d_se[m] += res[m][1]
+        d_gb += np.array([r[2] for r in res])
+        d_mu += np.array([r[3] for r in res])
+        if k > 0:
+            # slots enabled at the window's start, seen from the previous
+            # window's end: removed vertices carry no adjoint
+            lam = [res[m][0] * np.repeat(starts[k][m]["v_enabled"], 2) for m in range(n_members)]
+    return d_se, d_gb, d_mu
+
+
+def bench_calibration(sevc, members, iters, horizon, window, ckpt, dt, seed0=2000):
+    """Batched calibration: members with G ~ U[5000, 20000] grains sharing
+    global (gamma, mu) and holding their own per-grain SE (true values
+    lognormal, mean 0.3, sigma 0.5).  Each member's target is its own 20-bin
+    area histogram after a horizon of `horizon` steps at (1.1, 0.9, true SE),
+    split into topology-fixed windows with T1/T2 events between them.
+    Starting from gamma = mu = 1 and SE = 0.3 everywhere, Adam on
+    (log gamma, log mu, log SE_g of every grain of every member) with the
+    ensemble-mean loss; every iteration is one batched horizon forward, the
+    device histogram loss and one batched reverse pass over the windows."""
     from sevc.calib import grain_areas
-    opts, meshes, ranges = [], [], []
+    opts, meshes, ranges, se_true = [], [], [], []
     t0 = time.time()
     for i in range(members):
         rng = np.random.default_rng(seed0 + i)
         g = int(rng.integers(5000, 20001))
         pos, rid = sevc.voronoi(g, seed0 + i)
         o = window_options(sevc, g, dt)
+        o.events = 1                         # T1/T2 between windows
         se = np.exp(np.log(0.3) - 0.125 + 0.5 * rng.standard_normal(g))
         m = sevc.build_mesh(o, pos, rid, rng.random(g), se, g)
         am = float(np.abs(grain_areas(m)).mean())
         opts.append(o)
         meshes.append(m)
         ranges.append((0.0, 3.0 * am, 0.1 * am))
+        se_true.append(m["g_SE"].copy())
     grains = sum(o.n_grains for o in opts)
     log(f"calibration: {members} members, {grains} grains ({time.time() - t0:.1f} s)")
     st = sevc.State(opts, meshes)
 
-    def set_all(gb, mu):
+    def start(gb, mu, se_list):
         for k, m in enumerate(meshes):
-            st.set_params(k, pos=m["pos"], gb_scaling=gb, mobility=mu)
+            h = m.copy()
+            h["g_SE"][:] = se_list[k]
+            st.set_mesh(k, h)
+            st.set_params(k, gb_scaling=gb, mobility=mu)
 
-    set_all(1.1, 0.9)
-    st.window_forward(steps, ckpt)
-    targets = np.zeros((members, 20))
-    targets, _ = st.area_histogram(20, ranges, targets)
-    theta = np.zeros(2)                        # log gamma, log mu
+    start(1.1, 0.9, se_true)
+    horizon_forward(st, members, horizon, window, ckpt)
+    targets, _ = st.area_histogram(20, ranges, np.zeros((members, 20)))
+    se_init = [np.full(m.ng, 0.3) * m["g_enabled"] for m in meshes]
+    theta = np.zeros(2)                                   # log gamma, log mu
+    phi = [np.log(np.maximum(x, 1e-300)) for x in se_init]  # log SE per grain
     m1, m2 = np.zeros(2), np.zeros(2)
-    lr, b1, b2 = 0.02, 0.9, 0.999
+    p1 = [np.zeros_like(x) for x in phi]
+    p2 = [np.zeros_like(x) for x in phi]
+    lr, lr_se, b1, b2 = 0.02, 0.05, 0.9, 0.999
     trace = []
     t1 = time.perf_counter()
-    t_dev = 0.0
     for it in range(iters):
         gb, mu = float(np.exp(theta[0])), float(np.exp(theta[1]))
-        set_all(gb, mu)
-        a = time.perf_counter()
-        st.window_forward(steps, ckpt)
-        _, losses = st.area_histogram(20, ranges, targets, seed=True)    # dL/dx stays on the device
-        L = float(losses.sum())
-        res = st.adjoint(None)
-        t_dev += time.perf_counter() - a
-        g = np.array([sum(r[2] for r in res) * gb, sum(r[3] for r in res) * mu])
+        se_now = [np.exp(x) * mm["g_enabled"] for x, mm in zip(phi, meshes)]
+        start(gb, mu, se_now)
+        starts = horizon_forward(st, members, horizon, window, ckpt, keep=True)
+        _, losses, lam = st.area_histogram(20, ranges, targets, want_grad=True)
+        L = float(losses.mean())
+        d_se, d_gb, d_mu = horizon_adjoint(st, members, starts, window, ckpt, [x / members for x in lam])
+        g = np.array([d_gb.sum() * gb, d_mu.sum() * mu])
         trace.append((L, gb, mu))
         m1 = b1 * m1 + (1 - b1) * g
         m2 = b2 * m2 + (1 - b2) * g * g
-        theta -= lr * (m1 / (1 - b1 ** (it + 1))) / (np.sqrt(m2 / (1 - b2 ** (it + 1))) + 1e-12)
+        corr1, corr2 = 1 - b1 ** (it + 1), 1 - b2 ** (it + 1)
+        theta -= lr * (m1 / corr1) / (np.sqrt(m2 / corr2) + 1e-12)
+        for k in range(members):
+            gk = d_se[k] * se_now[k]
+            p1[k] = b1 * p1[k] + (1 - b1) * gk
+            p2[k] = b2 * p2[k] + (1 - b2) * gk * gk
+            phi[k] -= lr_se * (p1[k] / corr1) / (np.sqrt(p2[k] / corr2) + 1e-30)
+        if it % 10 == 0:
+            log(f"calibration: iteration {it}, loss {L:.4e}, (gamma, mu) = ({gb:.4f}, {mu:.4f})")
     el = time.perf_counter() - t1
     st.close()
-    log(f"calibration: loss {trace[0][0]:.3e} -> {trace[-1][0]:.3e}, (gamma, mu) -> "
-        f"({trace[-1][1]:.4f}, {trace[-1][2]:.4f}); {el / iters:.3f} s per iteration")
-    return {"members": members, "grains": grains, "iterations": iters, "steps": steps, "ckpt_every": ckpt,
-            "dt": dt, "s_per_iteration": el / iters, "device_s_per_iteration": t_dev / iters,
+    se_fit = [np.exp(x) * mm["g_enabled"] for x, mm in zip(phi, meshes)]
+    rec = np.array([np.linalg.norm(f - t) / max(np.linalg.norm(i0 - t), 1e-300)
+                    for f, t, i0 in zip(se_fit, se_true, se_init)])
+    gb, mu = trace[-1][1], trace[-1][2]
+    log(f"calibration: loss {trace[0][0]:.3e} -> {trace[-1][0]:.3e}, (gamma, mu) -> ({gb:.4f}, {mu:.4f}); "
+        f"SE error left {rec.mean():.3f} of the initial (min {rec.min():.3f}); {el / iters:.3f} s per iteration")
+    return {"members": members, "grains": grains, "iterations": iters, "horizon_steps": horizon,
+            "window_steps": window, "windows": horizon // window, "ckpt_every": ckpt, "dt": dt,
+            "parameters": 2 + grains, "s_per_iteration": el / iters,
+            "loss_curve": [t[0] for t in trace[::5]] + [trace[-1][0]],
             "loss_first": trace[0][0], "loss_last": trace[-1][0],
-            "gamma_mu_last": [trace[-1][1], trace[-1][2]], "gamma_mu_true": [1.1, 0.9],
-            "gamma_times_mu_last": trace[-1][1] * trace[-1][2], "gamma_times_mu_true": 1.1 * 0.9,
-            "workload": f"{members} members, G ~ U[5000, 20000], lognormal SE (mean 0.3, sigma 0.5), "
-                        "20-bin area histograms; Adam on (log gamma, log mu) from (1, 1)"}
+            "gamma_mu_last": [gb, mu], "gamma_mu_true": [1.1, 0.9],
+            "gamma_times_mu_last": gb * mu, "gamma_times_mu_true": 1.1 * 0.9,
+            "gamma_mu_separately_identified": bool(abs(gb - 1.1) < 0.02 and abs(mu - 0.9) < 0.02),
+            "se_error_left_per_member": {"mean": float(rec.mean()), "min": float(rec.min()),
+                                         "max": float(rec.max())},
+            "workload": f"{members} members, G ~ U[5000, 20000], true SE lognormal (mean 0.3, sigma 0.5), "
+                        f"targets: 20-bin area histograms after {horizon} steps at (1.1 gamma, 0.9 mu); "
+                        f"windows of {window} steps ({window - 1} topology-fixed + 1 with T1/T2), "
+                        f"checkpoint every {ckpt}; Adam on log gamma, log mu and log SE of every grain "
+                        "from (1, 1, SE = 0.3)"}
 
 
 def cpu_baseline(opts, mesh, steps, omp=False):
@@ -363,15 +535,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the step as a HIP graph (no per-kernel timing)")
     ap.add_argument("--graph-steps", type=int, default=200, help="extra steps timed as HIP graph replays (0: skip)")
+    ap.add_argument("--scale-steps", type=int, default=200,
+                    help="steps of the anisotropic and the Monte Carlo workloads at full size (0: skip)")
     ap.add_argument("--members", type=int, default=256, help="ensemble members (0: skip the ensemble line)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
-    ap.add_argument("--b2b-steps", type=int, default=50, help="steps per member when stepped back to back")
+    ap.add_argument("--b2b-steps", type=int, default=0,
+                    help="steps per member when stepped back to back (0: the batch's steps)")
     ap.add_argument("--adjoint-grains", type=int, default=10000, help="adjoint window workload (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--adjoint-ckpt", type=int, default=50)
     ap.add_argument("--calib-members", type=int, default=64, help="batched calibration workload (0: skip)")
     ap.add_argument("--calib-iters", type=int, default=50)
-    ap.add_argument("--calib-steps", type=int, default=1000)
+    ap.add_argument("--calib-steps", type=int, default=1000, help="calibration horizon (steps)")
+    ap.add_argument("--calib-window", type=int, default=200, help="steps per topology-fixed window of the horizon")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM ceiling and PMC byte-counter scale)")
     args = ap.parse_args()
@@ -394,6 +570,7 @@ def main():
     log(f"mesh: {mesh.nv} vertex slots, {mesh.ne} boundary slots, {mesh.ng} grain slots ({time.time() - t0:.1f} s)")
     st = sevc.State(opts, mesh)
     tile = st.autotune()
+    vpath, _ = st.velocity_config()
     st.step(args.warmup, graph=bool(args.graph))
     st.enable_timing(2)           # HIP events around velocity / translate / text only
     sync = st.step(0)
@@ -410,9 +587,19 @@ def main():
     def avg_s(name):
         ms, n = kt.get(name, (float("nan"), 0))
         return ms / max(1, n) * 1e-3
-    # the FD velocity is two kernels: pass 1 over grain tiles, pass 2 over vertices
-    g_avg, v_avg, tr_avg = avg_s("vel_grain"), avg_s("velocity"), avg_s("translate")
-    vel_avg = g_avg + v_avg
+    # the FD velocity is one kernel over vertex tiles (path 1) or two:
+    # pass 1 over grain tiles, pass 2 over vertices (path 0)
+    tr_avg = avg_s("translate")
+    if vpath == 1:
+        g_avg, v_avg = float("nan"), float("nan")
+        vel_avg = avg_s("vel_fused")
+        kname = f"FD velocity, one pass over {tile}-vertex tiles (k_vel_vtile)"
+        passes = {"k_vel_vtile": vel_avg * 1e6}
+    else:
+        g_avg, v_avg = avg_s("vel_grain"), avg_s("velocity")
+        vel_avg = g_avg + v_avg
+        kname = f"FD velocity, two passes ({tile}-grain tiles k_vel_grain + k_vel_vertex)"
+        passes = {"k_vel_grain": g_avg * 1e6, "k_vel_vertex": v_avg * 1e6}
     achieved = B_VELOCITY * verts_per_step / vel_avg / 1e9
     log(f"{s.steps_done} steps in {elapsed:.3f} s, {vsteps} vertex-steps; velocity {vel_avg * 1e6:.1f} us, "
         f"translate {tr_avg * 1e6:.1f} us; flips {s.n_flips} T2 {s.n_removals} nucl {s.n_nucleations} "
@@ -433,15 +620,17 @@ def main():
         "config": {"workload": f"plain step, periodic Voronoi of {args.grains} seeds (seed {args.seed}), "
                                "gamma=1, mu=1, lognormal SE (mean 0.3, sigma 0.5), nucleation 1e-4/grain/step, "
                                "adaptive dt, flip threshold 1e-4",
-                   "grains": args.grains, "graph": bool(args.graph), "tile_grains": tile},
+                   "grains": args.grains, "graph": bool(args.graph), "velocity_path": vpath, "tile": tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_VELOCITY_B_PER_VERTEX * verts_per_step
-                                 if args.grains == 1_000_000 and args.seed == 1 else None),
-                     "traffic_unit": "bytes per launch (both passes)",
-                     "kernel": "FD velocity (k_vel_grain + k_vel_vertex)", "bytes_per_vertex": B_VELOCITY,
+                     "traffic": (TRAFFIC_VELOCITY[vpath][0] * verts_per_step
+                                 if args.grains == 1_000_000 and args.seed == 1 and vpath in TRAFFIC_VELOCITY
+                                 else None),
+                     "traffic_unit": "HBM bytes per velocity step (all its launches)",
+                     "traffic_source": TRAFFIC_VELOCITY.get(vpath, (None, None))[1],
+                     "kernel": kname, "bytes_per_vertex": B_VELOCITY,
                      "avg_launch_us": vel_avg * 1e6,
-                     "pass_us": {"k_vel_grain": g_avg * 1e6, "k_vel_vertex": v_avg * 1e6}},
+                     "pass_us": passes},
         "velocity_translation": {"avg_us": (vel_avg + tr_avg) * 1e6,
                                  "achieved_GBs": (B_VELOCITY + B_TRANSLATE) * verts_per_step / (vel_avg + tr_avg) / 1e9},
         "events": {"flips": s.n_flips, "removals": s.n_removals, "nucleations": s.n_nucleations},
@@ -457,13 +646,16 @@ def main():
                                 "ms_per_step": (g1 - g0) / max(1, sg.steps_done) * 1e3,
                                 "steps": sg.steps_done}
     st.close()
+    if args.scale_steps > 0:
+        line["at_scale"] = bench_at_scale(sevc, args.grains, args.scale_steps)
     if args.members > 0:
-        line["ensemble"] = bench_ensemble(sevc, args.members, args.ensemble_steps, args.b2b_steps)
+        line["ensemble"] = bench_ensemble(sevc, args.members, args.ensemble_steps,
+                                          args.b2b_steps or args.ensemble_steps)
     if args.adjoint_grains > 0:
         line["adjoint"] = bench_adjoint(sevc, args.adjoint_grains, 3, args.adjoint_steps, args.adjoint_ckpt, 1e-6)
     if args.calib_members > 0:
-        line["calibration"] = bench_calibration(sevc, args.calib_members, args.calib_iters, args.calib_steps, 100,
-                                                5e-7)
+        line["calibration"] = bench_calibration(sevc, args.calib_members, args.calib_iters, args.calib_steps,
+                                                args.calib_window, 50, 5e-7)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
         line["cpu_baseline_openmp"] = cpu_baseline(opts, mesh, 2 * args.cpu_steps, omp=True)

@@ -97,6 +97,8 @@ typedef struct sevc_options {
     double extra_capacity;     /* spare disabled slots appended, fraction of size */
     int32_t velocity_model;    /* SEVC_VEL_FD / SEVC_VEL_ANALYTIC               */
     int32_t events;            /* 0: topology-fixed (no T1/T2/nucleation)       */
+    int32_t nucleation_gate;   /* MC mode opens once enabled grains <= this count;
+                                  -1: 5*n_grains0/6 (main.cu:137-139)            */
 } sevc_options;
 
 /* ---------------------------------------------------------------- */
@@ -296,11 +298,30 @@ int32_t sevc_n_members(const sevc_state* s);
  * (device_to_host, translation.h:184-199, without freeing the device copy). */
 int sevc_get_mesh(sevc_state* s, int32_t member, sevc_mesh* out);
 
-/* Pick the tile size of the velocity's grain pass (grains per workgroup,
- * 16..64) for this state by timing that pass on the current positions
- * (extension; no counterpart in the reference).  Results do not depend on
- * the choice; the state is left unchanged.  tile_grains may be NULL. */
+/* Pick the FD velocity's path and tile size for this state by timing each
+ * candidate on the current positions (extension; no counterpart in the
+ * reference): path 0, two passes (grain tiles of 16..64 grains, then one
+ * lane per vertex); path 1, one pass over tiles of 32 or 64 vertex slots;
+ * path 2, two passes with one lane per grain, then per vertex (paths 1 and
+ * 2 when the list stride is a multiple of 4, at least 8).  Results do
+ * not depend on the choice, and the state is left unchanged (velocities
+ * and grain areas written by the timed launches are put back).
+ * tile_grains (may be NULL) receives the chosen tile size: grains per tile
+ * for path 0, vertices per tile for path 1. */
 int sevc_autotune(sevc_state* s, int32_t* tile_grains);
+
+/* The velocity path and tile size in use (see sevc_autotune), and a way to
+ * set them (path 0: tile 16..64 grains; path 1: tile 32 or 64 vertices;
+ * path 2: tile ignored, reported as 0).
+ * A new state uses path 0 with 64 grains per tile. */
+int sevc_velocity_config(const sevc_state* s, int32_t* path, int32_t* tile);
+
+/* Longest candidate list that the vote rounds (polling_system,
+ * topological.h:549-578) run in one workgroup's LDS, 0..2048 (default
+ * 2048); longer lists run the general rounds over global memory.  Results
+ * do not depend on it (extension; a tuning and test knob). */
+int sevc_set_poll_limit(sevc_state* s, int32_t max_fast);
+int sevc_set_velocity_config(sevc_state* s, int32_t path, int32_t tile);
 
 /* Copy only the vertex positions of a member (2 per vertex slot). */
 int sevc_get_positions(sevc_state* s, int32_t member, double* pos);
@@ -309,6 +330,14 @@ int sevc_get_positions(sevc_state* s, int32_t member, double* pos);
  * parameter sweeps).  NULL arrays are left unchanged. */
 int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
                            const double* g_SE, double gb_scaling, double mobility);
+
+/* Replace member m's whole mesh (positions, topology, stored energies,
+ * orientations, flags) by a host mesh of the member's slot counts and
+ * stride, e.g. one taken with sevc_get_mesh earlier: a checkpoint restore
+ * (extension; the reference has no resume, io.h:280-377 only writes).  The
+ * mesh must pass check_data (SEVC_E_TOPOLOGY otherwise).  Velocities and
+ * t_ext are taken when given; the member's step counter and time go on. */
+int sevc_set_member_mesh(sevc_state* s, int32_t member, const sevc_mesh* mesh);
 
 /* Gradients of one member from sevc_adjoint_window (extension). */
 typedef struct sevc_grads {
@@ -358,8 +387,9 @@ int sevc_request_diagnostics(sevc_state* s);
  * number of entries written.  Used by the benchmark for roofline figures. */
 int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], double* ms,
                       int64_t* launches);
-/* on: 0 off, 1 every kernel, 2 only the velocity passes ("vel_grain",
- * "velocity"), "translate" and "text". */
+/* on: 0 off, 1 every kernel, 2 only the velocity ("vel_fused" for the
+ * one-pass path, "vel_grain" and "velocity" for the two passes),
+ * "translate" and "text". */
 int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
 
 /* Device self-test of the shared arithmetic: evaluates op on n inputs on

@@ -1349,7 +1349,8 @@ static void one_step(orc* s) {
         int gap = o->nucleation_gap > 0 ? o->nucleation_gap : 1;
         int made = 0, ran = 0;
         if (o->nucleation_mode == SEVC_NUCL_MC) {
-            if (s->curr_grains <= 5 * s->n_grains0 / 6) s->allow_nucl = 1;   /* :137 */
+            int gate = s->o.nucleation_gate >= 0 ? s->o.nucleation_gate : 5 * s->n_grains0 / 6;
+            if (s->curr_grains <= gate) s->allow_nucl = 1;   /* :137 */
             if ((s->n % gap) == 0 && s->allow_nucl) { made = nucleation_mc(s); ran = 1; }
         } else if ((s->n % gap) == 0) {
             made = nucleation_rate(s);

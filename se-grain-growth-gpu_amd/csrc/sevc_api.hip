@@ -23,7 +23,10 @@ __global__ void k_edge_len_energy(Dev d, int only_nucl);
 __global__ void k_grain_area(Dev d, int only_nucl);
 __global__ void k_vertex_energy(Dev d);
 __global__ void k_vel_grain(Dev d, int tuning);
-__global__ void k_vel_vertex(Dev d);
+__global__ void k_vel_vertex(Dev d, int tuning);
+template <int VT> __global__ void k_vel_vtile(Dev d, int tuning);
+__global__ void k_vel_glane(Dev d, int tuning);
+__global__ void k_vel_bigrain(Dev d, int tuning);
 __global__ void k_velocity_generic(Dev d);
 __global__ void k_velocity_fallback(Dev d);
 __global__ void k_velocity_analytic(Dev d);
@@ -92,9 +95,11 @@ struct sevc_state {
     bool ben_dirty = true;                   // boundary energies not computed since creation / a new scale
     bool diag_next = false;
     std::vector<MState> ms_host, ms_prev;
-    // graph replay
+    // graph replay: gexec holds kGraphSteps steps, gexec1 one step
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec1 = nullptr;
+    hipGraph_t graph1 = nullptr;
     // kernel timing
     bool timing = false;
     bool timing_hot_only = false;
@@ -105,6 +110,13 @@ struct sevc_state {
     // grains per tile of velocity pass 1 (sevc_autotune), table capacity
     int tile_grains = kTile;
     size_t tile_cap = 0;
+    // velocity path: 0 two passes over grain tiles and vertices, 1 one pass
+    // over tiles of vtile_size vertices, 2 two passes with one lane per
+    // grain then per vertex (1 and 2 need a list stride S that is a
+    // multiple of 4, at least 8)
+    int vel_path = 0;
+    int vtile_size = 64;
+    size_t vtile_cap = 0;
     // topology-fixed window: checkpoints of x every K steps (ck), the
     // positions and areas of one segment (seg, segA; `holds` = its index),
     // and the reverse-pass arrays
@@ -123,6 +135,20 @@ struct sevc_state {
 };
 
 namespace {
+
+// Captured steps hold table sizes and parameters: dropped when they change.
+void drop_graphs(sevc_state* s) {
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (s->gexec1) { (void)hipGraphExecDestroy(s->gexec1); s->gexec1 = nullptr; }
+    if (s->graph1) { (void)hipGraphDestroy(s->graph1); s->graph1 = nullptr; }
+}
+
+// A failed sevc_batch_create: the reason goes to sevc_last_error(NULL).
+int create_fail(int code, const std::string& msg) {
+    set_stateless_error(std::string("sevc_batch_create: ") + msg);
+    return code;
+}
 
 int fail(sevc_state* s, int code, const std::string& msg) {
     if (s) s->err = msg;
@@ -163,6 +189,8 @@ void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args
     bool timed = s->timing;
     if (timed && s->timing_hot_only)
         timed = std::strcmp(name, "vel_grain") == 0 || std::strcmp(name, "velocity") == 0 ||
+                std::strcmp(name, "vel_fused") == 0 || std::strcmp(name, "vel_glane") == 0 ||
+                std::strcmp(name, "vel_bigrain") == 0 ||
                 std::strcmp(name, "translate") == 0 || std::strcmp(name, "text") == 0;
     if (timed) {
         KTimer t{name, take_event(s), take_event(s)};
@@ -188,6 +216,7 @@ void collect_timers(sevc_state* s) {
 }
 
 constexpr int kListGrid = 512;
+constexpr int kGraphSteps = 8;           // steps per replay of the captured graph
 int count_grid(const Dev& d) { return (d.ngb + d.neb + d.nvb + kCountWaves - 1) / kCountWaves; }
 // one wave per flagged grain in the fix stages: enough workgroups that
 // every grain of a busy step has its own
@@ -211,9 +240,88 @@ int set_tiles(sevc_state* s, int T) {
     s->d.ntile = static_cast<int>(tt.size());
     s->tile_grains = T;
     // a captured step holds the old table size
-    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return SEVC_OK;
+}
+
+constexpr int kVTileMin = 32;            // smallest vertex tile of the one-pass velocity
+
+bool vtile_ok(const sevc_state* s) { return s->S % 4 == 0 && s->S >= 8; }
+
+// Enabled vertices in tiles of VT vertex slots whose polygons do not fit the
+// one-pass kernel's staging (more than 7 VT list entries or 2 VT distinct
+// grains; k_vel_vtile sends such tiles to the two-pass path), from host
+// copies of the vertex grains (global ids), enable flags and list lengths.
+// Meshes whose vertex ids follow space (the generator's) have none; a mesh
+// read in arbitrary vertex order has most.
+long long vtile_overflow(const sevc_state* s, int VT, const std::vector<int>& vg, const std::vector<uint8_t>& ven,
+                         const std::vector<int>& glen) {
+    long long bad = 0;
+    std::vector<int> gs;
+    for (int m = 0; m < s->B; ++m)
+        for (int b = 0; b < s->nv[m]; b += VT) {
+            gs.clear();
+            int nen = 0;
+            for (int i = s->voff[m] + b; i < s->voff[m] + std::min(b + VT, s->nv[m]); ++i) {
+                if (!ven[i]) continue;
+                ++nen;
+                for (int j = 0; j < 3; ++j) gs.push_back(vg[3 * static_cast<size_t>(i) + j]);
+            }
+            std::sort(gs.begin(), gs.end());
+            gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+            long long pts = 0;
+            for (int g : gs) pts += g >= 0 && g < s->G ? glen[g] : 0;
+            if (pts > 7LL * VT || static_cast<long long>(gs.size()) > 2LL * VT) bad += nen;
+        }
+    return bad;
+}
+
+// The one-pass tile size whose staging overflows for at most 1 % of the
+// enabled vertices (64 preferred), or 0 when neither fits (two-pass path).
+int pick_vtile(const sevc_state* s, const std::vector<int>& vg, const std::vector<uint8_t>& ven,
+               const std::vector<int>& glen) {
+    if (!vtile_ok(s)) return 0;
+    long long nen = 0;
+    for (uint8_t e : ven) nen += e ? 1 : 0;
+    for (int VT : {64, 32})
+        if (100 * vtile_overflow(s, VT, vg, ven, glen) <= nen) return VT;
+    return 0;
+}
+
+// Tile table of the one-pass velocity: VT consecutive vertex slots of one
+// member per tile (VT = 32 or 64).  Results do not depend on the tiling.
+int set_vtiles(sevc_state* s, int VT) {
+    if (VT != 32 && VT != 64) return SEVC_E_INVALID_ARG;
+    std::vector<Blk> tt;
+    for (int m = 0; m < s->B; ++m)
+        for (int b = 0; b < s->nv[m]; b += VT)
+            tt.push_back(Blk{m, s->voff[m] + b, s->voff[m] + std::min(b + VT, s->nv[m])});
+    if (tt.size() > s->vtile_cap) return SEVC_E_INVALID_ARG;
+    if (hipMemcpy(const_cast<Blk*>(s->d.vtile), tt.data(), tt.size() * sizeof(Blk), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return SEVC_E_HIP;
+    s->d.nvtile = static_cast<int>(tt.size());
+    s->vtile_size = VT;
+    drop_graphs(s);
+    return SEVC_OK;
+}
+
+// The FD velocity of the active path (both are bit-identical).
+constexpr int kBigGrid = 256;            // one wave per long grain (lane-per-grain path)
+
+void launch_velocity(sevc_state* s, const Dev& d) {
+    if (s->vel_path == 2) {
+        launch(s, "vel_glane", k_vel_glane, d.ngb, kBlock, d, 0);
+        launch(s, "vel_bigrain", k_vel_bigrain, kBigGrid, 64, d, 0);
+        launch(s, "velocity", k_vel_vertex, d.nvb, kBlock, d, 0);
+    } else if (s->vel_path == 1) {
+        if (s->vtile_size == 32) launch(s, "vel_fused", k_vel_vtile<32>, d.nvtile, 128, d, 0);
+        else launch(s, "vel_fused", k_vel_vtile<64>, d.nvtile, 256, d, 0);
+    } else {
+        launch(s, "vel_grain", k_vel_grain, d.ntile, 256, d, 0);
+        launch(s, "velocity", k_vel_vertex, d.nvb, kBlock, d, 0);
+    }
+    launch(s, "velocity_fallback", k_velocity_fallback, 64, 256, d);
 }
 
 // The device check of check_data over the state's slots; gate = 1 keeps
@@ -264,9 +372,7 @@ void launch_step(sevc_state* s, bool diag) {
     if (diag) {
         launch(s, "velocity_generic", k_velocity_generic, d.nvb, kBlock, d);
     } else {
-        launch(s, "vel_grain", k_vel_grain, d.ntile, 256, d, 0);
-        launch(s, "velocity", k_vel_vertex, d.nvb, kBlock, d);
-        launch(s, "velocity_fallback", k_velocity_fallback, 64, 256, d);
+        launch_velocity(s, d);
     }
     if (s->any_analytic) launch(s, "velocity_analytic", k_velocity_analytic, d.nvb, kBlock, d);
     if (diag) launch(s, "grain_dadt", k_grain_dadt, d.ngb, kBlock, d);
@@ -345,16 +451,19 @@ const char* sevc_last_error(const sevc_state* s) { return s ? s->err.c_str() : s
 int32_t sevc_n_members(const sevc_state* s) { return s ? s->B : 0; }
 
 int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t B, sevc_state** out) {
-    if (!opts || !meshes || B <= 0 || !out) return SEVC_E_INVALID_ARG;
+    if (!opts || !meshes || B <= 0 || !out) return create_fail(SEVC_E_INVALID_ARG, "null arguments or no members");
     *out = nullptr;
     // enumerated options and the domain: an unknown velocity model would
     // leave the velocities unwritten, an unknown nucleation mode would run
     // as another one
     for (int m = 0; m < B; ++m) {
         const sevc_options& o = opts[m];
-        if (o.velocity_model != SEVC_VEL_FD && o.velocity_model != SEVC_VEL_ANALYTIC) return SEVC_E_INVALID_ARG;
-        if (o.nucleation_mode != SEVC_NUCL_MC && o.nucleation_mode != SEVC_NUCL_RATE) return SEVC_E_INVALID_ARG;
-        if (!(o.domain_bound > 0) || !(meshes[m].domain_bound > 0)) return SEVC_E_INVALID_ARG;
+        if (o.velocity_model != SEVC_VEL_FD && o.velocity_model != SEVC_VEL_ANALYTIC)
+            return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": unknown velocity_model");
+        if (o.nucleation_mode != SEVC_NUCL_MC && o.nucleation_mode != SEVC_NUCL_RATE)
+            return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": unknown nucleation_mode");
+        if (!(o.domain_bound > 0) || !(meshes[m].domain_bound > 0))
+            return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": domain_bound not positive");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SEVC_E_NO_DEVICE;
@@ -368,7 +477,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
             !me.pos || !me.v_bnd || !me.v_grn || !me.v_enabled || !me.b_vtx || !me.b_enabled || !me.g_vtx ||
             !me.g_vlen || !me.g_enabled || !me.g_SE || !me.g_orientation) {
             delete s;
-            return SEVC_E_INVALID_ARG;
+            return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": empty mesh or missing arrays");
         }
         // Kernels index through every stored id of an enabled entity, so
         // an inconsistent input is rejected here (utils.h:85-336 checks).
@@ -382,7 +491,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         s->voff.push_back(s->V); s->eoff.push_back(s->E); s->goff.push_back(s->G);
         s->nv.push_back(me.n_vertices); s->ne.push_back(me.n_boundaries); s->ng.push_back(me.n_grains);
         s->V += me.n_vertices; s->E += me.n_boundaries; s->G += me.n_grains;
-        if (opts[m].dt <= 0) { delete s; return SEVC_E_INVALID_ARG; }
+        if (opts[m].dt <= 0) { delete s; return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": dt <= 0"); }
         if (opts[m].events && opts[m].do_nucleation) {
             s->any_nucl = true;
             if (opts[m].nucleation_mode == SEVC_NUCL_MC) s->any_mc = true;
@@ -394,7 +503,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         if (opts[m].GB_eps != 0.0) s->any_aniso = true;
         if (opts[m].debug_lvl >= 1) s->any_debug = true;
     }
-    if (s->S > 128) { delete s; return SEVC_E_INVALID_ARG; }
+    if (s->S > 128) { delete s; return create_fail(SEVC_E_INVALID_ARG, "grain list stride above 128"); }
     const size_t V = static_cast<size_t>(s->V), E = static_cast<size_t>(s->E), G = static_cast<size_t>(s->G);
     const size_t S = static_cast<size_t>(s->S);
     Dev& d = s->d;
@@ -404,6 +513,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     d.E = s->E;
     d.G = s->G;
     d.debug = s->any_debug ? 1 : 0;
+    d.poll_fast = kPollFast;
     int rc = SEVC_OK;
 #define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
     DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
@@ -423,6 +533,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.l_dirty, G); DA(d.l_nucl, G); DA(d.l_conf, 4 * E + 16);
     DA(d.nscr, 12 * G + 16);
     DA(d.l_vfall, V + 16);
+    DA(d.l_bigr, G + 16);
     DA(d.nrec, 4 * G + 16); DA(d.nal, G + 16); DA(d.l_nev, 8 * G + 16);
 
     // block tables
@@ -448,7 +559,17 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         s->tile_cap = cap;
         if ((rc = set_tiles(s, kTile)) != SEVC_OK) {
             sevc_destroy(s);
-            return rc;
+            return create_fail(rc, "grain tile table");
+        }
+        size_t vcap = 0;
+        for (int m = 0; m < B; ++m) vcap += (s->nv[m] + kVTileMin - 1) / kVTileMin;
+        Blk* dvt2;
+        DA(dvt2, vcap);
+        d.vtile = dvt2;
+        s->vtile_cap = vcap;
+        if ((rc = set_vtiles(s, 64)) != SEVC_OK) {
+            sevc_destroy(s);
+            return create_fail(rc, "vertex tile table (64)");
         }
     }
     DA(d.vterm, 6 * V);
@@ -510,7 +631,10 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         for (int g = 0; g < me.n_grains; ++g) {
             const size_t I = static_cast<size_t>(go + g);
             glen[I] = me.g_vlen[g];
-            if (glen[I] > Sm || glen[I] < 0) { sevc_destroy(s); return SEVC_E_INVALID_ARG; }
+            if (glen[I] > Sm || glen[I] < 0) {
+                sevc_destroy(s);
+                return create_fail(SEVC_E_INVALID_ARG, "member " + std::to_string(m) + ": grain list longer than its stride");
+            }
             for (int k = 0; k < glen[I]; ++k) gvx[I * S + k] = gid(me.g_vtx[static_cast<size_t>(g) * Sm + k], vo);
             genab[I] = me.g_enabled[g] ? 1 : 0;
             ngr += genab[I];
@@ -530,6 +654,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         P.debug = o.debug_lvl;
         P.cap = me.max_vrt_per_grn;
         P.n_grains0 = ngr;
+        P.gate_grains = o.nucleation_gate >= 0 ? o.nucleation_gate : 5 * ngr / 6;
         P.min_grains = o.leave_percent < 0 ? -1 : static_cast<int>(std::floor(o.leave_percent * ngr));
         P.voff = vo; P.nv = me.n_vertices; P.eoff = eo; P.ne = me.n_boundaries; P.goff = go; P.ng = me.n_grains;
         MState& M = s->ms_host[static_cast<size_t>(m)];
@@ -539,6 +664,17 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         M.tau_bits = 0x7FF0000000000000ull;
     }
     s->ms_prev = s->ms_host;
+    {
+        // the two-pass velocity (measured the faster at 1M grains); the
+        // one-pass tiling that fits the vertex order is prepared for
+        // sevc_autotune / sevc_set_velocity_config
+        const int VT = pick_vtile(s, vg, ven, glen);
+        if (VT > 0 && (rc = set_vtiles(s, VT)) != SEVC_OK) {
+            sevc_destroy(s);
+            return create_fail(rc, "vertex tile table");
+        }
+        s->vel_path = 0;
+    }
     UP(d.pos, pos.data(), pos.size() * 8); UP(d.vel, vel.data(), vel.size() * 8);
     UP(d.vb, vb.data(), vb.size() * 4); UP(d.vg, vg.data(), vg.size() * 4); UP(d.ven, ven.data(), V);
     UP(d.venergy, venergy.data(), V * 8); UP(d.vparts, vparts.data(), 4 * V * 8);
@@ -592,8 +728,7 @@ int sevc_create(const sevc_options* o, const sevc_mesh* m, sevc_state** out) {
 void sevc_destroy(sevc_state* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    drop_graphs(s);
     for (void* p : s->allocs) (void)hipFree(p);
     for (auto e : s->pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -685,24 +820,33 @@ int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     if (!s->gexec) {
         bool timing = s->timing;
         s->timing = false;
-        // the graph holds the steady step; a step that must recompute the
-        // energies runs outside it
+        // the graphs hold the steady step (kGraphSteps of them, and one); a
+        // step that must recompute the energies runs outside them
         const bool dirty = s->ben_dirty;
         s->ben_dirty = false;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        launch_step(s, false);
+        for (int k = 0; k < kGraphSteps; ++k) launch_step(s, false);
         HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
-        s->ben_dirty = dirty;
         HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        launch_step(s, false);
+        HIPCHK(hipStreamEndCapture(s->stream, &s->graph1));
+        HIPCHK(hipGraphInstantiate(&s->gexec1, s->graph1, nullptr, nullptr, 0));
+        s->ben_dirty = dirty;
         s->timing = timing;
     }
-    for (int k = 0; k < n_steps; ++k) {
+    for (int k = 0; k < n_steps;) {
         if (s->diag_next || s->ben_dirty) {
             const bool diag = s->diag_next;
             s->diag_next = false;
             launch_step(s, diag);
-        } else {
+            k += 1;
+        } else if (n_steps - k >= kGraphSteps) {
             HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+            k += kGraphSteps;
+        } else {
+            HIPCHK(hipGraphLaunch(s->gexec1, s->stream));
+            k += 1;
         }
     }
     int rc = finish_call(s);
@@ -776,30 +920,115 @@ int sevc_get_mesh(sevc_state* s, int32_t m, sevc_mesh* out) {
 int sevc_autotune(sevc_state* s, int32_t* tile_grains) {
     if (!s) return SEVC_E_INVALID_ARG;
     HIPCHK(hipStreamSynchronize(s->stream));
-    static const int cand[] = {16, 24, 32, 48, 64};
+    // candidates: the two-pass path with 16..64 grains per tile of its
+    // first pass, and the one-pass path with 32 or 64 vertices per tile;
+    // each timed over 3 launches of the whole velocity on the current state
+    struct Cand { int path, tile; };
+    std::vector<Cand> cands = {{0, 16}, {0, 24}, {0, 32}, {0, 48}, {0, 64}};
+    if (vtile_ok(s)) {
+        // one-pass tile sizes whose staging fits the current topology
+        // (timing launches skip the two-pass fallback, so an overflowing
+        // tiling would look faster than it is)
+        std::vector<int> vg(3 * static_cast<size_t>(s->V)), glen(s->G);
+        std::vector<uint8_t> ven(s->V);
+        HIPCHK(hipMemcpy(vg.data(), s->d.vg, vg.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(glen.data(), s->d.glen, glen.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ven.data(), s->d.ven, ven.size(), hipMemcpyDeviceToHost));
+        long long nen = 0;
+        for (uint8_t e : ven) nen += e ? 1 : 0;
+        for (int VT : {32, 64})
+            if (100 * vtile_overflow(s, VT, vg, ven, glen) <= nen) cands.push_back({1, VT});
+        cands.push_back({2, 0});
+    }
+    // the timed launches write velocities and grain areas: both are kept
+    // and put back, so the state is left as it was
+    double *keep_vel = nullptr, *keep_area = nullptr;
+    const size_t nvel = 2 * static_cast<size_t>(s->V), narea = static_cast<size_t>(s->G);
+    if (hipMalloc(&keep_vel, nvel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&keep_area, narea * sizeof(double)) != hipSuccess) {
+        (void)hipFree(keep_vel);
+        return fail(s, SEVC_E_NOMEM, "autotune: no room to keep velocities and areas");
+    }
+    HIPCHK(hipMemcpyAsync(keep_vel, s->d.vel, nvel * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(keep_area, s->d.garea, narea * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int best = s->tile_grains;
+    Cand best{s->vel_path, s->vel_path == 1 ? s->vtile_size : (s->vel_path == 2 ? 0 : s->tile_grains)};
     float best_ms = 1e30f;
     int rc = SEVC_OK;
-    for (int T : cand) {
-        if ((rc = set_tiles(s, T)) != SEVC_OK) break;
+    for (const Cand& c : cands) {
+        if ((rc = (c.path == 0 ? set_tiles(s, c.tile) : (c.path == 1 ? set_vtiles(s, c.tile) : SEVC_OK))) != SEVC_OK) break;
         const Dev d = s->d;
-        hipLaunchKernelGGL(k_vel_grain, dim3(d.ntile), dim3(256), 0, s->stream, d, 1);   // warm-up
+        auto once = [&]() {
+            if (c.path == 1) {
+                if (c.tile == 32) hipLaunchKernelGGL(k_vel_vtile<32>, dim3(d.nvtile), dim3(128), 0, s->stream, d, 1);
+                else hipLaunchKernelGGL(k_vel_vtile<64>, dim3(d.nvtile), dim3(256), 0, s->stream, d, 1);
+            } else if (c.path == 2) {
+                (void)hipMemsetAsync(d.cnt + C_BIGR, 0, sizeof(int), s->stream);
+                hipLaunchKernelGGL(k_vel_glane, dim3(d.ngb), dim3(kBlock), 0, s->stream, d, 1);
+                hipLaunchKernelGGL(k_vel_bigrain, dim3(kBigGrid), dim3(64), 0, s->stream, d, 1);
+                hipLaunchKernelGGL(k_vel_vertex, dim3(d.nvb), dim3(kBlock), 0, s->stream, d, 1);
+            } else {
+                hipLaunchKernelGGL(k_vel_grain, dim3(d.ntile), dim3(256), 0, s->stream, d, 1);
+                hipLaunchKernelGGL(k_vel_vertex, dim3(d.nvb), dim3(kBlock), 0, s->stream, d, 1);
+            }
+        };
+        once();   // warm-up
         (void)hipEventRecord(e0, s->stream);
-        for (int r = 0; r < 3; ++r) hipLaunchKernelGGL(k_vel_grain, dim3(d.ntile), dim3(256), 0, s->stream, d, 1);
+        for (int r = 0; r < 3; ++r) once();
         (void)hipEventRecord(e1, s->stream);
         if (hipEventSynchronize(e1) != hipSuccess) { rc = SEVC_E_HIP; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) { best_ms = ms; best = T; }
+        if (ms < best_ms) { best_ms = ms; best = c; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    const bool back = hipMemcpyAsync(s->d.vel, keep_vel, nvel * sizeof(double), hipMemcpyDeviceToDevice,
+                                     s->stream) == hipSuccess &&
+                      hipMemcpyAsync(s->d.garea, keep_area, narea * sizeof(double), hipMemcpyDeviceToDevice,
+                                     s->stream) == hipSuccess &&
+                      hipStreamSynchronize(s->stream) == hipSuccess;
+    (void)hipFree(keep_vel);
+    (void)hipFree(keep_area);
     if (rc != SEVC_OK) return fail(s, rc, "autotune failed");
-    if ((rc = set_tiles(s, best)) != SEVC_OK) return rc;
-    if (tile_grains) *tile_grains = best;
+    if (!back) return fail(s, SEVC_E_HIP, "autotune: restoring velocities and areas failed");
+    if ((rc = sevc_set_velocity_config(s, best.path, best.tile)) != SEVC_OK) return rc;
+    if (tile_grains) *tile_grains = best.tile;
+    return SEVC_OK;
+}
+
+int sevc_set_poll_limit(sevc_state* s, int32_t max_fast) {
+    if (!s || max_fast < 0 || max_fast > kPollFast) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->d.poll_fast = max_fast;
+    drop_graphs(s);
+    return SEVC_OK;
+}
+
+int sevc_velocity_config(const sevc_state* s, int32_t* path, int32_t* tile) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    if (path) *path = s->vel_path;
+    if (tile) *tile = s->vel_path == 1 ? s->vtile_size : (s->vel_path == 2 ? 0 : s->tile_grains);
+    return SEVC_OK;
+}
+
+int sevc_set_velocity_config(sevc_state* s, int32_t path, int32_t tile) {
+    if (!s) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int rc;
+    if (path == 0) {
+        if ((rc = set_tiles(s, tile)) != SEVC_OK) return fail(s, rc, "grain tile size outside 16..64");
+    } else if (path == 1) {
+        if (!vtile_ok(s)) return fail(s, SEVC_E_INVALID_ARG, "one-pass velocity needs a list stride that is a multiple of 4, at least 8");
+        if ((rc = set_vtiles(s, tile)) != SEVC_OK) return fail(s, rc, "vertex tile size must be 32 or 64");
+    } else if (path == 2) {
+        if (!vtile_ok(s)) return fail(s, SEVC_E_INVALID_ARG, "lane-per-grain velocity needs a list stride that is a multiple of 4, at least 8");
+    } else {
+        return fail(s, SEVC_E_INVALID_ARG, "unknown velocity path");
+    }
+    s->vel_path = path;
     return SEVC_OK;
 }
 
@@ -874,6 +1103,100 @@ int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad) {
     return rc;
 }
 
+int sevc_set_member_mesh(sevc_state* s, int32_t m, const sevc_mesh* me) {
+    if (!s || !me || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
+    const int nv = s->nv[m], ne = s->ne[m], ng = s->ng[m], Sm = s->Sm[m];
+    if (me->n_vertices != nv || me->n_boundaries != ne || me->n_grains != ng || me->max_vrt_per_grn != Sm ||
+        !me->pos || !me->v_bnd || !me->v_grn || !me->v_enabled || !me->b_vtx || !me->b_enabled || !me->g_vtx ||
+        !me->g_vlen || !me->g_enabled || !me->g_SE || !me->g_orientation)
+        return fail(s, SEVC_E_INVALID_ARG, "mesh sizes or arrays differ from the member's");
+    double total_area = 0.0;
+    if (sevc_check_mesh(me, &total_area) != 0) return fail(s, SEVC_E_TOPOLOGY, "mesh fails check_data");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const Dev& d = s->d;
+    const int vo = s->voff[m], eo = s->eoff[m], go = s->goff[m];
+    const size_t S = static_cast<size_t>(s->S);
+    auto gid = [](int x, int off) { return x < 0 ? -1 : x + off; };
+    std::vector<double> vel(2 * static_cast<size_t>(nv), 0.0), ben(ne), blen(ne, 0.0), bt(ne, SEVC_MAX_T_EXT),
+        garea(ng, 0.0), gdadt(ng, 0.0);
+    std::vector<int> vb(3 * static_cast<size_t>(nv)), vg(3 * static_cast<size_t>(nv)), bvx(2 * static_cast<size_t>(ne)),
+        gvx(static_cast<size_t>(ng) * S, -1), glen(ng), conf(ne, -1);
+    std::vector<uint8_t> ven(nv), benab(ne), bcand(ne, 0), genab(ng), gtype(ng, 0);
+    int ngr = 0, nve = 0;
+    for (int i = 0; i < nv; ++i) {
+        if (me->vel) { vel[2 * i] = me->vel[2 * i]; vel[2 * i + 1] = me->vel[2 * i + 1]; }
+        for (int k = 0; k < 3; ++k) {
+            vb[3 * i + k] = gid(me->v_bnd[3 * i + k], eo);
+            vg[3 * i + k] = gid(me->v_grn[3 * i + k], go);
+        }
+        ven[i] = me->v_enabled[i] ? 1 : 0;
+        nve += ven[i];
+    }
+    for (int b = 0; b < ne; ++b) {
+        bvx[2 * b] = gid(me->b_vtx[2 * b], vo);
+        bvx[2 * b + 1] = gid(me->b_vtx[2 * b + 1], vo);
+        benab[b] = me->b_enabled[b] ? 1 : 0;
+        ben[b] = me->b_energy ? me->b_energy[b] : 0.0;
+        if (me->b_arclength) blen[b] = me->b_arclength[b];
+        if (me->b_t_ext) bt[b] = me->b_t_ext[b];
+    }
+    for (int g = 0; g < ng; ++g) {
+        glen[g] = me->g_vlen[g];
+        if (glen[g] > Sm || glen[g] < 0) return fail(s, SEVC_E_INVALID_ARG, "grain list longer than the stride");
+        for (int k = 0; k < glen[g]; ++k) gvx[static_cast<size_t>(g) * S + k] = gid(me->g_vtx[static_cast<size_t>(g) * Sm + k], vo);
+        genab[g] = me->g_enabled[g] ? 1 : 0;
+        ngr += genab[g];
+        if (me->g_area) garea[g] = me->g_area[g];
+        if (me->g_type) gtype[g] = me->g_type[g];
+    }
+    std::vector<ModRec> mods(nv);
+    for (ModRec& r : mods)
+        for (int j = 0; j < 4; j++) { r.grn[j] = -1; r.vrt[j] = -1; r.prev[j] = -1; r.fix[j] = FIX_NONE; }
+    bool ok = true;
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        ok = ok && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    up(d.pos + 2 * static_cast<size_t>(vo), me->pos, 16 * static_cast<size_t>(nv));
+    up(d.vel + 2 * static_cast<size_t>(vo), vel.data(), 16 * static_cast<size_t>(nv));
+    up(d.vb + 3 * static_cast<size_t>(vo), vb.data(), 12 * static_cast<size_t>(nv));
+    up(d.vg + 3 * static_cast<size_t>(vo), vg.data(), 12 * static_cast<size_t>(nv));
+    up(d.ven + vo, ven.data(), static_cast<size_t>(nv));
+    up(d.mods + vo, mods.data(), sizeof(ModRec) * static_cast<size_t>(nv));
+    up(d.bvx + 2 * static_cast<size_t>(eo), bvx.data(), 8 * static_cast<size_t>(ne));
+    up(d.ben + eo, ben.data(), 8 * static_cast<size_t>(ne));
+    up(d.blen + eo, blen.data(), 8 * static_cast<size_t>(ne));
+    up(d.bt + eo, bt.data(), 8 * static_cast<size_t>(ne));
+    up(d.benab + eo, benab.data(), static_cast<size_t>(ne));
+    up(d.bcand + eo, bcand.data(), static_cast<size_t>(ne));
+    up(d.conf + eo, conf.data(), 4 * static_cast<size_t>(ne));
+    up(d.gvx + static_cast<size_t>(go) * S, gvx.data(), 4 * gvx.size());
+    up(d.glen + go, glen.data(), 4 * static_cast<size_t>(ng));
+    up(d.garea + go, garea.data(), 8 * static_cast<size_t>(ng));
+    up(d.gori + go, me->g_orientation, 8 * static_cast<size_t>(ng));
+    up(d.gse + go, me->g_SE, 8 * static_cast<size_t>(ng));
+    up(d.genab + go, genab.data(), static_cast<size_t>(ng));
+    up(d.gtype + go, gtype.data(), static_cast<size_t>(ng));
+    ok = ok && hipMemset(d.gfix + go, 0, 4 * static_cast<size_t>(ng)) == hipSuccess &&
+         hipMemset(d.gcand + go, 0, static_cast<size_t>(ng)) == hipSuccess &&
+         hipMemset(d.vnuc + vo, 0, static_cast<size_t>(nv)) == hipSuccess;
+    if (!ok) return fail(s, SEVC_E_HIP, "upload of the member's mesh failed");
+    // loop state: counts of the new mesh; the step counter and time go on
+    MState M;
+    HIPCHK(hipMemcpy(&M, d.ms + m, sizeof(MState), hipMemcpyDeviceToHost));
+    M.curr_grains = ngr;
+    M.n_venabled = nve;
+    HIPCHK(hipMemcpy(d.ms + m, &M, sizeof(MState), hipMemcpyHostToDevice));
+    s->ms_host[static_cast<size_t>(m)].curr_grains = ngr;
+    s->ms_host[static_cast<size_t>(m)].n_venabled = nve;
+    hipLaunchKernelGGL(k_count_all, dim3(count_grid(d)), dim3(64 * kCountWaves), 0, s->stream, d);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    s->ben_dirty = true;          // energies of the restored topology at the next step
+    s->win.ready = false;
+    drop_graphs(s);
+    return SEVC_OK;
+}
+
 int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const double* g_SE, double gb_scaling,
                            double mobility) {
     if (!s || m < 0 || m >= s->B) return SEVC_E_INVALID_ARG;
@@ -888,8 +1211,7 @@ int sevc_set_member_params(sevc_state* s, int32_t m, const double* pos, const do
     if (mobility > 0) { P.mobility = mobility; s->opts[m].mobility = mobility; }
     HIPCHK(hipMemcpy(const_cast<Params*>(d.prm) + m, &P, sizeof(Params), hipMemcpyHostToDevice));
     if (gb_scaling > 0) s->ben_dirty = true;      // energies at the new scale on the next step
-    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     s->win.ready = false;
     return SEVC_OK;
 }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "sevc_state.h"
 
 namespace sevc {
@@ -31,5 +33,6 @@ __global__ void k_check_stop(const Params* prm, MState* ms, const int* bad, int 
 
 // Message of the last failed call made without a state on this thread.
 const char* stateless_error();
+void set_stateless_error(const std::string& msg);
 
 }  // namespace sevc

@@ -212,6 +212,7 @@ int stateless_fail(int code, const std::string& msg) {
 }  // namespace
 
 const char* stateless_error() { return g_stateless_err.c_str(); }
+void set_stateless_error(const std::string& msg) { g_stateless_err = msg; }
 
 }  // namespace sevc
 

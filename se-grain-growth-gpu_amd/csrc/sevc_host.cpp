@@ -71,6 +71,7 @@ void sevc_options_default(sevc_options* o) {
     o->extra_capacity = 0.0;
     o->velocity_model = SEVC_VEL_FD;
     o->events = 1;
+    o->nucleation_gate = -1;
 }
 
 // load_config_file, options.h:69-206.  Lines starting with '#' or '[' are
@@ -129,6 +130,7 @@ int sevc_options_from_ini(const char* path, sevc_options* o) {
         else if (key == "extra_capacity") o->extra_capacity = std::atof(v);
         else if (key == "velocity_model") o->velocity_model = (std::strcmp(v, "analytic") == 0) ? SEVC_VEL_ANALYTIC : SEVC_VEL_FD;
         else if (key == "events") o->events = std::atoi(v);
+        else if (key == "nucleation_gate") o->nucleation_gate = std::atoi(v);
     }
     std::fclose(f);
     // options.h:172-181

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d) {
         if (P.events && P.do_nucl) {
             int gap = P.gap > 0 ? P.gap : 1;
             if (P.nucl_mode == SEVC_NUCL_MC) {
-                if (M.curr_grains <= 5 * P.n_grains0 / 6) M.allow_nucl = 1;
+                if (M.curr_grains <= P.gate_grains) M.allow_nucl = 1;   // main.cu:137-139
                 M.nucl_now = ((M.n % gap) == 0 && M.allow_nucl) ? 1 : 0;
             } else {
                 M.nucl_now = ((M.n % gap) == 0) ? 1 : 0;
@@ -420,13 +420,270 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     }
 }
 
+// Velocity pass 1 with one lane per grain (an alternative to the tiles of
+// k_vel_grain; both feed k_vel_vertex): the lane loads the grain's length,
+// SE and first kGL list entries together, then its vertices' positions and
+// grain triples; forms the unwrap deltas, the grain's area
+// (compute_grain_areas, calculus.h:224-234) and, for every slot k, the two
+// shifted areas of energy_diff (calculus.h:39-61) from the unshifted
+// track's prefix at k - 1 (the shifted polygons' tracks equal it up to
+// there); writes SE*(|A'| - |A|) for x and y into the vertex's term slot of
+// this grain.  No LDS, no barrier: every array index is a compile-time
+// constant, so the polygon lives in registers.  Grains longer than kGL go
+// to a list for k_vel_bigrain.
+constexpr int kGL = 8;
+
+// One shifted pair (x, y) of areas of a polygon in registers for slot k:
+// prefix (ux, uy) = U_{k-1}, a_pre = the sum of U_i x U_{i+1} for i <= k-2.
+template <int K>
+__device__ __forceinline__ void slot_areas(const double2 (&Pq)[kGL], const double (&dx)[kGL],
+                                           const double (&dy)[kGL], int n, double D, double ux, double uy,
+                                           double a_pre, double* ax_out, double* ay_out) {
+    const double px = dom_mod(Pq[K].x + SEVC_GRAD_EPS, D), qy = dom_mod(Pq[K].y + SEVC_GRAD_EPS, D);
+    double ax = a_pre, ay = a_pre, xX, yY;
+    if (K == 0) {
+        xX = 0.0;
+        yY = 0.0;
+    } else {
+        const double ex = wrapd(px, Pq[K - 1].x, D), ey = wrapd(qy, Pq[K - 1].y, D);
+        const double nx = ux + dx[K], ny = uy + dy[K];
+        xX = ux + ex;
+        yY = uy + ey;
+        ax += ux * ny - uy * xX;
+        ay += ux * yY - uy * nx;
+        ux = nx;
+        uy = ny;
+    }
+    if (K + 1 < kGL && K + 1 < n) {
+        const double ex1 = wrapd(Pq[K + 1].x, px, D), ey1 = wrapd(Pq[K + 1].y, qy, D);
+        const double nx = ux + dx[K + 1], ny = uy + dy[K + 1];
+        const double nxX = xX + ex1, nyY = yY + ey1;
+        ax += xX * ny - uy * nxX;
+        ay += ux * nyY - yY * nx;
+        ux = nx; uy = ny; xX = nxX; yY = nyY;
+#pragma unroll
+        for (int i = K + 2; i < kGL; i++) {
+            if (i < n) {
+                const double nx2 = ux + dx[i], ny2 = uy + dy[i];
+                const double nxX2 = xX + dx[i], nyY2 = yY + dy[i];
+                ax += xX * ny2 - uy * nxX2;
+                ay += ux * nyY2 - yY * nx2;
+                ux = nx2; uy = ny2; xX = nxX2; yY = nyY2;
+            }
+        }
+    }
+    const double z = 0;
+    ax += xX * z - uy * z;
+    ay += ux * z - yY * z;
+    *ax_out = ax * 0.5;
+    *ay_out = ay * 0.5;
+}
+
+template <int K>
+__device__ __forceinline__ void slot_terms(const Dev& d, int g, const int (&L)[kGL], const int (&jq)[kGL],
+                                           const double2 (&Pq)[kGL], const double (&dx)[kGL],
+                                           const double (&dy)[kGL], int n, double D, double a0, double se,
+                                           double& ux, double& uy, double& a_pre) {
+    if (K >= n) return;
+    double ax, ay;
+    slot_areas<K>(Pq, dx, dy, n, D, ux, uy, a_pre, &ax, &ay);
+    const double tx = se * (fabs(ax) - a0), ty = se * (fabs(ay) - a0);
+    const int v = L[K], j = jq[K];
+    if (j >= 0) reinterpret_cast<double2*>(d.vterm)[3 * (size_t)v + j] = make_double2(tx, ty);
+    // advance the unshifted prefix to (U_K, S_{K-1})
+    if (K > 0) {
+        const double nx = ux + dx[K], ny = uy + dy[K];
+        a_pre += ux * ny - uy * nx;
+        ux = nx;
+        uy = ny;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_vel_glane(Dev d, int tuning) {
+    TABLE_BLOCK(d.gblk, d.ngb)
+    const bool in = i < bd_.e;
+    uint8_t en = 0;
+    int n = 0;
+    double se = 0.0;
+    int4 r0 = make_int4(-1, -1, -1, -1), r1 = r0;
+    if (in) {
+        en = d.genab[i];
+        n = d.glen[i];
+        se = d.gse[i];
+        const int4* row = reinterpret_cast<const int4*>(glist(d, i));
+        r0 = row[0];
+        r1 = row[1];
+    }
+    if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD || !en) return;
+    if (n > kGL) {
+        const int s = atomicAdd(&d.cnt[C_BIGR], 1);
+        if (s < d.G) d.l_bigr[s] = i;
+        return;
+    }
+    const double D = P.D;
+    const int L[kGL] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    const double2* X = reinterpret_cast<const double2*>(d.pos);
+    double2 Pq[kGL];
+    int jq[kGL];
+#pragma unroll
+    for (int q = 0; q < kGL; q++) Pq[q] = q < n ? X[L[q]] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < kGL; q++) {
+        int j = -1;
+        if (q < n) {
+            const int v = L[q];
+            const int a = d.vg[3 * v], b = d.vg[3 * v + 1], c = d.vg[3 * v + 2];
+            j = (a == i) ? 0 : ((b == i) ? 1 : ((c == i) ? 2 : -1));
+        }
+        jq[q] = j;
+    }
+    double dx[kGL], dy[kGL];
+    dx[0] = 0.0;
+    dy[0] = 0.0;
+#pragma unroll
+    for (int q = 1; q < kGL; q++) {
+        dx[q] = q < n ? wrapd(Pq[q].x, Pq[q - 1].x, D) : 0.0;
+        dy[q] = q < n ? wrapd(Pq[q].y, Pq[q - 1].y, D) : 0.0;
+    }
+    // compute_area (geometry.h:757-765 after :908-931): U_q = U_{q-1} + D_q
+    double ux = 0.0, uy = 0.0, a = 0.0;
+#pragma unroll
+    for (int q = 1; q < kGL; q++) {
+        if (q < n) {
+            const double nx = ux + dx[q], ny = uy + dy[q];
+            a += ux * ny - uy * nx;
+            ux = nx;
+            uy = ny;
+        }
+    }
+    const double z = 0;
+    a += ux * z - uy * z;
+    a *= 0.5;
+    if (n <= 0) a = 0.0;
+    if (!tuning) d.garea[i] = a;
+    const double a0 = fabs(a);
+    double pux = 0.0, puy = 0.0, pre = 0.0;
+    slot_terms<0>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<1>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<2>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<3>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<4>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<5>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<6>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+    slot_terms<7>(d, i, L, jq, Pq, dx, dy, n, D, a0, se, pux, puy, pre);
+}
+
+// Velocity pass 1 for the grains k_vel_glane listed (more than kGL
+// vertices): one wave per grain.  The list, positions and the vertices'
+// grain slots are staged in LDS, lane 0 forms the area and the prefix of
+// the unshifted track, then lane k computes slot k's two shifted areas from
+// the prefix at k - 1 (as slot_areas does) and writes its terms.
+__global__ void __launch_bounds__(64) k_vel_bigrain(Dev d, int tuning) {
+    __shared__ double2 Ps[kMaxStride], Ds[kMaxStride], Us[kMaxStride];
+    __shared__ double Ss[kMaxStride];
+    __shared__ int Ls[kMaxStride];
+    __shared__ double a_s;
+    const int n_items = min(d.cnt[C_BIGR], d.G);
+    const int lane = threadIdx.x;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int g = d.l_bigr[item];
+        const Params& P = d.prm[member_of_grain(d, g)];
+        const double D = P.D;
+        const int n = min(d.glen[g], kMaxStride);
+        const double se = d.gse[g];
+        const int* Lg = glist(d, g);
+        for (int k = lane; k < n; k += 64) {
+            Ls[k] = Lg[k];
+            Ps[k] = reinterpret_cast<const double2*>(d.pos)[Ls[k]];
+        }
+        __syncthreads();
+        for (int k = lane; k < n; k += 64)
+            Ds[k] = k > 0 ? make_double2(wrapd(Ps[k].x, Ps[k - 1].x, D), wrapd(Ps[k].y, Ps[k - 1].y, D))
+                          : make_double2(0.0, 0.0);
+        __syncthreads();
+        if (lane == 0) {
+            // Us[k] = U_k, Ss[k] = t_0 + ... + t_k (left to right)
+            double ux = 0.0, uy = 0.0, a = 0.0;
+            Us[0] = make_double2(0.0, 0.0);
+            for (int k = 1; k < n; k++) {
+                const double nx = ux + Ds[k].x, ny = uy + Ds[k].y;
+                a += ux * ny - uy * nx;
+                Ss[k - 1] = a;
+                ux = nx;
+                uy = ny;
+                Us[k] = make_double2(ux, uy);
+            }
+            const double z = 0;
+            a += ux * z - uy * z;
+            a *= 0.5;
+            a_s = a;
+            if (!tuning) d.garea[g] = a;
+        }
+        __syncthreads();
+        const double a0 = fabs(a_s);
+        for (int k = lane; k < n; k += 64) {
+            const double px = dom_mod(Ps[k].x + SEVC_GRAD_EPS, D), qy = dom_mod(Ps[k].y + SEVC_GRAD_EPS, D);
+            double ux = 0.0, uy = 0.0, ax = 0.0, ay = 0.0, xX = 0.0, yY = 0.0;
+            if (k > 0) {
+                ux = Us[k - 1].x;
+                uy = Us[k - 1].y;
+                ax = ay = (k >= 2) ? Ss[k - 2] : 0.0;
+                const double ex = wrapd(px, Ps[k - 1].x, D), ey = wrapd(qy, Ps[k - 1].y, D);
+                const double nx = ux + Ds[k].x, ny = uy + Ds[k].y;
+                xX = ux + ex;
+                yY = uy + ey;
+                ax += ux * ny - uy * xX;
+                ay += ux * yY - uy * nx;
+                ux = nx;
+                uy = ny;
+            }
+            if (k + 1 < n) {
+                const double ex1 = wrapd(Ps[k + 1].x, px, D), ey1 = wrapd(Ps[k + 1].y, qy, D);
+                const double nx = ux + Ds[k + 1].x, ny = uy + Ds[k + 1].y;
+                const double nxX = xX + ex1, nyY = yY + ey1;
+                ax += xX * ny - uy * nxX;
+                ay += ux * nyY - yY * nx;
+                ux = nx; uy = ny; xX = nxX; yY = nyY;
+                for (int i = k + 2; i < n; i++) {
+                    const double2 di = Ds[i];
+                    const double nx2 = ux + di.x, ny2 = uy + di.y;
+                    const double nxX2 = xX + di.x, nyY2 = yY + di.y;
+                    ax += xX * ny2 - uy * nxX2;
+                    ay += ux * nyY2 - yY * nx2;
+                    ux = nx2; uy = ny2; xX = nxX2; yY = nyY2;
+                }
+            }
+            const double z = 0;
+            ax += xX * z - uy * z;
+            ay += ux * z - yY * z;
+            ax *= 0.5;
+            ay *= 0.5;
+            const int v = Ls[k];
+            const int a = d.vg[3 * v], b = d.vg[3 * v + 1], c = d.vg[3 * v + 2];
+            const int j = (a == g) ? 0 : ((b == g) ? 1 : ((c == g) ? 2 : -1));
+            if (j >= 0)
+                reinterpret_cast<double2*>(d.vterm)[3 * (size_t)v + j] =
+                    make_double2(se * (fabs(ax) - a0), se * (fabs(ay) - a0));
+        }
+        __syncthreads();
+    }
+}
+
+// A vertex for the two-pass velocity (k_velocity_fallback).  Each vertex is
+// listed at most once per step (the list holds V entries; the counter is
+// cleared by k_begin); timing launches of the autotuner list nothing.
+__device__ __forceinline__ void push_vfall(const Dev& d, int v) {
+    const int s = atomicAdd(&d.cnt[C_VFALL], 1);
+    if (s < d.V) d.l_vfall[s] = v;
+}
+
 // Velocity, pass 2 of 2: the line-tension part of energy_diff
 // (calculus.h:28-37,57-60) -- six shifted lengths -- plus the three
 // stored-energy terms of pass 1 summed in vertex.grains order, then
 // v = -dE/grad_eps * mobility (calculus.h:149-150).  Vertices whose
 // unshifted coordinate dom_mod would move (a coordinate equal to D) are
 // listed for the two-pass path.
-__global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
+__global__ void __launch_bounds__(256) k_vel_vertex(Dev d, int tuning) {
     TABLE_BLOCK(d.vblk, d.nvb)
     const int v = i;
     const bool in = i < bd_.e;
@@ -449,14 +706,13 @@ __global__ void __launch_bounds__(256) k_vel_vertex(Dev d) {
 #pragma unroll
         for (int j = 0; j < 3; j++) t[j] = reinterpret_cast<const double2*>(d.vterm)[3 * (size_t)v + j];
     }
-    if (!M.active || P.vel_model != SEVC_VEL_FD || !en) return;
+    if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD || !en) return;
     const double D = P.D;
     const double x = p.x, y = p.y;
     const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
     const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
     if (!((pyy == y) && (qx == x))) {
-        int s = atomicAdd(&d.cnt[C_VFALL], 1);
-        d.l_vfall[s] = v;
+        if (!tuning) push_vfall(d, v);
         return;
     }
     const bool uni = (P.gb_eps == 0.0);
@@ -505,12 +761,343 @@ __global__ void k_velocity_generic(Dev d) {
 }
 
 __global__ void k_velocity_fallback(Dev d) {
-    const int n = d.cnt[C_VFALL];
+    const int n = min(d.cnt[C_VFALL], d.V);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int v = d.l_vfall[k];
         velocity_generic(d, d.prm[member_of_vertex(d, v)], v, false);
     }
 }
+
+// Velocity in one pass over tiles of consecutive vertex slots (energy_diff,
+// calculus.h:19-73, for both components, compute_vertex_velocities,
+// :143-154, and compute_grain_areas, :224-234).  A workgroup takes VT
+// vertices of one member:
+//  1. their positions, grains and boundaries (coalesced), and the distinct
+//     grains they touch, gathered in an LDS hash set;
+//  2. those grains' lengths, stored energies and list rows, then the
+//     positions of every listed vertex (one gather each), staged in LDS;
+//     meanwhile each vertex lane loads its boundaries' other ends and their
+//     positions;
+//  3. the unwrap deltas wrap(P_k - P_{k-1}) of every staged polygon;
+//  4. one lane per (vertex, grain slot j): the grain's area and the two
+//     areas with the vertex shifted by grad_eps in x and in y, in one pass
+//     over the polygon's deltas (see below), giving the stored-energy terms
+//     SE*(|A'| - |A|); the lane of the grain's first list entry writes the
+//     grain's area (exactly one tile holds that vertex);
+//  5. one lane per vertex: the six shifted lengths, the terms summed in
+//     vertex.grains order, v = -dE/grad_eps * mobility.
+// A grain touched by several tiles is staged by each (its areas are
+// recomputed, identically); every (vertex, grain) term is computed once.
+// No intermediate leaves the workgroup.
+//
+// The shifted areas are the reference's compute_area (geometry.h:757-765
+// after the sequential unwrap of :908-931) on the polygon with point k
+// replaced.  Unwrapped points U_i = U_{i-1} + D_i (U_0 = 0) and the
+// left-to-right sum of U_i x U_{i+1} are carried for the unshifted polygon;
+// up to U_{k-1} the shifted polygons' tracks are that track, at i = k and
+// k + 1 their deltas are the ones to and from the shifted point, later they
+// add the shared deltas D_i.  Every sum is the reference's, term by term,
+// so the results are bit-identical.  Vertices whose unshifted coordinate
+// dom_mod would move go to the two-pass path (k_velocity_fallback), as do
+// all vertices of a tile whose polygons exceed the staging room.
+template <int VT>
+struct VTileCfg {
+    static constexpr int kPts = 7 * VT;          // staged list entries (a 1M-grain tessellation needs <= 6 VT)
+    static constexpr int kGr = 2 * VT;           // distinct grains (about 0.8 VT at 64)
+    static constexpr int kHash = 4 * VT;         // hash slots (power of two; > 3 VT keys, so probing ends)
+    static constexpr int kThreads = 4 * VT;      // one lane per (vertex, slot) needs 3 VT
+    static constexpr int kRow = 8;               // list entries loaded with the grain
+};
+
+template <int VT>
+__global__ void __launch_bounds__(4 * VT) k_vel_vtile(Dev d, int tuning) {
+    using C = VTileCfg<VT>;
+    __shared__ double2 Ps[C::kPts];          // raw positions of the staged entries
+    __shared__ double2 Ds[C::kPts];          // unwrap deltas; before that the grains' list rows
+    __shared__ int Ls[C::kPts];              // vertex of each staged entry
+    __shared__ unsigned char pg[C::kPts];    // its grain (index into ug)
+    __shared__ int hkey[C::kHash];           // hash set of grain ids (-1: empty)
+    __shared__ unsigned char hval[C::kHash]; // index of the grain in ug
+    __shared__ int ug[C::kGr];               // distinct grains
+    __shared__ int uoff[C::kGr + 1];         // their first staged entry
+    __shared__ int ulen[C::kGr];             // their list lengths
+    __shared__ double use_[C::kGr];          // their SE
+    __shared__ int vslot[3 * VT];            // hash slot of vertex.grains[j]
+    __shared__ int vid[VT];                  // vertex of each tile lane (-1: not computed here)
+    __shared__ double2 vterm_s[3 * VT];      // (x, y) stored-energy terms of (vertex, j)
+    __shared__ int nu_s, nover_s;
+    static_assert(C::kGr <= 255, "grain index fits a byte");
+    static_assert((C::kHash & (C::kHash - 1)) == 0, "hash size is a power of two");
+    int* rowbuf = reinterpret_cast<int*>(Ds);   // kGr rows of kRow entries
+    static_assert(sizeof(Ds) >= sizeof(int) * C::kGr * C::kRow, "row staging fits the delta array");
+
+    const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb >= d.nvtile) return;
+    const Blk bd = d.vtile[lb];
+    const Params& P = d.prm[bd.m];
+    const MState& M = d.ms[bd.m];
+    // tuning: timed by sevc_autotune on any member (velocities and areas
+    // are recomputed by every step before they are read)
+    if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD) return;
+    const double D = P.D;
+    const int tid = threadIdx.x;
+    const int nt = C::kThreads;
+
+    // 1. the tile's vertices (lane t < VT) and the hash set of their grains
+    const int v = bd.b + tid;
+    const bool vlane = tid < VT;
+    bool ven = false;
+    double2 pv = make_double2(0.0, 0.0);
+    int vgj[3] = {-1, -1, -1}, vbj[3] = {-1, -1, -1};
+    if (vlane && v < bd.e) {
+        ven = d.ven[v] != 0;
+        if (ven) {
+            pv = reinterpret_cast<const double2*>(d.pos)[v];
+#pragma unroll
+            for (int j = 0; j < 3; j++) { vgj[j] = d.vg[3 * v + j]; vbj[j] = d.vb[3 * v + j]; }
+        }
+    }
+    for (int h = tid; h < C::kHash; h += nt) hkey[h] = -1;
+    if (tid == 0) { nu_s = 0; nover_s = 0; }
+    __syncthreads();
+    // the other ends of the vertex's boundaries: loads issued now, used last
+    int2 ends[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
+    if (ven) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) ends[j] = reinterpret_cast<const int2*>(d.bvx)[vbj[j]];
+    }
+    if (vlane) vid[tid] = ven ? v : -1;
+    if (ven) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int g = vgj[j];
+            unsigned h = ((unsigned)g * 2654435761u) >> 7;
+            for (;;) {
+                h &= C::kHash - 1;
+                const int old = atomicCAS(&hkey[h], -1, g);
+                if (old == -1) {
+                    const int u = atomicAdd(&nu_s, 1);
+                    if (u < C::kGr) {
+                        hval[h] = (unsigned char)u;
+                        ug[u] = g;
+                    }
+                    break;
+                }
+                if (old == g) break;
+                h++;
+            }
+            vslot[3 * tid + j] = (int)h;
+        }
+    }
+    __syncthreads();
+    const int nu = min(nu_s, C::kGr);
+
+    // 2. the grains' lengths, SE and first list entries; the neighbours'
+    // positions (loads issued now, used last)
+    double2 q[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    if (ven) {
+        const double2* X = reinterpret_cast<const double2*>(d.pos);
+#pragma unroll
+        for (int j = 0; j < 3; j++) q[j] = X[ends[j].x == v ? ends[j].y : ends[j].x];
+    }
+    for (int u = tid; u < nu; u += nt) {
+        const int g = ug[u];
+        ulen[u] = d.glen[g];
+        use_[u] = d.gse[g];
+    }
+    for (int c = tid; c < nu * (C::kRow / 4); c += nt) {
+        const int u = c / (C::kRow / 4), part = c % (C::kRow / 4);
+        const int4 r = reinterpret_cast<const int4*>(glist(d, ug[u]))[part];
+        reinterpret_cast<int4*>(rowbuf)[c] = r;
+    }
+    __syncthreads();
+    // inclusive scan of the lengths by wave 0 (nu <= kGr, a few per lane)
+    if (tid < 64) {
+        constexpr int kPer = (C::kGr + 63) / 64;
+        int loc[kPer], sum = 0;
+#pragma unroll
+        for (int e = 0; e < kPer; e++) {
+            const int u = tid * kPer + e;
+            loc[e] = u < nu ? ulen[u] : 0;
+            sum += loc[e];
+        }
+        int inc = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o, 64);
+            if (tid >= o) inc += t;
+        }
+        int run = inc - sum;
+#pragma unroll
+        for (int e = 0; e < kPer; e++) {
+            const int u = tid * kPer + e;
+            if (u < nu) { uoff[u] = run; run += loc[e]; }
+        }
+        if (tid == 63) {
+            uoff[nu] = inc;
+            if (inc > C::kPts || nu_s > C::kGr) nover_s = 1;
+        }
+    }
+    __syncthreads();
+    const int npts = uoff[nu];
+    if (nover_s) {
+        // polygons too long to stage: the tile's vertices take the two-pass
+        // path, which reads the stored areas; the areas this tile would have
+        // written (grains whose first entry is one of its vertices) are
+        // written here
+        if (ven) {
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (glist(d, vgj[j])[0] == v) d.garea[vgj[j]] = grain_area(d, vgj[j], D);
+            if (!tuning) push_vfall(d, v);
+        }
+        return;
+    }
+    // each grain's entries: its vertices and grain index (entries past the
+    // first kRow are read from the list itself)
+    for (int u = tid; u < nu; u += nt) {
+        const int b0 = uoff[u], n = uoff[u + 1] - b0;
+        const int* row = rowbuf + u * C::kRow;
+        int tmp[C::kRow];
+#pragma unroll
+        for (int i = 0; i < C::kRow; i++) tmp[i] = row[i];
+        for (int i = 0; i < n; i++) pg[b0 + i] = (unsigned char)u;
+#pragma unroll
+        for (int i = 0; i < C::kRow; i++)
+            if (i < n) Ls[b0 + i] = tmp[i];
+        if (n > C::kRow) {
+            const int* L = glist(d, ug[u]);
+            for (int i = C::kRow; i < n; i++) Ls[b0 + i] = L[i];
+        }
+    }
+    __syncthreads();
+    {
+        const double2* X = reinterpret_cast<const double2*>(d.pos);
+        for (int p = tid; p < npts; p += nt) Ps[p] = X[Ls[p]];
+    }
+    __syncthreads();
+
+    // 3. unwrap deltas (the row staging in Ds is no longer needed)
+    for (int p = tid; p < npts; p += nt) {
+        const int b0 = uoff[pg[p]];
+        Ds[p] = (p > b0) ? make_double2(wrapd(Ps[p].x, Ps[p - 1].x, D), wrapd(Ps[p].y, Ps[p - 1].y, D))
+                         : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+
+    // 4. one lane per (vertex, slot j): lane t = j * VT + vertex
+    if (tid < 3 * VT) {
+        const int j = tid / VT, vi = tid % VT;
+        const int w = vid[vi];
+        double2 res = make_double2(0.0, 0.0);
+        if (w >= 0) {
+            const int u = hval[vslot[3 * vi + j]];
+            const int b0 = uoff[u], n = uoff[u + 1] - b0;
+            int k = -1;
+            for (int i = 0; i < n; i++)
+                if (Ls[b0 + i] == w) { k = i; break; }
+            if (k >= 0) {
+                const double2 pw = Ps[b0 + k];
+                const double px = dom_mod(pw.x + SEVC_GRAD_EPS, D), qy = dom_mod(pw.y + SEVC_GRAD_EPS, D);
+                // unshifted track up to U_{k-1}: sum a of U_i x U_{i+1}, i <= k-2
+                double ux = 0.0, uy = 0.0, a = 0.0;
+                for (int i = 1; i < k; i++) {
+                    const double2 di = Ds[b0 + i];
+                    const double nx = ux + di.x, ny = uy + di.y;
+                    a += ux * ny - uy * nx;
+                    ux = nx;
+                    uy = ny;
+                }
+                // tracks at index k: the x-shifted polygon's point (xX, Y_k),
+                // the y-shifted one's (X_k, yY)
+                double ax = a, ay = a, xX, yY;
+                if (k == 0) {
+                    xX = 0.0;
+                    yY = 0.0;
+                } else {
+                    const double2 pm = Ps[b0 + k - 1];
+                    const double2 dk = Ds[b0 + k];
+                    const double ex = wrapd(px, pm.x, D), ey = wrapd(qy, pm.y, D);
+                    const double nx = ux + dk.x, ny = uy + dk.y;
+                    xX = ux + ex;
+                    yY = uy + ey;
+                    a += ux * ny - uy * nx;
+                    ax += ux * ny - uy * xX;
+                    ay += ux * yY - uy * nx;
+                    ux = nx;
+                    uy = ny;
+                }
+                if (k + 1 < n) {
+                    const double2 pn = Ps[b0 + k + 1];
+                    const double2 d1 = Ds[b0 + k + 1];
+                    const double ex1 = wrapd(pn.x, px, D), ey1 = wrapd(pn.y, qy, D);
+                    const double nx = ux + d1.x, ny = uy + d1.y;
+                    const double nxX = xX + ex1, nyY = yY + ey1;
+                    a += ux * ny - uy * nx;
+                    ax += xX * ny - uy * nxX;
+                    ay += ux * nyY - yY * nx;
+                    ux = nx; uy = ny; xX = nxX; yY = nyY;
+                    for (int i = k + 2; i < n; i++) {
+                        const double2 di = Ds[b0 + i];
+                        const double nx2 = ux + di.x, ny2 = uy + di.y;
+                        const double nxX2 = xX + di.x, nyY2 = yY + di.y;
+                        a += ux * ny2 - uy * nx2;
+                        ax += xX * ny2 - uy * nxX2;
+                        ay += ux * nyY2 - yY * nx2;
+                        ux = nx2; uy = ny2; xX = nxX2; yY = nyY2;
+                    }
+                }
+                const double z = 0;
+                a += ux * z - uy * z;
+                ax += xX * z - uy * z;
+                ay += ux * z - yY * z;
+                a *= 0.5;
+                ax *= 0.5;
+                ay *= 0.5;
+                if (n <= 0) a = 0.0;
+                const double a0 = fabs(a), se = use_[u];
+                res = make_double2(se * (fabs(ax) - a0), se * (fabs(ay) - a0));
+                if (k == 0) d.garea[ug[u]] = a;       // compute_grain_areas: one writer per grain
+            }
+        }
+        vterm_s[tid] = res;
+    }
+    __syncthreads();
+
+    // 5. one lane per vertex: line tension and the sum (calculus.h:57-61,149-150)
+    if (!ven) return;
+    const double x = pv.x, y = pv.y;
+    const double px = dom_mod(x + SEVC_GRAD_EPS, D), pyy = dom_mod(y, D);
+    const double qx = dom_mod(x, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
+    if (!((pyy == y) && (qx == x))) {
+        if (!tuning) push_vfall(d, v);
+        return;
+    }
+    const bool uni = (P.gb_eps == 0.0);
+    double bx = 0.0, by = 0.0, gx = 0.0, gy = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        // with pyy == y and qx == x the shifted lengths share the unshifted
+        // length's other wrapped difference
+        const double dx0 = wrapd(x, q[j].x, D), dy0 = wrapd(y, q[j].y, D);
+        const double lx = mag(wrapd(px, q[j].x, D), dy0);
+        const double ly = mag(dx0, wrapd(qy, q[j].y, D));
+        const double L0 = mag(dx0, dy0);
+        const double gam = uni ? P.gb_scaling : d.ben[vbj[j]];
+        bx += (gam * (lx - L0));
+        by += (gam * (ly - L0));
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const double2 t = vterm_s[j * VT + tid];
+        gx += t.x;
+        gy += t.y;
+    }
+    reinterpret_cast<double2*>(d.vel)[v] =
+        make_double2((-(bx + gx) / SEVC_GRAD_EPS) * P.mobility, (-(by + gy) / SEVC_GRAD_EPS) * P.mobility);
+}
+
+template __global__ void k_vel_vtile<32>(Dev d, int tuning);
+template __global__ void k_vel_vtile<64>(Dev d, int tuning);
 
 // Analytic -grad E velocities (extension).
 __global__ void k_velocity_analytic(Dev d) {
@@ -671,7 +1258,7 @@ __global__ void k_label(Dev d) {
 // one-workgroup polling kernel reads contiguous columns only.
 __global__ void k_poll_gather(Dev d) {
     const int nc = d.cnt[C_CAND];
-    if (nc > kPollFast) return;
+    if (nc > d.poll_fast) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         const int c = d.l_cand[k];
         const int a[2] = {d.bvx[2 * c], d.bvx[2 * c + 1]};
@@ -724,7 +1311,7 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         }
         return d.vb[3 * v + arg];
     };
-    if (nc <= kPollFast && 2 * nt == kPollFast) {
+    if (nc <= d.poll_fast && 2 * nt == kPollFast) {
         // Up to two candidates per lane (kPollFast = 2 x 1024).  Their list
         // positions, t_ext and flags sit in LDS; k_poll_gather has mapped
         // every boundary the rounds and the conflict check look at to its

@@ -25,6 +25,7 @@ struct Params {
     double gb_eps, gb_scaling, mc_k, nucl_rate, nucl_scaling, tmax;
     unsigned long long seed;
     int nmax, adaptive, nucl_mode, do_nucl, gap, events, vel_model, min_grains, n_grains0;
+    int gate_grains;                 // MC nucleation opens once enabled grains <= this (5 n_grains0 / 6)
     int debug;                       // debug_lvl: >= 1 runs the checks of check_data inside the step
     int cap;                         // max_vrt_per_grn of this member (lists are stored at the batch's stride S >= cap)
     int voff, nv, eoff, ne, goff, ng;
@@ -45,7 +46,7 @@ struct Blk { int m, b, e; };
 
 enum Counter {
     C_CAND = 0, C_TWO, C_T2, C_T2C, C_FLIPC,
-    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_COUNT
+    C_FIX0, C_FIX1, C_FIX2, C_DIRTY, C_NUCL, C_CONF, C_NEV, C_VFALL, C_RELABEL, C_BIGR, C_COUNT
 };
 
 enum { FIX_ADD = 0, FIX_REMOVE = 1, FIX_REPLACE = 2, FIX_NONE = 3 };   // geometry.h:17
@@ -80,6 +81,7 @@ struct Dev {
     const Blk* vblk; const Blk* eblk; const Blk* gblk;
     int nvb, neb, ngb;
     const Blk* gtile; int ntile;         // 64-grain tiles for the velocity pass 1
+    const Blk* vtile; int nvtile;        // vertex tiles of the one-pass velocity
     double* vterm;                       // 6 per vertex: stored-energy terms (x,y) per grain slot
     int4* slist;                         // free-slot search: (kind, block, rank) lists, ngb+neb+nvb entries
     int* vblk_first; int* eblk_first; int* gblk_first;   // first table block of each member (B+1)
@@ -87,6 +89,7 @@ struct Dev {
     int* cnt;
     int* l_cand; int* l_two; int* l_t2; int* l_t2c; int* l_flipc;
     int* l_fix[kFixStages]; int* l_dirty; int* l_nucl; int* l_conf; int* l_vfall;
+    int* l_bigr;                          // grains longer than the lane-per-grain velocity pass holds
     int* nrec; double* nal;               // rate-mode candidate sites: (site or -1, its 3 grains), shortest edge
     int* l_nev;                           // nucleation events of the step: (member, A, B, C, AB, BC, CA, g)
     int2* bc_v; int2* bc_e; int2* bc_g;   // per table block: (enabled, free) slot counts
@@ -95,6 +98,7 @@ struct Dev {
     int diag;                             // compute export diagnostics this step
     int debug;                            // a member runs the checks of check_data (debug_lvl >= 1)
     int* chk_bad;                         // violations per member found by the last check
+    int poll_fast;                        // longest candidate list polled in LDS (<= kPollFast)
 };
 
 }  // namespace sevc

@@ -40,6 +40,7 @@ class Options(C.Structure):
         ("dt_min", C.c_double), ("flip_threshold", C.c_double), ("nucleation_mode", C.c_int32),
         ("nucleation_rate", C.c_double), ("seed", C.c_uint64), ("max_vrt_per_grn", C.c_int32),
         ("extra_capacity", C.c_double), ("velocity_model", C.c_int32), ("events", C.c_int32),
+        ("nucleation_gate", C.c_int32),
     ]
 
     def copy(self):
@@ -190,4 +191,5 @@ def default_options():
     o.extra_capacity = 0.0
     o.velocity_model = VEL_FD
     o.events = 1
+    o.nucleation_gate = -1
     return o

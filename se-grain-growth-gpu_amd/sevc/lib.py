@@ -62,8 +62,12 @@ def lib():
         "sevc_set_member_params": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                              C.c_double, C.c_double]),
         "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
+        "sevc_set_member_mesh": (C.c_int, [C.c_void_p, C.c_int32, P(Mesh)]),
         "sevc_get_positions": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double)]),
         "sevc_autotune": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+        "sevc_velocity_config": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
+        "sevc_set_velocity_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "sevc_set_poll_limit": (C.c_int, [C.c_void_p, C.c_int32]),
         "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -178,7 +182,8 @@ class State:
         arr_m = (Mesh * len(meshes))(*self._structs)
         h = C.c_void_p()
         rc = lib().sevc_batch_create(arr_o, arr_m, len(meshes), C.byref(h))
-        _check(rc)
+        if rc != 0:
+            raise SevcError(rc, lib().sevc_strerror(rc).decode() + " / " + lib().sevc_last_error(None).decode())
         self.h = h
         self.n_members = len(meshes)
 
@@ -239,6 +244,20 @@ class State:
         _check(lib().sevc_autotune(self.h, C.byref(t)), self.h)
         return t.value
 
+    def velocity_config(self):
+        """(path, tile): 0 = two passes (grains per tile), 1 = one pass over
+        vertex tiles (vertices per tile)."""
+        p, t = C.c_int32(), C.c_int32()
+        _check(lib().sevc_velocity_config(self.h, C.byref(p), C.byref(t)), self.h)
+        return p.value, t.value
+
+    def set_poll_limit(self, max_fast):
+        """Longest candidate list polled in LDS (0: always the general rounds)."""
+        _check(lib().sevc_set_poll_limit(self.h, int(max_fast)), self.h)
+
+    def set_velocity_config(self, path, tile):
+        _check(lib().sevc_set_velocity_config(self.h, int(path), int(tile)), self.h)
+
     def positions(self, m=0):
         """Vertex positions of member m, shape (n_vertex_slots, 2)."""
         nv = self.sizes(m)[0]
@@ -248,6 +267,10 @@ class State:
 
     def request_diagnostics(self):
         _check(lib().sevc_request_diagnostics(self.h), self.h)
+
+    def set_mesh(self, m, mesh):
+        """Restore member m's whole mesh (a HostMesh of the member's sizes)."""
+        _check(lib().sevc_set_member_mesh(self.h, int(m), C.byref(mesh.struct)), self.h)
 
     def set_params(self, m=0, pos=None, g_SE=None, gb_scaling=-1.0, mobility=-1.0):
         P = C.POINTER(C.c_double)

@@ -50,3 +50,26 @@ def test_calls_reject_bad_arguments_and_state_keeps_working():
     assert s.status == 0 and s.steps_done == 150 and s.n_flips > 0
     assert st.check() == [0]
     st.close()
+
+
+def test_member_mesh_restore_replays_the_same_steps():
+    """sevc_set_member_mesh restores a checkpoint taken with sevc_get_mesh:
+    stepping again from it gives the same state as the first time (no
+    nucleation, so the step counter does not enter), in a batch whose other
+    member is left alone."""
+    from _util import diff_fields
+    o = ref2k_options(dt=2e-4)
+    m = sevc.load_files(o)
+    st = sevc.State([o, o], [m, m])
+    st.step(50)
+    ck = st.get_mesh(0)
+    st.step(50)
+    a = st.get_mesh(0)
+    st.set_mesh(0, ck)
+    assert diff_fields(st.get_mesh(0), ck, ["pos", "v_bnd", "v_grn", "b_vtx", "g_vtx", "g_vlen", "g_enabled"]) == {}
+    st.step(50)
+    assert diff_fields(st.get_mesh(0), a) == {}
+    bad = ck.copy()
+    bad["b_enabled"][10] = 0
+    assert _code(lambda: st.set_mesh(0, bad)) == sevc.E_TOPOLOGY
+    st.close()

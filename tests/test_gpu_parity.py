@@ -122,6 +122,79 @@ def test_rate_nucleation_adaptive_threshold():
     assert tot["nucl"] >= 10 and tot["flips"] >= 50, tot
 
 
+@pytest.mark.parametrize("path,tile", [(0, 16), (0, 64), (1, 32), (1, 64), (2, 0)])
+def test_velocity_paths_bitwise(path, tile):
+    """Both FD velocity paths -- two passes (grain tiles, then vertices) and
+    one pass over vertex tiles -- at every tile size give the oracle's bits
+    through flips, removals and nucleations: on a generated tessellation
+    (vertex ids follow space, so every one-pass tile fits its staging) with
+    rate nucleation and adaptive dt, and on the reference's 2,000-grain
+    fixture with anisotropic energies (vertex ids in file order: one-pass
+    tiles overflow and their vertices take the two-pass fallback)."""
+    cases = [_generated(3000, 5, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3, do_nucleation=1,
+                        nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02, nucleation_scaling=0.9, seed=11,
+                        extra_capacity=0.1)]
+    o2 = ref2k_options(dt=2e-4, GB_eps=0.3)
+    cases.append((o2, sevc.load_files(o2)))
+    for ci, (o, mesh) in enumerate(cases):
+        st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+        assert st.velocity_config() == (0, 64)       # a new state's path
+        st.set_velocity_config(path, tile)
+        assert st.velocity_config() == (path, tile)
+        tot = 0
+        for chunk in (1, 20, 60):
+            print(f"  case {ci} path {path} tile {tile}: chunk of {chunk}", flush=True)
+            s1 = st.step(chunk)
+            rc, s2 = orc.step(chunk)
+            assert rc == 0 and s1.status == 0
+            assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+            for k in STAT_KEYS:
+                assert getattr(s1, k) == getattr(s2, k), k
+            tot += s1.n_flips + s1.n_removals + s1.n_nucleations
+        assert tot > 0
+        st.close()
+
+
+def test_autotune_keeps_results():
+    """The autotuner times every path and tile that fits the state and keeps
+    the fastest; the state is left as it was, and the run then equals the
+    oracle bit for bit."""
+    for o, mesh in (_generated(3000, 6, dt=1e-4), (ref2k_options(dt=2e-4), None)):
+        mesh = mesh or sevc.load_files(o)
+        st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+        before = st.get_mesh()
+        tile = st.autotune()
+        path, t2 = st.velocity_config()
+        assert t2 == tile and path in (0, 1)
+        assert diff_fields(st.get_mesh(), before) == {}
+        s1 = st.step(30)
+        rc, _ = orc.step(30)
+        assert rc == 0 and s1.status == 0
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+        st.close()
+
+
+@pytest.mark.parametrize("limit", [0, 64])
+def test_poll_general_rounds_match_lds_rounds(limit):
+    """The vote rounds in LDS (lists up to 2,048 candidates) and the general
+    rounds over global memory give the same flips: the same inputs run
+    with the LDS path switched off (limit 0) or limited to 64 candidates
+    stay bit-identical to the oracle, as the default run does."""
+    o, mesh = _generated(3000, 21, dt=1e-5, flip_threshold=0.3 * np.sqrt(1.0 / 3000))
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    st.set_poll_limit(limit)
+    flips = 0
+    for chunk in (5, 20):
+        print(f"  poll limit {limit}: chunk of {chunk}", flush=True)
+        s1 = st.step(chunk)
+        rc, s2 = orc.step(chunk)
+        assert rc == 0 and s1.status == 0
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+        flips += s1.n_flips
+    assert flips > 20
+    st.close()
+
+
 def test_graph_replay():
     o = ref2k_options(dt=2e-4)
     mesh = sevc.load_files(o)
