@@ -339,6 +339,14 @@ int sevc_set_member_params(sevc_state* s, int32_t member, const double* pos,
  * t_ext are taken when given; the member's step counter and time go on. */
 int sevc_set_member_mesh(sevc_state* s, int32_t member, const sevc_mesh* mesh);
 
+/* Device checkpoints of the whole state (every member's mesh, scratch and
+ * loop state; not the parameters of sevc_set_member_params): slot 0..63,
+ * kept in device memory until sevc_destroy.  sevc_restore puts the state
+ * back as it was at sevc_checkpoint, so that a later step repeats exactly
+ * (extension; e.g. the windows of a calibration horizon). */
+int sevc_checkpoint(sevc_state* s, int32_t slot);
+int sevc_restore(sevc_state* s, int32_t slot);
+
 /* Gradients of one member from sevc_adjoint_window (extension). */
 typedef struct sevc_grads {
     double* d_pos;           /* 2 per vertex slot: dL/dx at the window's start (NULL: not wanted) */

@@ -117,6 +117,13 @@ struct sevc_state {
     int vel_path = 0;
     int vtile_size = 64;
     size_t vtile_cap = 0;
+    // device checkpoints of the whole state (sevc_checkpoint / sevc_restore)
+    struct Ckpt {
+        char* buf = nullptr;
+        size_t bytes = 0;
+        std::vector<MState> ms;
+    };
+    std::vector<Ckpt> ckpts;
     // topology-fixed window: checkpoints of x every K steps (ck), the
     // positions and areas of one segment (seg, segA; `holds` = its index),
     // and the reverse-pass arrays
@@ -729,6 +736,7 @@ void sevc_destroy(sevc_state* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
+    for (auto& c : s->ckpts) (void)hipFree(c.buf);
     for (void* p : s->allocs) (void)hipFree(p);
     for (auto e : s->pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1101,6 +1109,67 @@ int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad) {
     }
     (void)hipFree(buf);
     return rc;
+}
+
+namespace {
+// Every device array a step reads and writes (the loop state of the members
+// aside): what a checkpoint holds.  Parameters (Params) are not state.
+std::vector<std::pair<void*, size_t>> state_arrays(const sevc_state* s) {
+    const Dev& d = s->d;
+    const size_t V = s->V, E = s->E, G = s->G, S = s->S;
+    return {{d.pos, 16 * V}, {d.vel, 16 * V}, {d.vb, 12 * V}, {d.vg, 12 * V}, {d.ven, V},
+            {d.venergy, 8 * V}, {d.vparts, 32 * V}, {d.mods, sizeof(ModRec) * V},
+            {d.nucl_r, 8 * V}, {d.nucl_ori, 8 * V}, {d.dE, 8 * V}, {d.nfac, 8 * V}, {d.vnuc, V},
+            {d.bvx, 8 * E}, {d.ben, 8 * E}, {d.blen, 8 * E}, {d.bt, 8 * E}, {d.benab, E}, {d.bcand, E},
+            {d.conf, 4 * E}, {d.blab, E}, {d.l_flipped, 4 * (E + 16)}, {d.n_flipped, 4},
+            {d.gvx, 4 * G * S}, {d.glen, 4 * G}, {d.garea, 8 * G}, {d.gdadt, 8 * G}, {d.gori, 8 * G},
+            {d.gse, 8 * G}, {d.genab, G}, {d.gtype, G}, {d.gcand, G}, {d.gfix, 4 * G},
+            {d.cnt, 4 * static_cast<size_t>(C_COUNT)}, {d.bc_v, 8 * static_cast<size_t>(d.nvb)},
+            {d.bc_e, 8 * static_cast<size_t>(d.neb)}, {d.bc_g, 8 * static_cast<size_t>(d.ngb)},
+            {d.ms, sizeof(MState) * static_cast<size_t>(s->B)}};
+}
+constexpr int kMaxCkpt = 64;
+}  // namespace
+
+int sevc_checkpoint(sevc_state* s, int32_t slot) {
+    if (!s || slot < 0 || slot >= kMaxCkpt) return SEVC_E_INVALID_ARG;
+    if (static_cast<int>(s->ckpts.size()) <= slot) s->ckpts.resize(slot + 1);
+    auto arrays = state_arrays(s);
+    size_t total = 0;
+    for (auto& a : arrays) total += (a.second + 255) / 256 * 256;
+    sevc_state::Ckpt& c = s->ckpts[slot];
+    if (!c.buf) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (hipMalloc(&c.buf, total) != hipSuccess) return fail(s, SEVC_E_NOMEM, "no room for a checkpoint");
+        c.bytes = total;
+    }
+    size_t off = 0;
+    for (auto& a : arrays) {
+        HIPCHK(hipMemcpyAsync(c.buf + off, a.first, a.second, hipMemcpyDeviceToDevice, s->stream));
+        off += (a.second + 255) / 256 * 256;
+    }
+    HIPCHK(hipMemcpyAsync(s->ms_host.data(), s->d.ms, sizeof(MState) * s->ms_host.size(), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    c.ms = s->ms_host;
+    return SEVC_OK;
+}
+
+int sevc_restore(sevc_state* s, int32_t slot) {
+    if (!s || slot < 0 || slot >= static_cast<int>(s->ckpts.size()) || !s->ckpts[slot].buf)
+        return SEVC_E_INVALID_ARG;
+    const sevc_state::Ckpt& c = s->ckpts[slot];
+    size_t off = 0;
+    for (auto& a : state_arrays(s)) {
+        HIPCHK(hipMemcpyAsync(a.first, c.buf + off, a.second, hipMemcpyDeviceToDevice, s->stream));
+        off += (a.second + 255) / 256 * 256;
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->ms_host = c.ms;
+    s->ms_prev = c.ms;
+    s->ben_dirty = true;
+    s->win.ready = false;
+    return SEVC_OK;
 }
 
 int sevc_set_member_mesh(sevc_state* s, int32_t m, const sevc_mesh* me) {

@@ -308,16 +308,33 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD) return;
     const double D = P.D;
     const int tid = threadIdx.x, nt = blockDim.x;
+    // the first kRowEntries list entries of the entry's grains, loaded
+    // together with their lengths (one 16-byte load per thread) and staged
+    // in the delta array until the first tile's deltas overwrite them; later
+    // tiles of the entry read the lists themselves
+    constexpr int kRowEntries = 16;
+    static_assert(kRowEntries * kTile * 4 <= (int)sizeof(Ds), "rows fit the delta array");
+    int* rowbuf = reinterpret_cast<int*>(Ds);
+    const bool rows = (d.S % 4 == 0) && d.S >= kRowEntries;
+    if (rows && tid < kTile * (kRowEntries / 4)) {
+        const int g = bd.b + tid / (kRowEntries / 4), c = tid % (kRowEntries / 4);
+        if (g < bd.e)
+            reinterpret_cast<int4*>(rowbuf)[tid] = reinterpret_cast<const int4*>(glist(d, g))[c];
+    }
     for (int gs = bd.b; gs < bd.e;) {
+        const bool from_rows = rows && gs == bd.b;
         if (tid < kTile) {
             // wave 0: inclusive scan of list lengths; the tile takes the
             // longest prefix of grains whose lists fit in kTilePts points
             const int g = gs + tid;
-            const bool en = g < bd.e && d.genab[g];
-            const int n = en ? d.glen[g] : 0;
+            const bool in = g < bd.e;
+            const uint8_t en8 = in ? d.genab[g] : 0;
+            const int len = in ? d.glen[g] : 0;
+            const bool en = en8 != 0;
+            const int n = en ? len : 0;
             // the values the later phases need from global memory, loaded here
             en_s[tid] = en ? 1 : 0;
-            se_s[tid] = g < bd.e ? d.gse[g] : 0.0;
+            se_s[tid] = in ? d.gse[g] : 0.0;
             int inc = n;
             for (int o = 1; o < kTile; o <<= 1) {
                 const int t = __shfl_up(inc, o, kTile);
@@ -338,8 +355,8 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
         __syncthreads();
         const int gend = gend_s, ngr = gend - gs, npts = off[ngr];
         for (int p = tid; p < npts; p += nt) {
-            const int g = pgl[p];
-            const int w = glist(d, gs + g)[p - off[g]];
+            const int g = pgl[p], e = p - off[g];
+            const int w = (from_rows && e < kRowEntries) ? rowbuf[g * kRowEntries + e] : glist(d, gs + g)[e];
             Ls[p] = w;
             Ps[p] = make_double2(d.pos[2 * w], d.pos[2 * w + 1]);
             unsigned char mk = 0;
@@ -381,30 +398,51 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             const int v = Ls[p];
             const double x = Ps[p].x, y = Ps[p].y;
             const double px = dom_mod(x + SEVC_GRAD_EPS, D), qy = dom_mod(y + SEVC_GRAD_EPS, D);
-            double exk = 0, exk1 = 0, eyk = 0, eyk1 = 0;
+            // the unshifted track up to U_{k-1}, with the sum a of
+            // U_i x U_{i+1} for i <= k - 2 (both shifted polygons share it)
+            double ux = 0, uy = 0, a = 0;
+            for (int i = 1; i < k; i++) {
+                const double2 di = Ds[b0 + i];
+                const double nx = ux + di.x, ny = uy + di.y;
+                a += ux * ny - uy * nx;
+                ux = nx;
+                uy = ny;
+            }
+            // index k: the x-shifted polygon's point (xX, Y_k), the
+            // y-shifted one's (X_k, yY); index k + 1: deltas from the shifted
+            // point; later the shared deltas
+            double sx = a, sy = a, xX = 0, yY = 0;
             if (k > 0) {
-                exk = wrapd(px, Ps[p - 1].x, D);
-                eyk = wrapd(qy, Ps[p - 1].y, D);
+                const double ex = wrapd(px, Ps[p - 1].x, D), ey = wrapd(qy, Ps[p - 1].y, D);
+                const double2 dk = Ds[p];
+                const double nx = ux + dk.x, ny = uy + dk.y;
+                xX = ux + ex;
+                yY = uy + ey;
+                sx += ux * ny - uy * xX;
+                sy += ux * yY - uy * nx;
+                ux = nx;
+                uy = ny;
             }
-            if (k < n - 1) {
-                exk1 = wrapd(Ps[p + 1].x, px, D);
-                eyk1 = wrapd(Ps[p + 1].y, qy, D);
-            }
-            double ux = 0, uy = 0, xx = 0, yy = 0, sx = 0, sy = 0;
-            for (int i = 1; i < n; i++) {
-                const double2 du = Ds[b0 + i];
-                const double dux = du.x, duy = du.y;
-                const double dxx = (i == k) ? exk : ((i == k + 1) ? exk1 : dux);
-                const double dyy = (i == k) ? eyk : ((i == k + 1) ? eyk1 : duy);
-                const double nux = ux + dux, nuy = uy + duy;
-                const double nxx = xx + dxx, nyy = yy + dyy;
-                sx += xx * nuy - uy * nxx;
-                sy += ux * nyy - yy * nux;
-                ux = nux; uy = nuy; xx = nxx; yy = nyy;
+            if (k + 1 < n) {
+                const double ex1 = wrapd(Ps[p + 1].x, px, D), ey1 = wrapd(Ps[p + 1].y, qy, D);
+                const double2 d1 = Ds[p + 1];
+                const double nx = ux + d1.x, ny = uy + d1.y;
+                const double nxX = xX + ex1, nyY = yY + ey1;
+                sx += xX * ny - uy * nxX;
+                sy += ux * nyY - yY * nx;
+                ux = nx; uy = ny; xX = nxX; yY = nyY;
+                for (int i = k + 2; i < n; i++) {
+                    const double2 di = Ds[b0 + i];
+                    const double nx2 = ux + di.x, ny2 = uy + di.y;
+                    const double nxX2 = xX + di.x, nyY2 = yY + di.y;
+                    sx += xX * ny2 - uy * nxX2;
+                    sy += ux * nyY2 - yY * nx2;
+                    ux = nx2; uy = ny2; xX = nxX2; yY = nyY2;
+                }
             }
             const double z = 0;
-            sx += xx * z - uy * z;
-            sy += ux * z - yy * z;
+            sx += xX * z - uy * z;
+            sy += ux * z - yY * z;
             const double ax = sx * 0.5, ay = sy * 0.5;
             const double a0 = fabs(area_s[g]), se = se_s[g];
             const double tx = se * (fabs(ax) - a0), ty = se * (fabs(ay) - a0);
@@ -850,12 +888,11 @@ __global__ void __launch_bounds__(4 * VT) k_vel_vtile(Dev d, int tuning) {
     double2 pv = make_double2(0.0, 0.0);
     int vgj[3] = {-1, -1, -1}, vbj[3] = {-1, -1, -1};
     if (vlane && v < bd.e) {
+        // one round: the flag and the slot's data (valid addresses for any slot)
         ven = d.ven[v] != 0;
-        if (ven) {
-            pv = reinterpret_cast<const double2*>(d.pos)[v];
+        pv = reinterpret_cast<const double2*>(d.pos)[v];
 #pragma unroll
-            for (int j = 0; j < 3; j++) { vgj[j] = d.vg[3 * v + j]; vbj[j] = d.vb[3 * v + j]; }
-        }
+        for (int j = 0; j < 3; j++) { vgj[j] = d.vg[3 * v + j]; vbj[j] = d.vb[3 * v + j]; }
     }
     for (int h = tid; h < C::kHash; h += nt) hkey[h] = -1;
     if (tid == 0) { nu_s = 0; nover_s = 0; }

@@ -63,6 +63,8 @@ def lib():
                                              C.c_double, C.c_double]),
         "sevc_request_diagnostics": (C.c_int, [C.c_void_p]),
         "sevc_set_member_mesh": (C.c_int, [C.c_void_p, C.c_int32, P(Mesh)]),
+        "sevc_checkpoint": (C.c_int, [C.c_void_p, C.c_int32]),
+        "sevc_restore": (C.c_int, [C.c_void_p, C.c_int32]),
         "sevc_get_positions": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double)]),
         "sevc_autotune": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "sevc_velocity_config": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
@@ -267,6 +269,14 @@ class State:
 
     def request_diagnostics(self):
         _check(lib().sevc_request_diagnostics(self.h), self.h)
+
+    def checkpoint(self, slot=0):
+        """Keep the whole device state in checkpoint `slot` (device memory)."""
+        _check(lib().sevc_checkpoint(self.h, int(slot)), self.h)
+
+    def restore(self, slot=0):
+        """Put the state back as it was at checkpoint(slot)."""
+        _check(lib().sevc_restore(self.h, int(slot)), self.h)
 
     def set_mesh(self, m, mesh):
         """Restore member m's whole mesh (a HostMesh of the member's sizes)."""

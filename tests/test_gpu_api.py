@@ -73,3 +73,27 @@ def test_member_mesh_restore_replays_the_same_steps():
     bad["b_enabled"][10] = 0
     assert _code(lambda: st.set_mesh(0, bad)) == sevc.E_TOPOLOGY
     st.close()
+
+
+def test_device_checkpoint_restore_repeats_steps():
+    """sevc_checkpoint / sevc_restore keep and put back the whole device state
+    (meshes, scratch, loop state, with the step counter the nucleation stream
+    is keyed by): the steps after a restore repeat the first run bit for
+    bit, nucleations included, in a batch of two members."""
+    from _util import diff_fields
+    o = ref2k_options(dt=2e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                      nucleation_mode=sevc.NUCL_RATE, nucleation_rate=2e-2, seed=4, extra_capacity=0.05)
+    m = sevc.load_files(o)
+    o2 = ref2k_options(dt=1e-4)
+    st = sevc.State([o, o2], [m, sevc.load_files(o2)])
+    st.step(20)
+    st.checkpoint(3)
+    s_a = st.step(40)
+    a = [st.get_mesh(k) for k in range(2)]
+    st.restore(3)
+    s_b = st.step(40)
+    for k in range(2):
+        assert diff_fields(st.get_mesh(k), a[k]) == {}, k
+    assert s_a.n == s_b.n and s_a.n_nucleations == s_b.n_nucleations > 0 and s_a.t == s_b.t
+    assert _code(lambda: st.restore(9)) == sevc.E_INVALID_ARG
+    st.close()
