@@ -491,6 +491,25 @@ def bench_calibration(sevc, members, iters, horizon, window, ckpt, dt, seed0=200
                         "from (1, 1, SE = 0.3)"}
 
 
+def host_cpu():
+    """The host's CPU model (/proc/cpuinfo) and the threads this process may
+    use (its affinity mask) next to os.cpu_count()."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count() or 1}
+
+
 def cpu_baseline(opts, mesh, steps, omp=False):
     """The oracle timed on the host: serial (the checker itself), or its
     OpenMP build (per-entity physics loops on all threads OpenMP is given,
@@ -506,7 +525,7 @@ def cpu_baseline(opts, mesh, steps, omp=False):
     how = f"OpenMP build, {threads} threads" if omp else "1 thread"
     return {"value": st.n_vertex_steps / dt, "unit": "vertex-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the same 1M-grain workload on the FP64 oracle "
-                      f"(gcc -O2 -ffp-contract=off, {how}), {dt:.1f} s"}
+                      f"(gcc -O2 -ffp-contract=off, {how}), {dt:.1f} s", **host_cpu()}
 
 
 def copy_calibration(mib, reps=10):
@@ -550,6 +569,9 @@ def main():
     ap.add_argument("--calib-iters", type=int, default=50)
     ap.add_argument("--calib-steps", type=int, default=1000, help="calibration horizon (steps)")
     ap.add_argument("--calib-window", type=int, default=200, help="steps per topology-fixed window of the horizon")
+    ap.add_argument("--velocity", default="auto", metavar="PATH:TILE",
+                    help="velocity path and tile of the main workload (auto: sevc_autotune; counter runs fix "
+                         "it, since the tuner's timings change under the profiler)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM ceiling and PMC byte-counter scale)")
     args = ap.parse_args()
@@ -571,8 +593,11 @@ def main():
     opts, mesh = make_mesh(sevc, args.grains, args.seed)
     log(f"mesh: {mesh.nv} vertex slots, {mesh.ne} boundary slots, {mesh.ng} grain slots ({time.time() - t0:.1f} s)")
     st = sevc.State(opts, mesh)
-    tile = st.autotune()
-    vpath, _ = st.velocity_config()
+    if args.velocity == "auto":
+        st.autotune()
+    else:
+        st.set_velocity_config(*(int(x) for x in args.velocity.split(":")))
+    vpath, tile = st.velocity_config()
     st.step(args.warmup, graph=bool(args.graph))
     st.enable_timing(2)           # HIP events around velocity / translate / text only
     sync = st.step(0)

@@ -27,13 +27,33 @@ __device__ __forceinline__ double wrapd(double dest, double orig, double D) {
     return (d < -h) ? b : r;
 }
 
+// Correctly rounded square root of s in [1, 2] (or NaN): the sequence the
+// compiler emits for sqrt(double) -- reciprocal square root, one
+// Newton-Raphson refinement of (g, h) ~ (sqrt(s), 1/(2 sqrt(s))), then two
+// corrections by the exact residual s - g*g -- without its scaling of
+// arguments below 2^-767 and its pass-through of 0 and infinity, neither of
+// which can occur in this range.  Bit-identical to sqrt(s) there.
+__device__ __forceinline__ double sqrt_1_2(double s) {
+    const double y0 = __builtin_amdgcn_rsq(s);
+    double g = s * y0, h = y0 * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, s);
+    g = fma(d, h, g);
+    d = fma(-g, g, s);
+    g = fma(d, h, g);
+    return g;
+}
+
 // vector2_mag, geometry.h:217-223: p*sqrt(1+(q/p)^2), p = max |.|, q = min |.|
+// (1 + (q/p)^2 lies in [1, 2]; p = 0 gives NaN there, replaced by 0).
 __device__ __forceinline__ double mag(double x, double y) {
     const double ax = fabs(x), ay = fabs(y);
     const double p = ax > ay ? ax : ay;
     const double q = ax < ay ? ax : ay;
     const double r = q / p;
-    const double m = p * sqrt(1.0 + r * r);
+    const double m = p * sqrt_1_2(1.0 + r * r);
     return (p == 0) ? 0.0 : m;
 }
 

@@ -3372,6 +3372,7 @@ __global__ void k_selftest(int op, int n, const double* x, const double* y, doub
         case 5: r = sqrt(a); break;
         case 6: r = mag(a, b); break;
         case 7: r = wrapd(a, b, 1.0); break;
+        case 9: r = sqrt_1_2(a); break;
         case 8: {
             uint32_t c[4] = {(uint32_t)a, (uint32_t)b, 0u, 0u}, k[2] = {1u, 2u}, o[4];
             sevc_philox4x32_10(c, k, o);

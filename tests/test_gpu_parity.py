@@ -61,6 +61,20 @@ def test_selftest_math_bits():
     ys = y * 10.0 ** rng.integers(-150, 150, n).astype(np.float64)
     with np.errstate(over="ignore", under="ignore"):
             assert np.array_equal(sevc.selftest_math(4, xs, ys).view(np.uint64), (xs / ys).view(np.uint64))
+    # the square root of mag's argument 1 + (q/p)^2 in [1, 2], shortened for
+    # that range, equals the correctly rounded sqrt bit for bit
+    r = rng.uniform(0, 1, 200000)
+    s12 = np.concatenate([1.0 + r * r, rng.uniform(1, 2, 200000),
+                          [1.0, 2.0, np.nextafter(1.0, 2.0), np.nextafter(2.0, 1.0), 1.5, 1.25]])
+    assert np.array_equal(sevc.selftest_math(9, s12, np.zeros_like(s12)).view(np.uint64),
+                          np.sqrt(s12).view(np.uint64))
+    # mag on axis-aligned, equal-component, zero, tiny and huge vectors
+    special = np.array([0.0, -0.0, 1e-300, -1e-300, 3e-17, 0.5, -0.5, 1.0, 1e300, 5e-324])
+    ax, ay = np.meshgrid(special, special)
+    ax, ay = ax.ravel(), ay.ravel()
+    dev = sevc.selftest_math(6, ax, ay)
+    host = np.array([L.orc_mag(a, b) for a, b in zip(ax, ay)])
+    assert np.array_equal(dev.view(np.uint64), host.view(np.uint64))
     # Philox word 0 on the device equals the host restatement
     ctr = rng.integers(0, 2 ** 32, n).astype(np.float64)
     dev = sevc.selftest_math(8, ctr, ctr[::-1].copy())
