@@ -316,11 +316,14 @@ int sevc_autotune(sevc_state* s, int32_t* tile_grains);
  * A new state uses path 0 with 64 grains per tile. */
 int sevc_velocity_config(const sevc_state* s, int32_t* path, int32_t* tile);
 
-/* Longest candidate list that the vote rounds (polling_system,
- * topological.h:549-578) run in one workgroup's LDS, 0..2048 (default
- * 2048); longer lists run the general rounds over global memory.  Results
- * do not depend on it (extension; a tuning and test knob). */
-int sevc_set_poll_limit(sevc_state* s, int32_t max_fast);
+/* How the vote rounds (polling_system, topological.h:549-578) and the
+ * conflict check run, by the length of the step's candidate list: lists up
+ * to max_fast (0..2048, default 2048) with every candidate's mapped
+ * neighbours in registers, lists up to max_lds (0..16384, default 16384)
+ * with the flags and extinction times in LDS, longer lists by the general
+ * rounds over global memory.  Results do not depend on it (extension; a
+ * tuning and test knob). */
+int sevc_set_poll_limit(sevc_state* s, int32_t max_fast, int32_t max_lds);
 int sevc_set_velocity_config(sevc_state* s, int32_t path, int32_t tile);
 
 /* Copy only the vertex positions of a member (2 per vertex slot). */

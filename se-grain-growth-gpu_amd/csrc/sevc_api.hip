@@ -16,7 +16,7 @@
 
 namespace sevc {
 // kernels of sevc_kernels.hip
-__global__ void k_begin(Dev d);
+__global__ void k_begin(Dev d, int rescale_flipped);
 __global__ void k_count_all(Dev d);
 __global__ void k_member_reduce(Dev d);
 __global__ void k_edge_len_energy(Dev d, int only_nucl);
@@ -48,7 +48,6 @@ __global__ void k_flip_par(Dev d);
 __global__ void k_flip_serial(Dev d);
 __global__ void k_nucl_area_l1(Dev d);
 __global__ void k_nucl_area_up(Dev d);
-__global__ void k_nucl_factor(Dev d);
 __global__ void k_rate_cand(Dev d);
 __global__ void k_nucl_manager(Dev d);
 __global__ void k_nucl_insert(Dev d);
@@ -359,17 +358,17 @@ void launch_check(sevc_state* s, int phase) {
 void launch_step(sevc_state* s, bool diag) {
     Dev d = s->d;
     d.diag = diag ? 1 : 0;
-    launch(s, "begin", k_begin, 1, 1024, d);
     // Lengths come from k_text.  Energies change from step to step only with
     // GB_eps != 0; with GB_eps = 0 they are the scale, computed at the first
     // step after creation or after a new scale (for the members active then,
     // as the reference's per-step pass would).
-    if (diag || s->any_aniso || s->ben_dirty) launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
+    const bool all_energies = diag || s->any_aniso || s->ben_dirty;
+    launch(s, "begin", k_begin, 1, 1024, d, all_energies ? 0 : 1);
+    if (all_energies) launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
     s->ben_dirty = false;
     if (diag || s->any_analytic) launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
     if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     if (s->any_nucl) {
-        if (s->any_mc) launch(s, "nucl_factor", k_nucl_factor, d.nvb, kBlock, d);
         if (s->any_rate) launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
         launch(s, "nucl_manager", k_nucl_manager, s->B, kNuclThreads, d);
         launch(s, "nucl_insert", k_nucl_insert, kNuclInsGrid, 64, d);
@@ -386,7 +385,7 @@ void launch_step(sevc_state* s, bool diag) {
     launch(s, "text", k_text, d.neb, kBlock, d);
     if (s->any_events) {
         launch(s, "label", k_label, d.neb, kBlock, d);
-        launch(s, "poll_gather", k_poll_gather, kPollFast / 64, 64, d);
+        launch(s, "poll_gather", k_poll_gather, kPollBig / 64, 64, d);
         launch(s, "poll", k_poll, 1, 1024, d);
         launch(s, "scan3", k_scan3, d.ngb, kBlock, d);
         launch(s, "two_grain", k_two_grain, 1, 64, d);
@@ -521,6 +520,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     d.G = s->G;
     d.debug = s->any_debug ? 1 : 0;
     d.poll_fast = kPollFast;
+    d.poll_lds = kPollBig;
     int rc = SEVC_OK;
 #define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
     DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
@@ -528,7 +528,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     DA(d.mods, V);
     DA(d.nucl_r, V); DA(d.nucl_ori, V); DA(d.dE, V); DA(d.nfac, V); DA(d.vnuc, V);
     DA(d.bvx, 2 * E); DA(d.ben, E); DA(d.blen, E); DA(d.bt, E); DA(d.benab, E); DA(d.bcand, E); DA(d.conf, E);
-    DA(d.pgat, 25 * kPollFast); DA(d.pgt, kPollFast); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
+    DA(d.pgat, 25 * kPollBig); DA(d.pgt, kPollBig); DA(d.blab, E); DA(d.l_flipped, E + 16); DA(d.n_flipped, 1);
     DA(d.vmem, V); DA(d.emem, E); DA(d.gmem, G);
     DA(d.chk_bad, static_cast<size_t>(B));
     DA(d.gvx, G * S); DA(d.glen, G); DA(d.garea, G); DA(d.gdadt, G); DA(d.gori, G); DA(d.gse, G);
@@ -1007,10 +1007,11 @@ int sevc_autotune(sevc_state* s, int32_t* tile_grains) {
     return SEVC_OK;
 }
 
-int sevc_set_poll_limit(sevc_state* s, int32_t max_fast) {
-    if (!s || max_fast < 0 || max_fast > kPollFast) return SEVC_E_INVALID_ARG;
+int sevc_set_poll_limit(sevc_state* s, int32_t max_fast, int32_t max_lds) {
+    if (!s || max_fast < 0 || max_fast > kPollFast || max_lds < 0 || max_lds > kPollBig) return SEVC_E_INVALID_ARG;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->d.poll_fast = max_fast;
+    s->d.poll_lds = max_lds;
     drop_graphs(s);
     return SEVC_OK;
 }
@@ -1111,6 +1112,7 @@ int sevc_check_mesh_device(const sevc_mesh* m, int32_t* bad) {
     return rc;
 }
 
+extern "C++" {
 namespace {
 // Every device array a step reads and writes (the loop state of the members
 // aside): what a checkpoint holds.  Parameters (Params) are not state.
@@ -1130,6 +1132,7 @@ std::vector<std::pair<void*, size_t>> state_arrays(const sevc_state* s) {
 }
 constexpr int kMaxCkpt = 64;
 }  // namespace
+}  // extern "C++"
 
 int sevc_checkpoint(sevc_state* s, int32_t slot) {
     if (!s || slot < 0 || slot >= kMaxCkpt) return SEVC_E_INVALID_ARG;

@@ -30,7 +30,7 @@ namespace sevc {
 
 // Loop condition of main.cu:120 (time rule), grain rule, nucleation gate
 // main.cu:137-141; clears the per-step counters.
-__global__ void __launch_bounds__(1024) k_begin(Dev d) {
+__global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0)
         for (int c = 0; c < C_COUNT; c++) d.cnt[c] = 0;
@@ -59,9 +59,10 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d) {
     // GB_scaling factor until compute_boundary_energies (calculus.h:104-114)
     // runs at the next step's start.  Steps that skip that pass (GB_eps = 0,
     // where every other energy is the scale itself) recompute only the
-    // edges flipped since, for the members active in this step.
+    // edges flipped since, for the members active in this step (steps that
+    // run k_edge_len_energy recompute every energy there).
     __syncthreads();
-    const int nf = min(*d.n_flipped, d.E);
+    const int nf = rescale_flipped ? min(*d.n_flipped, d.E) : 0;
     for (int k = tid; k < nf; k += nt) {
         const int b = d.l_flipped[k];
         const int m = member_of_edge(d, b);
@@ -1295,7 +1296,7 @@ __global__ void k_label(Dev d) {
 // one-workgroup polling kernel reads contiguous columns only.
 __global__ void k_poll_gather(Dev d) {
     const int nc = d.cnt[C_CAND];
-    if (nc > d.poll_fast) return;
+    if (nc > d.poll_fast && nc > d.poll_lds) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         const int c = d.l_cand[k];
         const int a[2] = {d.bvx[2 * c], d.bvx[2 * c + 1]};
@@ -1307,7 +1308,7 @@ __global__ void k_poll_gather(Dev d) {
         int en[6], x0[6], x1[6];
 #pragma unroll
         for (int j = 0; j < 6; j++) {
-            d.pgat[j * kPollFast + k] = d.conf[eb[j]];
+            d.pgat[j * kPollBig + k] = d.conf[eb[j]];
             en[j] = eb[j] != c && d.benab[eb[j]];
             x0[j] = d.bvx[2 * eb[j]];
             x1[j] = d.bvx[2 * eb[j] + 1];
@@ -1324,8 +1325,8 @@ __global__ void k_poll_gather(Dev d) {
         for (int j = 0; j < 6; j++)
 #pragma unroll
             for (int r = 0; r < 3; r++)
-                d.pgat[(6 + 3 * j + r) * kPollFast + k] = (en[j] && yw[j][r] != eb[j]) ? d.conf[yw[j][r]] : -1;
-        d.pgat[24 * kPollFast + k] = bits;
+                d.pgat[(6 + 3 * j + r) * kPollBig + k] = (en[j] && yw[j][r] != eb[j]) ? d.conf[yw[j][r]] : -1;
+        d.pgat[24 * kPollBig + k] = bits;
     }
 }
 
@@ -1333,9 +1334,17 @@ __global__ void k_poll_gather(Dev d) {
 // then check_conflictive_boundaries (:83-121) and
 // unlabel_conflictive_boundaries (:129-155), over the compacted candidate
 // list.  Only vertices that touch a candidate can vote for one, so the
-// rounds are exact on the list.  One workgroup.
+// rounds are exact on the list.  One workgroup, in one of three ways by the
+// list's length: up to 2,048 candidates with their mapped neighbours in
+// registers; up to 16,384 with flags and t_ext in LDS and the mapped
+// neighbours re-read from k_poll_gather's columns; longer lists by the
+// general rounds over global memory.
 __global__ void __launch_bounds__(1024) k_poll(Dev d) {
     __shared__ int changed;
+    // LDS of the first two ways: [cf int, cid int, ct double] x kPollFast,
+    // or [ct double, cf byte] x kPollBig
+    __shared__ __align__(16) unsigned char pbuf[9 * kPollBig];
+    static_assert(16 * kPollFast <= 9 * kPollBig, "the register way's arrays fit the buffer");
     const int nc = d.cnt[C_CAND];
     if (nc == 0) return;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -1357,8 +1366,9 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         // and k_label lists every enabled candidate).  The rounds and the
         // check then read LDS only.
         constexpr int kPer = kPollFast / 1024;
-        __shared__ int cf[kPollFast], cid[kPollFast];
-        __shared__ double ct[kPollFast];
+        int* cf = reinterpret_cast<int*>(pbuf);
+        int* cid = cf + kPollFast;
+        double* ct = reinterpret_cast<double*>(pbuf + 8 * kPollFast);
         int ep[kPer][6], bits[kPer];
         bool two[kPer];
 #pragma unroll
@@ -1371,8 +1381,8 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
                 cf[k] = 1;
             }
 #pragma unroll
-            for (int j = 0; j < 6; j++) ep[u][j] = on ? d.pgat[j * kPollFast + k] : -1;
-            bits[u] = on ? d.pgat[24 * kPollFast + k] : 0;
+            for (int j = 0; j < 6; j++) ep[u][j] = on ? d.pgat[j * kPollBig + k] : -1;
+            bits[u] = on ? d.pgat[24 * kPollBig + k] : 0;
         }
         __syncthreads();
         // vertices_vote (topological.h:362-386) at end e of the candidate at
@@ -1428,7 +1438,7 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
 #pragma unroll
             for (int j = 0; j < 6; j++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) pw[j][r] = keep[u] ? d.pgat[(6 + 3 * j + r) * kPollFast + k] : -1;
+                for (int r = 0; r < 3; r++) pw[j][r] = keep[u] ? d.pgat[(6 + 3 * j + r) * kPollBig + k] : -1;
 #pragma unroll
             for (int j = 0; j < 6; j++) {
                 if (!keep[u] || !((bits[u] >> (2 + j)) & 1) || flagp(ep[u][j])) continue;
@@ -1458,6 +1468,91 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
                 d.blab[c] = (uint8_t)(keep[u] ? 7 : 3);     // + the label after polling (bit 2)
                 d.conf[c] = -1;
             }
+        }
+        return;
+    } else if (nc <= d.poll_lds && nt * 32 >= kPollBig) {
+        // The same rounds and check with the flags (bytes) and t_ext of up
+        // to kPollBig candidates in LDS; each candidate's mapped neighbours
+        // and bits are read again from k_poll_gather's columns (coalesced,
+        // L2-resident) where needed, and the per-candidate results of a
+        // lane are bits of one word (at most 16 candidates per lane).
+        double* ct = reinterpret_cast<double*>(pbuf);
+        uint8_t* cf = pbuf + 8 * kPollBig;
+        const int* G = d.pgat;
+        for (int k = tid; k < nc; k += nt) {
+            ct[k] = d.pgt[k];
+            cf[k] = 1;
+        }
+        __syncthreads();
+        // vertices_vote at end e of the candidate at position k (as in the
+        // register way above)
+        auto votes_self = [&](int e, int k, int bk) {
+            double min_t = 99999999;
+            int arg = -1;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const int p = G[(3 * e + r) * kPollBig + k];
+                if (p >= 0 && cf[p] && ct[p] <= min_t) { min_t = ct[p]; arg = p; }
+            }
+            return arg >= 0 ? arg == k : ((bk >> e) & 1) != 0;
+        };
+        for (;;) {
+            if (tid == 0) changed = 0;
+            unsigned two = 0;
+            for (int u = 0, k = tid; k < nc; u++, k += nt) {
+                const int bk = G[24 * kPollBig + k];
+                if (cf[k] && votes_self(0, k, bk) && votes_self(1, k, bk)) two |= 1u << u;
+            }
+            __syncthreads();
+            for (int u = 0, k = tid; k < nc; u++, k += nt)
+                if ((two >> u) & 1u)
+#pragma unroll
+                    for (int j = 0; j < 6; j++) {
+                        const int p = G[j * kPollBig + k];
+                        if (p >= 0 && p != k && cf[p]) { cf[p] = 0; changed = 1; }
+                    }
+            __syncthreads();
+            const int ch = changed;
+            __syncthreads();
+            if (!ch) break;
+        }
+        auto flagp = [&](int p) { return p >= 0 && cf[p] != 0; };
+        unsigned keep = 0;
+        for (int u = 0, k = tid; k < nc; u++, k += nt) {
+            bool kp = cf[k] != 0;
+            if (kp) {
+                const int bk = G[24 * kPollBig + k];
+                int ep[6];
+#pragma unroll
+                for (int j = 0; j < 6; j++) ep[j] = G[j * kPollBig + k];
+                for (int j = 0; j < 6 && kp; j++) {
+                    if (!((bk >> (2 + j)) & 1) || flagp(ep[j])) continue;
+                    int pbi = -1, pbe = -1;
+                    const int e3 = 3 * (j / 3);
+#pragma unroll
+                    for (int r = 2; r >= 0; r--) {           // first in slot order
+                        if (flagp(ep[e3 + r])) pbi = ep[e3 + r];
+                        const int pw = G[(6 + 3 * j + r) * kPollBig + k];
+                        if (flagp(pw)) pbe = pw;
+                    }
+                    if (pbi < 0 || pbe < 0) continue;
+                    const double ti = ct[pbi], tb = ct[pbe];
+                    const int bi = d.l_cand[pbi], be = d.l_cand[pbe];
+                    int w;
+                    if (ti == tb) w = bi < be ? bi : be;
+                    else w = ti < tb ? bi : be;
+                    if (w != d.l_cand[k]) kp = false;
+                }
+            }
+            if (kp) keep |= 1u << u;
+        }
+        __syncthreads();                 // every flag read
+        for (int u = 0, k = tid; k < nc; u++, k += nt) {
+            const int c = d.l_cand[k];
+            const bool kp = (keep >> u) & 1u;
+            d.bcand[c] = kp ? 1 : 0;
+            d.blab[c] = (uint8_t)(kp ? 7 : 3);
+            d.conf[c] = -1;
         }
         return;
     } else for (;;) {
@@ -2634,11 +2729,11 @@ __device__ __forceinline__ int generate_grain(const Dev& d, const Params& P, int
 
 // compute_nucleation_factor, nucleation2.cu:239-282, with the random pair
 // Philox(seed; step, local vertex id, stream 1).
-__global__ void k_nucl_factor(Dev d) {
-    TABLE_BLOCK(d.vblk, d.nvb)
-    if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_MC) return;
-    if (i >= bd_.e || !d.ven[i]) return;
-    const int v = i;
+// compute_nucleation_factor (nucleation2.cu) for one alive vertex v of a
+// Monte Carlo member: the nucleus radius and orientation from the vertex's
+// own Philox pair, dE and the factor dE - dE'.  The manager needs it only
+// for the vertex it picks, so it runs there, for that vertex alone.
+__device__ void nucl_factor_at(const Dev& d, const Params& P, const MState& M, int v) {
     d.dE[v] = 0;
     d.nfac[v] = 0;
     double ALs[3];
@@ -2967,6 +3062,7 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
         if (k >= n_alive) k = n_alive - 1;
         const int A = find_kth(d.vblk, rs, d.bc_v, d.ven, 1, k);
         if (tid == 0 && A >= 0) {
+            nucl_factor_at(d, P, M, A);
             d.vnuc[A] = 0;
             if (d.dE[A] < 0) d.vnuc[A] = 1;
             else if (d.nfac[A] < 0) {

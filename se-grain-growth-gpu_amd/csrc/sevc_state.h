@@ -16,7 +16,8 @@ constexpr int kFixStages = 3;        // 0: two-grain, 1: T2 removal, 2: flips
 constexpr int kTile = 64;            // grains per velocity tile (pass 1)
 constexpr int kMaxHistBins = 64;     // bins of the device area histogram
 constexpr int kNuclThreads = 1024;   // workgroup of the nucleation manager (one per member)
-constexpr int kPollFast = 2048;      // candidate lists up to this length poll from LDS (two per lane)
+constexpr int kPollFast = 2048;      // candidate lists up to this length poll with registers and LDS (two per lane)
+constexpr int kPollBig = 16384;      // lists up to this length poll with flags and t_ext in LDS
 constexpr int kCountWaves = 4;       // table blocks per workgroup of k_count_all (one wave each)
 
 // Per-member run parameters (from sevc_options).
@@ -67,7 +68,7 @@ struct Dev {
     double* nucl_r; double* nucl_ori; double* dE; double* nfac; uint8_t* vnuc;
     // boundaries
     int* bvx; double* ben; double* blen; double* bt; uint8_t* benab; uint8_t* bcand; int* conf;
-    int* pgat; double* pgt;          // polling gathers: 25 columns of kPollFast, t_ext
+    int* pgat; double* pgt;          // polling gathers: 25 columns of kPollBig, t_ext
     uint8_t* blab;                  // per edge, from k_label: bit 0 candidate and enabled, bit 1 enabled
     int* l_flipped; int* n_flipped;  // edges flipped since the last step start (energies to rescale)
     int* vmem; int* emem; int* gmem;  // owning member of each vertex / edge / grain slot (batches)
@@ -98,7 +99,8 @@ struct Dev {
     int diag;                             // compute export diagnostics this step
     int debug;                            // a member runs the checks of check_data (debug_lvl >= 1)
     int* chk_bad;                         // violations per member found by the last check
-    int poll_fast;                        // longest candidate list polled in LDS (<= kPollFast)
+    int poll_fast;                        // longest candidate list polled from registers (<= kPollFast)
+    int poll_lds;                         // longest candidate list polled from LDS (<= kPollBig)
 };
 
 }  // namespace sevc

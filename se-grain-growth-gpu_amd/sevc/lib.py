@@ -69,7 +69,7 @@ def lib():
         "sevc_autotune": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "sevc_velocity_config": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "sevc_set_velocity_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
-        "sevc_set_poll_limit": (C.c_int, [C.c_void_p, C.c_int32]),
+        "sevc_set_poll_limit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -253,9 +253,10 @@ class State:
         _check(lib().sevc_velocity_config(self.h, C.byref(p), C.byref(t)), self.h)
         return p.value, t.value
 
-    def set_poll_limit(self, max_fast):
-        """Longest candidate list polled in LDS (0: always the general rounds)."""
-        _check(lib().sevc_set_poll_limit(self.h, int(max_fast)), self.h)
+    def set_poll_limit(self, max_fast, max_lds=16384):
+        """Candidate lists up to max_fast poll from registers, up to max_lds
+        from LDS, longer ones with the general rounds (same results)."""
+        _check(lib().sevc_set_poll_limit(self.h, int(max_fast), int(max_lds)), self.h)
 
     def set_velocity_config(self, path, tile):
         _check(lib().sevc_set_velocity_config(self.h, int(path), int(tile)), self.h)

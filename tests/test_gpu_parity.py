@@ -188,18 +188,19 @@ def test_autotune_keeps_results():
         st.close()
 
 
-@pytest.mark.parametrize("limit", [0, 64])
-def test_poll_general_rounds_match_lds_rounds(limit):
-    """The vote rounds in LDS (lists up to 2,048 candidates) and the general
-    rounds over global memory give the same flips: the same inputs run
-    with the LDS path switched off (limit 0) or limited to 64 candidates
-    stay bit-identical to the oracle, as the default run does."""
+@pytest.mark.parametrize("limit,lds", [(0, 0), (64, 0), (0, 16384), (64, 16384), (0, 200)])
+def test_poll_general_rounds_match_lds_rounds(limit, lds):
+    """The three ways of running the vote rounds and the conflict check --
+    mapped neighbours in registers (lists up to `limit`), flags in LDS
+    (lists up to `lds`) and the general rounds over global memory -- give
+    the same flips: the same inputs stay bit-identical to the oracle with
+    each way switched on or off, as the default run does."""
     o, mesh = _generated(3000, 21, dt=1e-5, flip_threshold=0.3 * np.sqrt(1.0 / 3000))
     st, orc = sevc.State(o, mesh), Oracle(o, mesh)
-    st.set_poll_limit(limit)
+    st.set_poll_limit(limit, lds)
     flips = 0
     for chunk in (5, 20):
-        print(f"  poll limit {limit}: chunk of {chunk}", flush=True)
+        print(f"  poll limits {limit}, {lds}: chunk of {chunk}", flush=True)
         s1 = st.step(chunk)
         rc, s2 = orc.step(chunk)
         assert rc == 0 and s1.status == 0
