@@ -510,22 +510,85 @@ def host_cpu():
     return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count() or 1}
 
 
+class _pinned:
+    """Run the enclosed block on one core of this process's affinity mask
+    (the serial baseline), restoring the mask afterwards."""
+    def __enter__(self):
+        try:
+            self.mask = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {min(self.mask)})
+        except (AttributeError, OSError):
+            self.mask = None
+        return self
+
+    def __exit__(self, *exc):
+        if self.mask is not None:
+            os.sched_setaffinity(0, self.mask)
+        return False
+
+
 def cpu_baseline(opts, mesh, steps, omp=False):
-    """The oracle timed on the host: serial (the checker itself), or its
-    OpenMP build (per-entity physics loops on all threads OpenMP is given,
-    topology serial; the same results), a multi-core baseline only."""
+    """The oracle timed on the host: serial (the checker itself, pinned to
+    one core), or its OpenMP build (per-entity physics loops on all threads
+    OpenMP is given, topology serial; the same results), a multi-core
+    baseline only."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from _util import Oracle  # the oracle is the CPU comparator
     orc = Oracle(opts, mesh, omp=omp)
-    t0 = time.perf_counter()
-    rc, st = orc.step(steps)
-    dt = time.perf_counter() - t0
+    if omp:
+        t0 = time.perf_counter()
+        rc, st = orc.step(steps)
+        dt = time.perf_counter() - t0
+    else:
+        with _pinned():
+            t0 = time.perf_counter()
+            rc, st = orc.step(steps)
+            dt = time.perf_counter() - t0
     orc.close()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if omp else 1
-    how = f"OpenMP build, {threads} threads" if omp else "1 thread"
+    how = f"OpenMP build, {threads} threads" if omp else "1 thread pinned to one core"
     return {"value": st.n_vertex_steps / dt, "unit": "vertex-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the same 1M-grain workload on the FP64 oracle "
                       f"(gcc -O2 -ffp-contract=off, {how}), {dt:.1f} s", **host_cpu()}
+
+
+def bench_config1(sevc, seed=0):
+    """The 2,000-grain workload run to its stop (50 % of the grains
+    removed): the reference's 2K tessellation, SE = 0.5 U[0,1), rate
+    nucleation 1e-3 SE per grain and step, adaptive dt, flip threshold
+    1e-4.  The whole run on the GPU against the same run on the serial
+    oracle (pinned to one core); both end in the same state."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from _util import Oracle, ref2k_options  # the oracle is the CPU comparator
+    o = ref2k_options(dt=2e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                      nucleation_mode=sevc.NUCL_RATE, nucleation_rate=1e-3, nucleation_scaling=0.9,
+                      seed=seed, extra_capacity=0.05, leave_percent=0.5)
+    mesh = sevc.load_files(o)
+    warm = sevc.State(o, mesh)
+    warm.step(3)
+    warm.close()
+    st = sevc.State(o, mesh)
+    t0 = time.perf_counter()
+    s = st.step(100000)                  # ends at the stop rule
+    t_gpu = time.perf_counter() - t0
+    orc = Oracle(o, mesh)
+    with _pinned():
+        t0 = time.perf_counter()
+        rc, s2 = orc.step(100000)
+        t_cpu = time.perf_counter() - t0
+    same = (s.n == s2.n and s.n_vertex_steps == s2.n_vertex_steps and s.n_flips == s2.n_flips
+            and s.n_grains == s2.n_grains)
+    orc.close()
+    st.close()
+    log(f"config 1 (2K grains to 50 %): {s.n} steps, GPU {t_gpu:.3f} s, oracle {t_cpu:.2f} s "
+        f"({t_cpu / t_gpu:.1f}x), same end state counts: {same}")
+    return {"steps": s.n, "vertex_steps": s.n_vertex_steps, "flips": s.n_flips, "removals": s.n_removals,
+            "nucleations": s.n_nucleations, "grains_left": s.n_grains, "gpu_s": t_gpu,
+            "gpu_ms_per_step": t_gpu / max(1, s.n) * 1e3, "gpu_value": s.n_vertex_steps / t_gpu,
+            "cpu_s": t_cpu, "cpu_value": s2.n_vertex_steps / t_cpu, "cpu_cores": 1, "speedup": t_cpu / t_gpu,
+            "same_counts_as_oracle": bool(same), **host_cpu(),
+            "workload": "reference 2K tessellation, SE = 0.5 U[0,1), rate nucleation 1e-3 SE per grain and step, "
+                        f"Philox seed {seed}, adaptive dt (bound 2e-4), flip threshold 1e-4, run to 50 % removed"}
 
 
 def copy_calibration(mib, reps=10):
@@ -553,7 +616,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grains", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for cpu_baseline (0: skip)")
+    ap.add_argument("--config1", type=int, default=1, help="time the 2K workload to its stop, GPU and oracle (0: skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the step as a HIP graph (no per-kernel timing)")
     ap.add_argument("--graph-steps", type=int, default=200, help="extra steps timed as HIP graph replays (0: skip)")
     ap.add_argument("--scale-steps", type=int, default=200,
@@ -683,6 +747,8 @@ def main():
     if args.calib_members > 0:
         line["calibration"] = bench_calibration(sevc, args.calib_members, args.calib_iters, args.calib_steps,
                                                 args.calib_window, 50, 5e-7)
+    if args.config1:
+        line["config1"] = bench_config1(sevc)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
         line["cpu_baseline_openmp"] = cpu_baseline(opts, mesh, 2 * args.cpu_steps, omp=True)

@@ -93,7 +93,9 @@ struct sevc_state {
     bool any_debug = false;                  // a member with debug_lvl >= 1 (checks inside the step)
     bool ben_dirty = true;                   // boundary energies not computed since creation / a new scale
     bool diag_next = false;
+    bool any_stop = false;                   // a member with a stop rule (tmax, nmax, leave_percent)
     std::vector<MState> ms_host, ms_prev;
+    std::vector<MState> ms_poll;             // members' state read while a call runs (all finished?)
     // graph replay: gexec holds kGraphSteps steps, gexec1 one step
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -663,6 +665,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
         P.n_grains0 = ngr;
         P.gate_grains = o.nucleation_gate >= 0 ? o.nucleation_gate : 5 * ngr / 6;
         P.min_grains = o.leave_percent < 0 ? -1 : static_cast<int>(std::floor(o.leave_percent * ngr));
+        if (P.tmax > 0 || P.nmax > 0 || P.min_grains > 0) s->any_stop = true;
         P.voff = vo; P.nv = me.n_vertices; P.eoff = eo; P.ne = me.n_boundaries; P.goff = go; P.ng = me.n_grains;
         MState& M = s->ms_host[static_cast<size_t>(m)];
         M.curr_grains = ngr;
@@ -803,9 +806,27 @@ static int finish_call(sevc_state* s) {
     return SEVC_OK;
 }
 
+// Every member finished (stop rule or error)?  Steps after that do nothing
+// on the device, so a call stops launching them.  Read every stop_poll(s)
+// steps: 64 when a member has a stop rule, else 512 (an error can still
+// end every member).
+static int stop_poll(const sevc_state* s) { return s->any_stop ? 64 : 512; }
+
+static bool all_done(sevc_state* s) {
+    if (s->ms_poll.size() != static_cast<size_t>(s->B)) s->ms_poll.resize(static_cast<size_t>(s->B));
+    if (hipMemcpyAsync(s->ms_poll.data(), s->d.ms, sizeof(MState) * s->ms_poll.size(), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return false;                            // the call's end reports the error
+    for (const MState& M : s->ms_poll)
+        if (!M.done) return false;
+    return true;
+}
+
 int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     if (!s || n_steps < 0) return SEVC_E_INVALID_ARG;
     if (n_steps > 0) s->win.ready = false;       // the topology may change
+    const int poll = stop_poll(s);
     for (int k = 0; k < n_steps; ++k) {
         bool diag = s->diag_next;
         s->diag_next = false;
@@ -814,6 +835,7 @@ int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
             (void)hipStreamSynchronize(s->stream);
             collect_timers(s);
         }
+        if ((k + 1) % poll == 0 && k + 1 < n_steps && all_done(s)) break;
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return fail(s, SEVC_E_HIP, std::string("launch: ") + hipGetErrorString(le));
@@ -843,7 +865,8 @@ int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
         s->ben_dirty = dirty;
         s->timing = timing;
     }
-    for (int k = 0; k < n_steps;) {
+    const int poll = stop_poll(s);
+    for (int k = 0, next_poll = poll; k < n_steps;) {
         if (s->diag_next || s->ben_dirty) {
             const bool diag = s->diag_next;
             s->diag_next = false;
@@ -855,6 +878,10 @@ int sevc_step_graph(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
         } else {
             HIPCHK(hipGraphLaunch(s->gexec1, s->stream));
             k += 1;
+        }
+        if (k >= next_poll && k < n_steps) {
+            next_poll = k + poll;
+            if (all_done(s)) break;
         }
     }
     int rc = finish_call(s);

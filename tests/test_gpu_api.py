@@ -97,3 +97,27 @@ def test_device_checkpoint_restore_repeats_steps():
     assert s_a.n == s_b.n and s_a.n_nucleations == s_b.n_nucleations > 0 and s_a.t == s_b.t
     assert _code(lambda: st.restore(9)) == sevc.E_INVALID_ARG
     st.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_call_ends_when_every_member_has_stopped(graph):
+    """A call asked for far more steps than the stop rule allows (the
+    2,000-grain run to 50 % removed) stops launching once every member has
+    finished: it returns in well under the time 100,000 launched steps take
+    (about 10 s), with the oracle's step count and end state."""
+    import time
+    from _util import Oracle, diff_fields
+    o = ref2k_options(dt=2e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                      nucleation_mode=sevc.NUCL_RATE, nucleation_rate=1e-3, nucleation_scaling=0.9, seed=3,
+                      extra_capacity=0.05, leave_percent=0.5)
+    mesh = sevc.load_files(o)
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    t0 = time.perf_counter()
+    s1 = st.step(100000, graph=graph)
+    el = time.perf_counter() - t0
+    rc, s2 = orc.step(100000)
+    assert rc == 0 and s1.status == 0 and s1.done
+    assert s1.n == s2.n and s1.n_grains == s2.n_grains <= 1000
+    assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+    assert el < 3.0, el
+    st.close()
