@@ -571,6 +571,13 @@ def bench_config1(sevc, seed=0):
     t0 = time.perf_counter()
     s = st.step(100000)                  # ends at the stop rule
     t_gpu = time.perf_counter() - t0
+    sg = sevc.State(o, mesh)             # the same run replayed as captured HIP graphs of 8 steps
+    sg.step(0, graph=True)               # capture outside the timed region
+    t0 = time.perf_counter()
+    s_g = sg.step(100000, graph=True)
+    t_graph = time.perf_counter() - t0
+    same_graph = s_g.n == s.n and s_g.n_flips == s.n_flips and s_g.n_grains == s.n_grains
+    sg.close()
     orc = Oracle(o, mesh)
     with _pinned():
         t0 = time.perf_counter()
@@ -580,11 +587,12 @@ def bench_config1(sevc, seed=0):
             and s.n_grains == s2.n_grains)
     orc.close()
     st.close()
-    log(f"config 1 (2K grains to 50 %): {s.n} steps, GPU {t_gpu:.3f} s, oracle {t_cpu:.2f} s "
-        f"({t_cpu / t_gpu:.1f}x), same end state counts: {same}")
+    log(f"config 1 (2K grains to 50 %): {s.n} steps, GPU {t_gpu:.3f} s ({t_graph:.3f} s as graph replays), "
+        f"oracle {t_cpu:.2f} s ({t_cpu / t_gpu:.1f}x), same end state counts: {same and same_graph}")
     return {"steps": s.n, "vertex_steps": s.n_vertex_steps, "flips": s.n_flips, "removals": s.n_removals,
             "nucleations": s.n_nucleations, "grains_left": s.n_grains, "gpu_s": t_gpu,
             "gpu_ms_per_step": t_gpu / max(1, s.n) * 1e3, "gpu_value": s.n_vertex_steps / t_gpu,
+            "graph_s": t_graph, "graph_ms_per_step": t_graph / max(1, s.n) * 1e3, "graph_same_counts": bool(same_graph),
             "cpu_s": t_cpu, "cpu_value": s2.n_vertex_steps / t_cpu, "cpu_cores": 1, "speedup": t_cpu / t_gpu,
             "same_counts_as_oracle": bool(same), **host_cpu(),
             "workload": "reference 2K tessellation, SE = 0.5 U[0,1), rate nucleation 1e-3 SE per grain and step, "
