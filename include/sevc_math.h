@@ -89,13 +89,17 @@ SEVC_HD double sevc_atan(double x) {
     int sign = 0, flag = 0;
     if (x == 0.0) return x;
     if (x < 0.0) { sign = 1; x = -x; }
+    /* the three reductions as one division: -(1/x) = (-1)/x and x/1 = x
+     * exactly, so each branch keeps Cephes's value */
+    double num = x, den = 1.0;
     if (x > T3P8) {
-        y = SEVC_PIO2; flag = 1; x = -(1.0 / x);
+        y = SEVC_PIO2; flag = 1; num = -1.0; den = x;
     } else if (x <= 0.66) {
         y = 0.0;
     } else {
-        y = SEVC_PIO4; flag = 2; x = (x - 1.0) / (x + 1.0);
+        y = SEVC_PIO4; flag = 2; num = x - 1.0; den = x + 1.0;
     }
+    x = num / den;
     z = x * x;
     {
         double p = -8.750608600031904122785e-1;
