@@ -246,6 +246,7 @@ int set_tiles(sevc_state* s, int T) {
         hipSuccess)
         return SEVC_E_HIP;
     s->d.ntile = static_cast<int>(tt.size());
+    s->d.tile_T = T;
     s->tile_grains = T;
     // a captured step holds the old table size
     drop_graphs(s);

@@ -197,10 +197,22 @@ __device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
     }
 }
 
+// Block lb of one of the three entity tables.  A single member's table is
+// runs of kBlock consecutive slots from 0, so its bounds are computed: the
+// block's own loads then need not wait for a table load.
+__device__ __forceinline__ Blk table_blk(const Dev& d, const Blk* tbl, int lb) {
+    if (d.B == 1 && (tbl == d.vblk || tbl == d.eblk || tbl == d.gblk)) {
+        const int total = tbl == d.vblk ? d.V : (tbl == d.eblk ? d.E : d.G);
+        const int b = lb * kBlock;
+        return Blk{0, b, min(b + kBlock, total)};
+    }
+    return tbl[lb];
+}
+
 #define TABLE_BLOCK(tbl, nblk)                                   \
     int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);        \
     if (lb_ >= (nblk)) return;                                   \
-    const Blk bd_ = (tbl)[lb_];                                  \
+    const Blk bd_ = table_blk(d, (tbl), lb_);                    \
     const Params& P = d.prm[bd_.m];                              \
     MState& M = d.ms[bd_.m];                                     \
     (void)P; (void)M;                                            \

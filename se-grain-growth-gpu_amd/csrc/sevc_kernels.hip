@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
     } else {
         lb -= d.ngb + d.neb; tbl = d.vblk; en = d.ven; out = d.bc_v;
     }
-    const Blk bd = tbl[lb];
+    const Blk bd = table_blk(d, tbl, lb);
     uint8_t f[kBlock / 64];
 #pragma unroll
     for (int j = 0; j < kBlock / 64; j++) {
@@ -287,7 +287,13 @@ __device__ __forceinline__ void analytic_force(const Dev& d, const Params& P, in
 // area the reference's left-to-right shoelace sum, so the results are
 // bit-identical.  Pass 2 sends vertices with dom_mod(x) != x or
 // dom_mod(y) != y to the two-pass path, which this form does not cover.
-constexpr int kTilePts = 480;   // with the per-slot and per-grain arrays: 8 workgroups of 256 per CU
+//
+// The (grain, slot) pairs of the shifted areas are dealt to the lanes in
+// order of their loop length after slot k, n - k (then grain), so that the
+// lanes of a wave run that loop, the longer one, for the same number of
+// iterations: pair (g, k) goes to position start(n - k) + (rank of g among
+// the tile's grains with at least n - k entries).
+constexpr int kTilePts = 448;   // with the per-slot and per-grain arrays: 8 workgroups of 256 per CU
 static_assert(kTilePts >= kMaxStride, "a tile must hold the longest grain list");
 __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     __shared__ double2 Ps[kTilePts];
@@ -295,33 +301,55 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
     __shared__ int Ls[kTilePts];
     __shared__ unsigned char pgl[kTilePts];
     __shared__ unsigned char vmask[kTilePts]; // bit j: vg[j] of the slot's vertex is the slot's grain
+    __shared__ unsigned short ord[kTilePts];  // the list entry of each term lane
+    static_assert(kTilePts <= 65535, "entries fit 16 bits");
     __shared__ int off[kTile + 1];
     __shared__ double area_s[kTile], se_s[kTile];
     __shared__ unsigned char en_s[kTile];
     __shared__ int gend_s;
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (lb >= d.ntile) return;
-    const Blk bd = d.gtile[lb];
+    // a single member's tiles are consecutive runs of tile_T grains: their
+    // bounds need no table load, so the tile's first loads (below) issue at
+    // once, together with the member's state
+    Blk bd;
+    if (d.B == 1) {
+        bd.m = 0;
+        bd.b = lb * d.tile_T;
+        bd.e = min(bd.b + d.tile_T, d.G);
+    } else {
+        bd = d.gtile[lb];
+    }
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // the first kRowEntries list entries of the tile's grains (one 16-byte
+    // load per thread), staged in the delta array until the first deltas
+    // overwrite them, and the first sub-tile's enable flags, lengths and
+    // stored energies: loaded before the member test (in bounds whatever
+    // the member's state); a tile split for its length reads its later
+    // parts itself
+    constexpr int kRowEntries = 16;
+    static_assert(kRowEntries * kTile * 4 <= (int)sizeof(Ds), "rows fit the delta array");
+    int* rowbuf = reinterpret_cast<int*>(Ds);
+    const bool rows = (d.S % 4 == 0) && d.S >= kRowEntries;
+    int4 row0 = make_int4(0, 0, 0, 0);
+    const int grow = bd.b + tid / (kRowEntries / 4);
+    const bool has_row = rows && tid < kTile * (kRowEntries / 4) && grow < bd.e;
+    if (has_row) row0 = reinterpret_cast<const int4*>(glist(d, grow))[tid % (kRowEntries / 4)];
+    uint8_t en0 = 0;
+    int len0 = 0;
+    double se0 = 0.0;
+    if (tid < kTile && bd.b + tid < bd.e) {
+        en0 = d.genab[bd.b + tid];
+        len0 = d.glen[bd.b + tid];
+        se0 = d.gse[bd.b + tid];
+    }
     const Params& P = d.prm[bd.m];
     const MState& M = d.ms[bd.m];
     // tuning: timed by sevc_autotune on any member, without writing areas
     // (the terms it writes are scratch of the next step's pass 2)
     if ((!M.active && !tuning) || P.vel_model != SEVC_VEL_FD) return;
     const double D = P.D;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    // the first kRowEntries list entries of the entry's grains, loaded
-    // together with their lengths (one 16-byte load per thread) and staged
-    // in the delta array until the first tile's deltas overwrite them; later
-    // tiles of the entry read the lists themselves
-    constexpr int kRowEntries = 16;
-    static_assert(kRowEntries * kTile * 4 <= (int)sizeof(Ds), "rows fit the delta array");
-    int* rowbuf = reinterpret_cast<int*>(Ds);
-    const bool rows = (d.S % 4 == 0) && d.S >= kRowEntries;
-    if (rows && tid < kTile * (kRowEntries / 4)) {
-        const int g = bd.b + tid / (kRowEntries / 4), c = tid % (kRowEntries / 4);
-        if (g < bd.e)
-            reinterpret_cast<int4*>(rowbuf)[tid] = reinterpret_cast<const int4*>(glist(d, g))[c];
-    }
+    if (has_row) reinterpret_cast<int4*>(rowbuf)[tid] = row0;
     for (int gs = bd.b; gs < bd.e;) {
         const bool from_rows = rows && gs == bd.b;
         if (tid < kTile) {
@@ -329,13 +357,14 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             // longest prefix of grains whose lists fit in kTilePts points
             const int g = gs + tid;
             const bool in = g < bd.e;
-            const uint8_t en8 = in ? d.genab[g] : 0;
-            const int len = in ? d.glen[g] : 0;
+            const bool first = gs == bd.b;
+            const uint8_t en8 = first ? en0 : (in ? d.genab[g] : 0);
+            const int len = first ? len0 : (in ? d.glen[g] : 0);
             const bool en = en8 != 0;
             const int n = en ? len : 0;
             // the values the later phases need from global memory, loaded here
             en_s[tid] = en ? 1 : 0;
-            se_s[tid] = in ? d.gse[g] : 0.0;
+            se_s[tid] = first ? se0 : (in ? d.gse[g] : 0.0);
             int inc = n;
             for (int o = 1; o < kTile; o <<= 1) {
                 const int t = __shfl_up(inc, o, kTile);
@@ -343,6 +372,7 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             }
             const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
             const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
+            const int nm = tid < cnt ? n : 0;   // entries of this sub-tile's grain
             if (tid < cnt) {
                 off[tid] = inc - n;
                 for (int q = inc - n; q < inc; q++) pgl[q] = (unsigned char)tid;   // the grain of each point
@@ -351,6 +381,16 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             if (tid == 0) {
                 gend_s = gs + cnt;
                 if (cnt == 0) off[0] = 0;
+            }
+            // term lanes by loop length r = n - k: every grain with n >= r
+            // has one slot with that length (k = n - r)
+            const unsigned long long below = (1ull << tid) - 1ull;
+            int base = 0;
+            for (int r = 1;; r++) {
+                const unsigned long long has = __ballot(nm >= r);
+                if (!has) break;
+                if (nm >= r) ord[base + (int)__popcll(has & below)] = (unsigned short)(inc - r);
+                base += (int)__popcll(has);
             }
         }
         __syncthreads();
@@ -394,7 +434,8 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             area_s[g] = a;
         }
         __syncthreads();
-        for (int p = tid; p < npts; p += nt) {
+        for (int q = tid; q < npts; q += nt) {
+            const int p = ord[q];
             const int g = pgl[p], b0 = off[g], n = off[g + 1] - b0, k = p - b0;
             const int v = Ls[p];
             const double x = Ps[p].x, y = Ps[p].y;
@@ -1197,7 +1238,7 @@ __global__ void k_text(Dev d) {
     __shared__ unsigned long long wmin[kBlock / 64];
     const int lb_ = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (lb_ >= d.neb) return;
-    const Blk bd_ = d.eblk[lb_];
+    const Blk bd_ = table_blk(d, d.eblk, lb_);
     const Params& P = d.prm[bd_.m];
     MState& M = d.ms[bd_.m];
     const int i = bd_.b + (int)threadIdx.x;
@@ -2580,7 +2621,7 @@ __global__ void __launch_bounds__(64) k_nucl_area_l1(Dev d) {
     __shared__ uint8_t ok_s[kBlock];
     const int lb = blockIdx.x;
     if (lb >= d.ngb) return;
-    const Blk bd = d.gblk[lb];
+    const Blk bd = table_blk(d, d.gblk, lb);
     const int lane = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; q++) {

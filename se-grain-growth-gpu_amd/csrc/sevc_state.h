@@ -81,7 +81,8 @@ struct Dev {
     const Params* prm; MState* ms; int B;
     const Blk* vblk; const Blk* eblk; const Blk* gblk;
     int nvb, neb, ngb;
-    const Blk* gtile; int ntile;         // 64-grain tiles for the velocity pass 1
+    const Blk* gtile; int ntile;         // tiles of tile_T consecutive grains of one member (velocity pass 1)
+    int tile_T;
     const Blk* vtile; int nvtile;        // vertex tiles of the one-pass velocity
     double* vterm;                       // 6 per vertex: stored-energy terms (x,y) per grain slot
     int4* slist;                         // free-slot search: (kind, block, rank) lists, ngb+neb+nvb entries
