@@ -545,6 +545,32 @@ def test_boundary_energy_scale_with_flips():
     assert tot["flips"] >= 50 and tot["nucl"] >= 10, tot
 
 
+def test_batch_scaled_member_beside_anisotropic_member():
+    """A batch of a GB_scaling = 1.7 member (GB_eps = 0) and an anisotropic
+    member (GB_eps = 0.3): every step then recomputes all boundary energies
+    at its start, which replaces the rescaling of the flipped edges alone;
+    each member stays equal to its own oracle after every chunk."""
+    o1, m1 = _generated(3000, 17, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3, GB_scaling=1.7,
+                        do_nucleation=1, nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02,
+                        nucleation_scaling=0.9, seed=3, extra_capacity=0.1)
+    o2 = ref2k_options(dt=2e-4, GB_eps=0.3)
+    m2 = sevc.load_files(o2)
+    st = sevc.State([o1, o2], [m1, m2])
+    orcs = [Oracle(o1, m1), Oracle(o2, m2)]
+    flips = 0
+    for chunk in (1, 1, 1, 20, 40):
+        print(f"  chunk of {chunk}", flush=True)
+        s1 = st.step(chunk)
+        assert s1.status == 0
+        for k, orc in enumerate(orcs):
+            rc, s2 = orc.step(chunk)
+            assert rc == 0
+            assert diff_fields(st.get_mesh(k), orc.get_mesh()) == {}, k
+            flips += s2.n_flips
+    assert flips >= 20
+    st.close()
+
+
 def test_two_triangle_collapse():
     """Flip threshold twice the mean edge length: flips leave pairs of
     adjacent 3-sided grains, which the two-triangle stage collapses (about
