@@ -324,6 +324,15 @@ int sevc_velocity_config(const sevc_state* s, int32_t* path, int32_t* tile);
  * rounds over global memory.  Results do not depend on it (extension; a
  * tuning and test knob). */
 int sevc_set_poll_limit(sevc_state* s, int32_t max_fast, int32_t max_lds);
+
+/* The nucleation free-slot search (the first free grain, boundary and
+ * vertex slots in index order, nucleation2.cu:33-72) reads each member's
+ * per-block free counts (256 slots a block) in rounds: by default 4096,
+ * 12288 and 8192 blocks (one round for members up to about a million
+ * grains).  blocks = 1..1024 makes each round read that many blocks of each
+ * kind, so that small states search in several rounds; 0 restores the
+ * default.  Results do not depend on it (extension; a test knob). */
+int sevc_set_search_round(sevc_state* s, int32_t blocks);
 int sevc_set_velocity_config(sevc_state* s, int32_t path, int32_t tile);
 
 /* Copy only the vertex positions of a member (2 per vertex slot). */

@@ -524,6 +524,7 @@ int sevc_batch_create(const sevc_options* opts, const sevc_mesh* meshes, int32_t
     d.debug = s->any_debug ? 1 : 0;
     d.poll_fast = kPollFast;
     d.poll_lds = kPollBig;
+    d.free_rows = 0;
     int rc = SEVC_OK;
 #define DA(p, n) do { if ((rc = dalloc(s, &(p), (n))) != SEVC_OK) { sevc_destroy(s); return rc; } } while (0)
     DA(d.pos, 2 * V); DA(d.vel, 2 * V); DA(d.vb, 3 * V); DA(d.vg, 3 * V); DA(d.ven, V);
@@ -1040,6 +1041,14 @@ int sevc_set_poll_limit(sevc_state* s, int32_t max_fast, int32_t max_lds) {
     HIPCHK(hipStreamSynchronize(s->stream));
     s->d.poll_fast = max_fast;
     s->d.poll_lds = max_lds;
+    drop_graphs(s);
+    return SEVC_OK;
+}
+
+int sevc_set_search_round(sevc_state* s, int32_t blocks) {
+    if (!s || blocks < 0 || blocks > kNuclThreads) return SEVC_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->d.free_rows = blocks;
     drop_graphs(s);
     return SEVC_OK;
 }

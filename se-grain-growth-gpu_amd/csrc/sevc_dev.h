@@ -197,6 +197,27 @@ __device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
     }
 }
 
+// Inclusive prefix sum of v over the 64 lanes of a wave (every lane active),
+// by DPP lane moves: Hillis-Steele steps within each row of 16 lanes (row
+// shifts by 1, 2, 4, 8; lanes shifted in from outside the row add 0), then
+// the total of row 0 / row 2 (its lane 15, broadcast) into rows 1 / 3, and
+// the total of rows 0-1 (lane 31, broadcast) into rows 2 and 3.  Rows that
+// a broadcast does not write add 0.
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);    // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);    // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);    // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);    // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// Sum of v over the 64 lanes of a wave (every lane active), in every lane.
+__device__ __forceinline__ int wave_sum(int v) {
+    return __builtin_amdgcn_readlane(wave_inclusive_scan(v), 63);
+}
+
 // Block lb of one of the three entity tables.  A single member's table is
 // runs of kBlock consecutive slots from 0, so its bounds are computed: the
 // block's own loads then need not wait for a table load.

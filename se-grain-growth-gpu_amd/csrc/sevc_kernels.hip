@@ -113,42 +113,40 @@ __global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
 // One workgroup per member: sums the per-block partials of the enabled
 // grain and vertex counts.
 __global__ void __launch_bounds__(1024) k_member_reduce(Dev d) {
-    __shared__ long long sa[16], sb[16];
+    __shared__ int sa[16], sb[16];
     const int m = blockIdx.x;
     if (m >= d.B) return;
     MState& M = d.ms[m];
-    if (!M.active) return;
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-    // each lane reads its share of the partials 16 at a time (independent
-    // loads), then the waves reduce with shuffles and wave 0 reduces the
-    // wave results: integer sums, so any order gives the same value
+    // each lane reads its share of the grain and vertex partials, up to 16
+    // of each at a time (independent loads, issued with the member state;
+    // a single member's ranges are the whole tables), then the waves reduce
+    // (wave_sum) and thread 0 sums the wave results: integer sums (at most
+    // the slot counts), so any order gives the same value
     constexpr int kBatch = 16;
+    const int glo = d.B == 1 ? 0 : d.gblk_first[m], ghi = d.B == 1 ? d.ngb : d.gblk_first[m + 1];
+    const int vlo = d.B == 1 ? 0 : d.vblk_first[m], vhi = d.B == 1 ? d.nvb : d.vblk_first[m + 1];
+    const int ngb = ghi - glo, nvb = vhi - vlo, nmax = max(ngb, nvb);
+    int a = 0, b2 = 0;
+    for (int k0 = tid; k0 < nmax; k0 += kBatch * nt) {
+        int tg[kBatch], tv[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const int k = k0 + q * nt;
+            tg[q] = k < ngb ? d.bc_g[glo + k].x : 0;
+            tv[q] = k < nvb ? d.bc_v[vlo + k].x : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) { a += tg[q]; b2 += tv[q]; }
+    }
+    if (!M.active) return;
     {
-        long long a = 0, b2 = 0;
-        const int glo = d.gblk_first[m], ghi = d.gblk_first[m + 1];
-        const int vlo = d.vblk_first[m], vhi = d.vblk_first[m + 1];
-        for (int b0 = glo + tid; b0 < ghi; b0 += kBatch * nt) {
-            int t[kBatch];
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < ghi ? d.bc_g[b0 + q * nt].x : 0;
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) a += t[q];
-        }
-        for (int b0 = vlo + tid; b0 < vhi; b0 += kBatch * nt) {
-            int t[kBatch];
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) t[q] = b0 + q * nt < vhi ? d.bc_v[b0 + q * nt].x : 0;
-#pragma unroll
-            for (int q = 0; q < kBatch; q++) b2 += t[q];
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o, 64);
-            b2 += __shfl_xor(b2, o, 64);
-        }
+        a = wave_sum(a);
+        b2 = wave_sum(b2);
         if (lane == 0) { sa[wave] = a; sb[wave] = b2; }
         __syncthreads();
         if (tid == 0) {
-            long long ra = 0, rb = 0;
+            int ra = 0, rb = 0;
             for (int w = 0; w < nw; w++) { ra += sa[w]; rb += sb[w]; }
             M.curr_grains = (int)ra;
             M.n_venabled = (int)rb;
@@ -365,11 +363,8 @@ __global__ void __launch_bounds__(256) k_vel_grain(Dev d, int tuning) {
             // the values the later phases need from global memory, loaded here
             en_s[tid] = en ? 1 : 0;
             se_s[tid] = first ? se0 : (in ? d.gse[g] : 0.0);
-            int inc = n;
-            for (int o = 1; o < kTile; o <<= 1) {
-                const int t = __shfl_up(inc, o, kTile);
-                if (tid >= o) inc += t;
-            }
+            static_assert(kTile == 64, "one wave scans a sub-tile's lengths");
+            const int inc = wave_inclusive_scan(n);
             const unsigned long long over = __ballot(g >= bd.e || inc > kTilePts);
             const int cnt = over ? __ffsll((long long)over) - 1 : kTile;
             const int nm = tid < cnt ? n : 0;   // entries of this sub-tile's grain
@@ -1000,11 +995,7 @@ __global__ void __launch_bounds__(4 * VT) k_vel_vtile(Dev d, int tuning) {
             loc[e] = u < nu ? ulen[u] : 0;
             sum += loc[e];
         }
-        int inc = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(inc, o, 64);
-            if (tid >= o) inc += t;
-        }
+        const int inc = wave_inclusive_scan(sum);
         int run = inc - sum;
 #pragma unroll
         for (int e = 0; e < kPer; e++) {
@@ -2284,28 +2275,32 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
     if (G1 < 0 || G2 < 0 || G3 < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); return; }
     d.vel[2 * A] = 0.0;
     d.vel[2 * A + 1] = 0.0;
+    // The searches below take the first match in index order, written as
+    // unrolled loops (constant indices keep the small arrays in registers).
+    bool found = false;
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         const int ei = xC[i][0], ee = xC[i][1];
-        if ((ei == B && ee == C) || (ee == B && ei == C)) { d.benab[vC[i]] = 0; break; }
+        if (!found && ((ei == B && ee == C) || (ee == B && ei == C))) { d.benab[vC[i]] = 0; found = true; }
     }
-    int BEb = -1, CF = -1, iBE = -1, iCF = -1;
-    for (int i = 0; i < 3; i++) {
+    int BEb = -1, CF = -1, be0 = -1, be1 = -1, cf0 = -1, cf1 = -1;
+#pragma unroll
+    for (int i = 2; i >= 0; i--) {
         const int ei = xB[i][0], ee = xB[i][1];
-        if (!(ei == A || ee == A) && !(ei == C || ee == C)) { BEb = vB[i]; iBE = i; break; }
+        if (!(ei == A || ee == A) && !(ei == C || ee == C)) { BEb = vB[i]; be0 = ei; be1 = ee; }
     }
-    for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int i = 2; i >= 0; i--) {
         const int ei = xC[i][0], ee = xC[i][1];
-        if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = vC[i]; iCF = i; break; }
+        if (!(ei == A || ee == A) && !(ei == B || ee == B)) { CF = vC[i]; cf0 = ei; cf1 = ee; }
     }
     if (BEb < 0 || CF < 0) { set_error(d, member_of_grain(d, g), SEVC_E_STATE); return; }
     // the ends of BEb and CF after the rewiring (CF's as seen after BEb's,
     // should they be the same edge).  Only the end that changes is written:
     // the other end of BEb or CF can be another removal's vertex, which that
     // removal rewires in the same launch.
-    int be0 = xB[iBE][0], be1 = xB[iBE][1];
     if (be0 == B) { be0 = A; d.bvx[2 * BEb] = A; }
     else { be1 = A; d.bvx[2 * BEb + 1] = A; }
-    int cf0 = xC[iCF][0], cf1 = xC[iCF][1];
     if (CF == BEb) { cf0 = be0; cf1 = be1; }
     if (cf0 == C) { cf0 = A; d.bvx[2 * CF] = A; }
     else { cf1 = A; d.bvx[2 * CF + 1] = A; }
@@ -2317,16 +2312,22 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
         else { e0 = xA[i][0]; e1 = xA[i][1]; }
     };
     int curA[3] = {vA[0], vA[1], vA[2]};
+    found = false;
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         int e0, e1;
         ends(curA[i], i, e0, e1);
-        if (e0 == B || e1 == B) { d.benab[curA[i]] = 0; curA[i] = BEb; d.vb[3 * A + i] = BEb; break; }
+        if (!found && (e0 == B || e1 == B)) {
+            d.benab[curA[i]] = 0; curA[i] = BEb; d.vb[3 * A + i] = BEb; found = true;
+        }
     }
+    found = false;
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         int e0, e1;
         if (curA[i] == BEb) { e0 = be0; e1 = be1; }
         else ends(curA[i], i, e0, e1);
-        if (e0 == C || e1 == C) { d.benab[curA[i]] = 0; d.vb[3 * A + i] = CF; break; }
+        if (!found && (e0 == C || e1 == C)) { d.benab[curA[i]] = 0; d.vb[3 * A + i] = CF; found = true; }
     }
     d.ven[B] = 0;
     d.ven[C] = 0;
@@ -2832,25 +2833,17 @@ __global__ void k_rate_cand(Dev d) {
 }
 
 // Exclusive prefix sum of one int per thread over the workgroup: an
-// in-wave shuffle scan, the wave totals scanned by wave 0, two barriers.
+// in-wave scan, the wave totals scanned by wave 0, two barriers.
 // Returns the prefix; *total receives the sum.  All threads must call it;
 // blockDim.x is a multiple of 64 (at most 1024) and `sh` holds >= 65 ints.
 __device__ int wg_exclusive_scan(int x, int* sh, int* total) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
-    int inc = x;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += t;
-    }
+    const int inc = wave_inclusive_scan(x);
     if (lane == 63) sh[wave] = inc;
     __syncthreads();
     if (wave == 0) {
         const int v = lane < nw ? sh[lane] : 0;
-        int vi = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(vi, o, 64);
-            if (lane >= o) vi += t;
-        }
+        const int vi = wave_inclusive_scan(v);
         if (lane < nw) sh[lane] = vi - v;
         if (lane == nw - 1) sh[64] = vi;
     }
@@ -2907,53 +2900,132 @@ struct SlotKind {
     int* out;
 };
 
-// The first R free slots (enable flag 0) of each of three kinds, in index
-// order, cooperatively in one workgroup.  Each kind's blocks are read front
-// to back in rounds of 4 consecutive blocks per lane (coalesced free
-// counts); a workgroup scan gives each lane the first rank of its blocks,
-// lanes list (kind, block, first rank) for their blocks with free slots and
-// a wanted rank in `bl` (LDS when the caller's bound on the entries fits
-// there, else global scratch), and the rounds stop once R ranks are found,
-// which after T2 removals have left holes is usually the first round.  The
-// waves then sweep the listed blocks, kSweep at a time, and write each free
-// slot at its rank.  got[t] = min(R_t, free slots of kind t).  `sh` holds
-// >= 65 ints; `bl` holds at least R_0 + R_1 + R_2 entries (a listed block
-// holds a wanted rank).
-__device__ __forceinline__ void collect_free3(const SlotKind K0, const SlotKind K1, const SlotKind K2, int* got,
-                                              int* sh, int4* bl) {
-    const SlotKind K[3] = {K0, K1, K2};    // by value: kept in registers
-    __shared__ int nl;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    constexpr int kPer = 4;                 // consecutive blocks per lane and round
-    if (tid == 0) nl = 0;
-    int tot[3];
+// Rows per kind in one round of the free-slot search (grains, boundaries,
+// vertices: a trivalent mesh has 3 boundaries and 2 vertices per grain, so
+// a round covers about the same grains of every kind).
+constexpr int kFreeRun = 12;
+constexpr int kFreeRunOf[3] = {4, 12, 8};
+constexpr int kFreeRowBase[3] = {0, 4, 16};   // first row of each kind in the LDS tables (24 rows)
+
+// Shape of a round: rows of kind t, and blocks per row.  By default a row
+// is one block per thread and a kind has kFreeRunOf rows; lim = 1..1024
+// makes a round one row of lim blocks (several rounds on small states).
+__device__ __forceinline__ int free_rows_of(int t, int lim) {
+    return lim > 0 ? 1 : (t == 0 ? kFreeRunOf[0] : (t == 1 ? kFreeRunOf[1] : kFreeRunOf[2]));
+}
+__device__ __forceinline__ int free_row_len(int lim) {
+    const int nt = blockDim.x;
+    return lim > 0 && lim < nt ? lim : nt;
+}
+
+// Round r of a kind's free-slot search covers its blocks [r C W, (r+1) C W)
+// (C = free_rows_of(kind, lim) <= kRun rows of W = free_row_len(lim)
+// blocks): thread tid < W reads the free count of block r C W + q W + tid
+// into y[q] (0 past the kind's blocks, for q >= C and for tid >= W),
+// coalesced over the workgroup.
+template <int kRun>
+__device__ __forceinline__ void read_free_rows(const SlotKind& K, int r, int C, int W, int (&y)[kFreeRun]) {
+    const int tid = threadIdx.x;
+    const int b0 = r * C * W + tid;
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
-        const int first = K[t].first, nblk = K[t].nblk, R = K[t].R;
-        const int2* bc = K[t].bc;
-        int off = 0;                        // free slots in the blocks of earlier rounds
-        for (int r0 = 0; r0 < nblk && off < R; r0 += kPer * nt) {   // the same for every lane
-            const int l0 = r0 + kPer * tid;                          // this lane's blocks [l0, l0 + kPer)
-            int v[kPer], c = 0;
-#pragma unroll
-            for (int q = 0; q < kPer; q++) v[q] = l0 + q < nblk ? bc[first + l0 + q].y : 0;
-#pragma unroll
-            for (int q = 0; q < kPer; q++) c += v[q];
-            int total;
-            int rank = off + wg_exclusive_scan(c, sh, &total);
-#pragma unroll
-            for (int q = 0; q < kPer; q++)
-                if (v[q] > 0 && rank < R) {
-                    bl[atomicAdd(&nl, 1)] = make_int4(t, first + l0 + q, rank, 0);
-                    rank += v[q];
-                }
-            off += total;
-        }
-        tot[t] = off;
+    for (int q = 0; q < kFreeRun; q++) {
+        const int b = b0 + q * W;
+        y[q] = q < kRun && q < C && tid < W && b < K.nblk ? K.bc[K.first + b].y : 0;
     }
-    __syncthreads();
-    const int n = nl;
-    const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
+}
+
+__device__ __forceinline__ void read_free_rounds(const SlotKind (&K)[3], int r, const bool (&need)[3], int lim,
+                                                 int (&y)[3][kFreeRun]) {
+    const int W = free_row_len(lim);
+    read_free_rows<kFreeRunOf[0]>(K[0], need[0] ? r : 0, need[0] ? free_rows_of(0, lim) : 0, W, y[0]);
+    read_free_rows<kFreeRunOf[1]>(K[1], need[1] ? r : 0, need[1] ? free_rows_of(1, lim) : 0, W, y[1]);
+    read_free_rows<kFreeRunOf[2]>(K[2], need[2] ? r : 0, need[2] ? free_rows_of(2, lim) : 0, W, y[2]);
+}
+
+// The first R free slots (enable flag 0) of each of three kinds, in index
+// order, cooperatively in one workgroup of at least 3 waves.  In rounds
+// (one while a kind has at most kFreeRunOf nt blocks) every lane holds one
+// free count per row of each kind (read_free_rows; y holds round 0 on
+// entry, read by the caller ahead of time).  The waves scan each row
+// (shuffles), waves 0-2 scan the (row, wave) totals of one kind each in
+// block order, and every lane then knows the first rank of each of its
+// blocks: it lists (kind, block, first rank, free count) for those that
+// hold a wanted rank in `bl` (LDS when the caller's bound on the entries
+// fits there, else global scratch).  The rounds stop once each kind has R
+// ranks.  The waves then sweep the listed blocks, kSweep at a time, and
+// write each free slot at its rank.  got[t] = min(R_t, free slots of kind
+// t).  `sh` holds >= 24 * 16 + 3 ints; `bl` holds nbl >= R_0 + R_1 + R_2
+// entries (a listed block holds a wanted rank, so no more are listed).
+__device__ __forceinline__ void collect_free3(const SlotKind K0, const SlotKind K1, const SlotKind K2,
+                                              int (&y)[3][kFreeRun], int lim, int* got, int* sh, int4* bl, int nbl) {
+    const SlotKind K[3] = {K0, K1, K2};    // by value and indexed by constants only: kept in registers
+    __shared__ int nl;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    constexpr int kTot = 24 * 16;           // sh[kTot + t]: the round's free slots of kind t
+    const int W = free_row_len(lim);        // blocks per row
+    if (tid == 0) nl = 0;                   // seen by all after the first barrier below
+    int off[3];                             // free slots in the blocks of earlier rounds
+    bool need[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) { off[t] = 0; need[t] = K[t].R > 0 && K[t].nblk > 0; }
+    for (int r = 0;; r++) {                 // the same for every lane
+        if (r > 0) read_free_rounds(K, r, need, lim, y);
+        // inclusive scan of every row within the wave, kept with the count
+        // as (scan << 16) | count (a wave's sum is at most 64 * 256); wave
+        // totals to LDS
+        int pk[3][kFreeRun];
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+#pragma unroll
+            for (int q = 0; q < kFreeRunOf[t]; q++) {
+                const int c = y[t][q];              // 0 for kinds done and rows past free_rows_of
+                const int v = wave_inclusive_scan(c);
+                pk[t][q] = (v << 16) | c;
+                if (lane == 63) sh[(kFreeRowBase[t] + q) * 16 + wave] = v;
+            }
+        __syncthreads();
+        // wave t: exclusive scan of kind t's (row, wave) totals in block order
+        if (wave < 3) {
+            const int t = wave;
+            const int base = (t == 0 ? kFreeRowBase[0] : (t == 1 ? kFreeRowBase[1] : kFreeRowBase[2])) * 16;
+            const int rows = free_rows_of(t, lim);
+            int carry = 0;
+            for (int c0 = 0; c0 < rows * nw; c0 += 64) {
+                const int k = c0 + lane;
+                const int v = k < rows * nw ? sh[base + (k / nw) * 16 + k % nw] : 0;
+                const int vi = wave_inclusive_scan(v);
+                if (k < rows * nw) sh[base + (k / nw) * 16 + k % nw] = carry + vi - v;
+                carry += __builtin_amdgcn_readlane(vi, 63);
+            }
+            if (lane == 0) sh[kTot + t] = carry;
+        }
+        __syncthreads();
+        int tot[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            tot[t] = sh[kTot + t];
+            if (!need[t]) continue;
+#pragma unroll
+            for (int q = 0; q < kFreeRunOf[t]; q++) {
+                const int c = pk[t][q] & 0xFFFF;
+                const int rank = off[t] + sh[(kFreeRowBase[t] + q) * 16 + wave] + (pk[t][q] >> 16) - c;
+                if (c > 0 && rank < K[t].R) {
+                    const int e = atomicAdd(&nl, 1);
+                    if (e < nbl) bl[e] = make_int4(t, K[t].first + (r * free_rows_of(t, lim) + q) * W + tid, rank, c);
+                }
+            }
+        }
+        __syncthreads();                    // sh is read above and written by the next round
+        bool more = false;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            if (need[t]) off[t] += tot[t];
+            need[t] = need[t] && off[t] < K[t].R && (r + 1) * free_rows_of(t, lim) * W < K[t].nblk;
+            more = more || need[t];
+        }
+        if (!more) break;
+    }
+    const int n = min(nl, nbl);
     const unsigned long long below = (1ull << lane) - 1ull;
     constexpr int kSweep = 8;
     for (int q0 = wave; q0 < n; q0 += nw * kSweep) {
@@ -3002,7 +3074,7 @@ __device__ __forceinline__ void collect_free3(const SlotKind K0, const SlotKind 
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 3; t++) got[t] = tot[t] < K[t].R ? tot[t] : K[t].R;
+    for (int t = 0; t < 3; t++) got[t] = off[t] < K[t].R ? off[t] : K[t].R;
 }
 
 // The k-th (0-based) slot of a member whose enable flag equals `want`,
@@ -3060,7 +3132,8 @@ __device__ int find_kth(const Blk* tbl, const RunScan& rs, const int2* bc, const
 // Scratch: 12 ints per grain slot of the member, at 12*goff in d.nscr:
 //  [0,ng) candidates, [ng,4ng) their site grains, [4ng,5ng) kept sites,
 //  [5ng,6ng) free grains, [6ng,9ng) free boundaries, [9ng,11ng) free
-//  vertices, [11ng,12ng) candidate sites.
+//  vertices (held in LDS instead while 6 K <= kSlotLds for K events),
+//  [11ng,12ng) candidate sites.
 constexpr int kNuclLds = 1024;       // rate-mode candidates held in LDS
 constexpr int kNuclHash = 4096;      // grain table of the clash test (> 3 kNuclLds keys: never full)
 constexpr int kHashEmpty = INT_MIN;
@@ -3075,10 +3148,15 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     __shared__ double al_s[kNuclLds];         // shortest edge at each candidate's site
     __shared__ int hk[kNuclHash], hv[kNuclHash];
     __shared__ int4 bl_s[kSlotLds];
+    __shared__ int fo_s[kSlotLds];            // the free slots found (grains, boundaries, vertices)
     const int m = blockIdx.x;
     if (m >= d.B) return;
     const Params& P = d.prm[m];
     MState& M = d.ms[m];
+    // a single member's block ranges are the whole tables
+    const int gb0 = d.B == 1 ? 0 : d.gblk_first[m], gb1 = d.B == 1 ? d.ngb : d.gblk_first[m + 1];
+    const int eb0 = d.B == 1 ? 0 : d.eblk_first[m], eb1 = d.B == 1 ? d.neb : d.eblk_first[m + 1];
+    const int vb0 = d.B == 1 ? 0 : d.vblk_first[m], vb1 = d.B == 1 ? d.nvb : d.vblk_first[m + 1];
     if (!M.active || !M.nucl_now) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int ng = P.ng;
@@ -3086,10 +3164,16 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     int* cand = base;
     int* sgr = base + ng;
     int* kept = base + 4 * (size_t)ng;
-    int* fg = base + 5 * (size_t)ng;
-    int* fe = base + 6 * (size_t)ng;
-    int* fv = base + 9 * (size_t)ng;
     int* site = base + 11 * (size_t)ng;
+    // Free slots of each kind, taken in rank order before any insertion of
+    // this step: the first round of their search reads its free counts here,
+    // so that the loads overlap the selection below.
+    SlotKind kinds[3] = {{gb0, gb1 - gb0, P.goff, P.ng, d.bc_g, d.genab, 0, base + 5 * (size_t)ng},
+                         {eb0, eb1 - eb0, P.eoff, P.ne, d.bc_e, d.benab, 0, base + 6 * (size_t)ng},
+                         {vb0, vb1 - vb0, P.voff, P.nv, d.bc_v, d.ven, 0, base + 9 * (size_t)ng}};
+    int fy[3][kFreeRun];
+    const bool all3[3] = {true, true, true};
+    read_free_rounds(kinds, 0, all3, d.free_rows, fy);
     if (tid == 0) { nkeep = 0; ncand = 0; }
     __syncthreads();
     if (P.nucl_mode == SEVC_NUCL_MC) {
@@ -3271,14 +3355,21 @@ __global__ void __launch_bounds__(kNuclThreads) k_nucl_manager(Dev d) {
     __syncthreads();
     const int K = nkeep;
     if (K == 0) return;
-    // free slots in rank order, taken before any insertion of this step
-    const SlotKind kinds[3] = {
-        {d.gblk_first[m], d.gblk_first[m + 1] - d.gblk_first[m], P.goff, P.ng, d.bc_g, d.genab, K, fg},
-        {d.eblk_first[m], d.eblk_first[m + 1] - d.eblk_first[m], P.eoff, P.ne, d.bc_e, d.benab, 3 * K, fe},
-        {d.vblk_first[m], d.vblk_first[m + 1] - d.vblk_first[m], P.voff, P.nv, d.bc_v, d.ven, 2 * K, fv}};
+    kinds[0].R = K;
+    kinds[1].R = 3 * K;
+    kinds[2].R = 2 * K;
+    const bool in_lds = 6 * K <= kSlotLds;
+    if (in_lds) {
+        kinds[0].out = fo_s;
+        kinds[1].out = fo_s + K;
+        kinds[2].out = fo_s + 4 * K;
+    }
     int got[3];
-    if (6 * K <= kSlotLds) collect_free3(kinds[0], kinds[1], kinds[2], got, sh, bl_s);
-    else collect_free3(kinds[0], kinds[1], kinds[2], got, sh, d.slist + d.gblk_first[m] + d.eblk_first[m] + d.vblk_first[m]);
+    collect_free3(kinds[0], kinds[1], kinds[2], fy, d.free_rows, got, sh, in_lds ? bl_s : d.slist + gb0 + eb0 + vb0,
+                  in_lds ? kSlotLds : (gb1 - gb0) + (eb1 - eb0) + (vb1 - vb0));
+    const int* fg = kinds[0].out;
+    const int* fe = kinds[1].out;
+    const int* fv = kinds[2].out;
     // event r needs grain slot r, boundary slots 3r..3r+2, vertex slots 2r, 2r+1
     const int n_ev = min(K, min(got[0], min(got[1] / 3, got[2] / 2)));
     if (tid == 0) ev0 = n_ev > 0 ? atomicAdd(&d.cnt[C_NEV], n_ev) : 0;
@@ -3498,6 +3589,13 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
 namespace sevc {
 __global__ void k_selftest(int op, int n, const double* x, const double* y, double* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (op == 10 || op == 11) {
+        // wave scan / sum of integer inputs (every lane of the wave active)
+        const int v = i < n ? (int)x[i] : 0;
+        const int r = op == 10 ? wave_inclusive_scan(v) : wave_sum(v);
+        if (i < n) out[i] = (double)r;
+        return;
+    }
     if (i >= n) return;
     double a = x[i], b = y[i], r = 0.0;
     switch (op) {

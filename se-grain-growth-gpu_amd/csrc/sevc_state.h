@@ -102,6 +102,7 @@ struct Dev {
     int* chk_bad;                         // violations per member found by the last check
     int poll_fast;                        // longest candidate list polled from registers (<= kPollFast)
     int poll_lds;                         // longest candidate list polled from LDS (<= kPollBig)
+    int free_rows;                        // blocks per kind of one free-slot search round (0: the default)
 };
 
 }  // namespace sevc

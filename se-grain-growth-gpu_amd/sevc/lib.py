@@ -70,6 +70,7 @@ def lib():
         "sevc_velocity_config": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "sevc_set_velocity_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "sevc_set_poll_limit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "sevc_set_search_round": (C.c_int, [C.c_void_p, C.c_int32]),
         "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -257,6 +258,11 @@ class State:
         """Candidate lists up to max_fast poll from registers, up to max_lds
         from LDS, longer ones with the general rounds (same results)."""
         _check(lib().sevc_set_poll_limit(self.h, int(max_fast), int(max_lds)), self.h)
+
+    def set_search_round(self, blocks):
+        """Blocks of each kind read per round of the nucleation free-slot
+        search (0 = default; same results)."""
+        _check(lib().sevc_set_search_round(self.h, int(blocks)), self.h)
 
     def set_velocity_config(self, path, tile):
         _check(lib().sevc_set_velocity_config(self.h, int(path), int(tile)), self.h)

@@ -18,9 +18,12 @@ STAT_KEYS = ("n", "n_grains", "n_vertices", "n_flips", "n_removals", "n_two_grai
              "n_vertex_steps", "steps_done", "done")
 
 
-def _run_pair(opts, mesh, chunks, graph=False):
-    """Step GPU and oracle in the given chunk sizes; compare after each."""
+def _run_pair(opts, mesh, chunks, graph=False, setup=None):
+    """Step GPU and oracle in the given chunk sizes; compare after each.
+    `setup`, if given, is called with the GPU state before the first step."""
     st, orc = sevc.State(opts, mesh), Oracle(opts, mesh)
+    if setup is not None:
+        setup(st)
     tot = {"flips": 0, "removals": 0, "nucl": 0, "two": 0}
     for n in chunks:
         print(f"  chunk of {n} steps", flush=True)
@@ -87,6 +90,22 @@ def test_selftest_math_bits():
         L.orc_philox(c, k, o)
         host.append(float(o[0]))
     assert np.array_equal(dev, np.array(host))
+
+
+def test_wave_scan_lane_layout():
+    """The DPP wave scan used by the free-slot search and the workgroup
+    scans: inclusive prefix sums and totals over each wave of 64 lanes equal
+    numpy's, for counts up to 256 per lane, zeros, and a partial last
+    workgroup (lanes past the input hold 0)."""
+    rng = np.random.default_rng(3)
+    for x in (rng.integers(0, 257, 64 * 300), np.zeros(640), rng.integers(0, 2, 64 * 40),
+              rng.integers(0, 1000, 64 * 7 + 19)):
+        x = x.astype(np.float64)
+        pad = np.concatenate([x, np.zeros(-x.size % 64)]).reshape(-1, 64)
+        scan = sevc.selftest_math(10, x, np.zeros_like(x))
+        assert np.array_equal(scan, np.cumsum(pad, axis=1).ravel()[:x.size])
+        tot = sevc.selftest_math(11, x, np.zeros_like(x))
+        assert np.array_equal(tot, np.repeat(pad.sum(axis=1), 64)[:x.size])
 
 
 def test_step0_velocities_bitwise():
@@ -531,6 +550,32 @@ def test_nucleation_bursts(rate):
                          extra_capacity=0.15)
     tot = _run_pair(o, mesh, [1, 1, 2])
     assert tot["nucl"] >= (1000 if rate < 0.1 else 3000), tot
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_free_slot_search_in_rounds(blocks):
+    """The nucleation free-slot search read in rounds of `blocks` table
+    blocks of each kind (instead of one round of up to 4096, 12288 and 8192
+    blocks), so that it walks several rounds to find its ranks: rate mode
+    with tens of events per step, more than the in-LDS slot list holds, and
+    the Metropolis mode, all bit-equal to the oracle."""
+    def rounds(st):
+        st.set_search_round(blocks)
+    o, mesh = _generated(3000, 5, dt=1e-4, adaptive_dt=1, dt_safety=0.5, flip_threshold=2e-3,
+                         do_nucleation=1, nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.02,
+                         nucleation_scaling=0.9, seed=11, extra_capacity=0.1)
+    tot = _run_pair(o, mesh, [1, 1, 20, 40], setup=rounds)
+    assert tot["nucl"] >= 10 and tot["removals"] >= 5, tot
+    o, mesh = _generated(20000, 31, dt=1e-6, adaptive_dt=1, dt_safety=0.5, flip_threshold=1e-4, do_nucleation=1,
+                         nucleation_mode=sevc.NUCL_RATE, nucleation_rate=0.05, nucleation_scaling=0.9, seed=5,
+                         extra_capacity=0.15)
+    tot = _run_pair(o, mesh, [1, 1], setup=rounds)
+    assert tot["nucl"] >= 400, tot
+    o = ref2k_options(domain_bound=40.0, dt=1e-3, dtau=1e-3, do_nucleation=1, leave_percent=0.05,
+                      SE_eps=1.0, MC_k=1.0)
+    o.SE_min, o.SE_max = 3.0, 6.0
+    tot = _run_pair(o, sevc.load_files(o), [600, 600], setup=rounds)
+    assert tot["nucl"] >= 10, tot
 
 
 def test_boundary_energy_scale_with_flips():
