@@ -202,6 +202,24 @@ SEVC_HD double sevc_cos(double x) {
     return sign ? -y : y;
 }
 
+/* sin(x) and cos(x) of one argument with one octant reduction: each value
+ * is the one sevc_sin / sevc_cos returns (the same reduction of |x|, the
+ * same polynomials and the same octant and sign rules). */
+SEVC_HD void sevc_sincos(double x, double* s, double* c) {
+    int ssign = 0, csign = 0, j;
+    double ax = x;
+    if (x < 0.0) { ssign = 1; ax = -x; }
+    double z = sevc__reduce_pio4(ax, &j);
+    if (j > 3) { j -= 4; ssign = !ssign; csign = !csign; }
+    if (j > 1) csign = !csign;
+    double zz = z * z;
+    double ps = sevc__sin_poly(z, zz), pc = sevc__cos_poly(zz);
+    double ys = (j == 1 || j == 2) ? pc : ps;
+    double yc = (j == 1 || j == 2) ? ps : pc;
+    *s = (x == 0.0) ? x : (ssign ? -ys : ys);
+    *c = csign ? -yc : yc;
+}
+
 /* exp(x), Cephes exp.c: x = n ln2 + r with a two-part ln2, then the Pade
  * form 1 + 2 r P(r^2) / (Q(r^2) - r P(r^2)), scaled by 2^n. */
 SEVC_HD double sevc_exp(double x) {

@@ -218,6 +218,30 @@ __device__ __forceinline__ int wave_sum(int v) {
     return __builtin_amdgcn_readlane(wave_inclusive_scan(v), 63);
 }
 
+// Minimum of v over the 64 lanes of a wave (every lane active), in every
+// lane: the row-shift and broadcast pattern of wave_inclusive_scan with min
+// for sum, on both 32-bit halves; lanes with no source keep the identity
+// (all ones).
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#define SEVC_MIN_STEP(ctrl, rows)                                                                        \
+    {                                                                                                    \
+        const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(-1, (int)(unsigned)v, ctrl, rows, 0xf, false);  \
+        const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(-1, (int)(unsigned)(v >> 32), ctrl, rows, 0xf, false); \
+        const unsigned long long s = ((unsigned long long)hi << 32) | lo;                                \
+        v = s < v ? s : v;                                                                               \
+    }
+    SEVC_MIN_STEP(0x111, 0xf)   // row_shr:1
+    SEVC_MIN_STEP(0x112, 0xf)   // row_shr:2
+    SEVC_MIN_STEP(0x114, 0xf)   // row_shr:4
+    SEVC_MIN_STEP(0x118, 0xf)   // row_shr:8
+    SEVC_MIN_STEP(0x142, 0xa)   // row_bcast:15 into rows 1, 3
+    SEVC_MIN_STEP(0x143, 0xc)   // row_bcast:31 into rows 2, 3
+#undef SEVC_MIN_STEP
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Block lb of one of the three entity tables.  A single member's table is
 // runs of kBlock consecutive slots from 0, so its bounds are computed: the
 // block's own loads then need not wait for a table load.

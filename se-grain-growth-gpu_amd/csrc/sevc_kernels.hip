@@ -1259,8 +1259,7 @@ __global__ void k_text(Dev d) {
         if (norm == 0) { tx = 0; ty = 0; }
         else {
             double ang = sevc_atan2(dy, dx);
-            tx = sevc_cos(ang);
-            ty = sevc_sin(ang);
+            sevc_sincos(ang, &ty, &tx);
         }
         double px = vb.x - va.x;
         double py = vb.y - va.y;
@@ -1269,10 +1268,7 @@ __global__ void k_text(Dev d) {
         d.bt[i] = te;
         if (P.adaptive && te > 0 && !(norm < P.flip_thr)) v = (unsigned long long)__double_as_longlong(te);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off);
-        v = o < v ? o : v;
-    }
+    v = wave_min_u64(v);
     if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -3596,6 +3592,13 @@ __global__ void k_selftest(int op, int n, const double* x, const double* y, doub
         if (i < n) out[i] = (double)r;
         return;
     }
+    if (op == 14) {
+        // wave minimum of the inputs' bit patterns (lanes past n: all ones)
+        const unsigned long long v = i < n ? (unsigned long long)__double_as_longlong(x[i]) : ~0ull;
+        const unsigned long long r = wave_min_u64(v);
+        if (i < n) out[i] = __longlong_as_double((long long)r);
+        return;
+    }
     if (i >= n) return;
     double a = x[i], b = y[i], r = 0.0;
     switch (op) {
@@ -3608,6 +3611,8 @@ __global__ void k_selftest(int op, int n, const double* x, const double* y, doub
         case 6: r = mag(a, b); break;
         case 7: r = wrapd(a, b, 1.0); break;
         case 9: r = sqrt_1_2(a); break;
+        case 12: { double c; sevc_sincos(a, &r, &c); break; }
+        case 13: { double sn; sevc_sincos(a, &sn, &r); break; }
         case 8: {
             uint32_t c[4] = {(uint32_t)a, (uint32_t)b, 0u, 0u}, k[2] = {1u, 2u}, o[4];
             sevc_philox4x32_10(c, k, o);

@@ -78,6 +78,14 @@ def test_selftest_math_bits():
     dev = sevc.selftest_math(6, ax, ay)
     host = np.array([L.orc_mag(a, b) for a, b in zip(ax, ay)])
     assert np.array_equal(dev.view(np.uint64), host.view(np.uint64))
+    # sin and cos from one reduction equal sevc_sin and sevc_cos, over the
+    # atan2 range (the edge tangents) and wider, with signed zeros and the
+    # octant edges
+    a = np.concatenate([rng.uniform(-np.pi, np.pi, n), rng.uniform(-50, 50, n),
+                        np.arange(-16, 17) * (np.pi / 4), [0.0, -0.0, np.pi, -np.pi, 1e-300, -1e-300]])
+    z = np.zeros_like(a)
+    assert np.array_equal(sevc.selftest_math(12, a, z).view(np.uint64), sevc.selftest_math(1, a, z).view(np.uint64))
+    assert np.array_equal(sevc.selftest_math(13, a, z).view(np.uint64), sevc.selftest_math(2, a, z).view(np.uint64))
     # Philox word 0 on the device equals the host restatement
     ctr = rng.integers(0, 2 ** 32, n).astype(np.float64)
     dev = sevc.selftest_math(8, ctr, ctr[::-1].copy())
@@ -96,7 +104,8 @@ def test_wave_scan_lane_layout():
     """The DPP wave scan used by the free-slot search and the workgroup
     scans: inclusive prefix sums and totals over each wave of 64 lanes equal
     numpy's, for counts up to 256 per lane, zeros, and a partial last
-    workgroup (lanes past the input hold 0)."""
+    workgroup (lanes past the input hold 0); and the DPP wave minimum of
+    64-bit patterns that the extinction-time minimum uses."""
     rng = np.random.default_rng(3)
     for x in (rng.integers(0, 257, 64 * 300), np.zeros(640), rng.integers(0, 2, 64 * 40),
               rng.integers(0, 1000, 64 * 7 + 19)):
@@ -106,6 +115,15 @@ def test_wave_scan_lane_layout():
         assert np.array_equal(scan, np.cumsum(pad, axis=1).ravel()[:x.size])
         tot = sevc.selftest_math(11, x, np.zeros_like(x))
         assert np.array_equal(tot, np.repeat(pad.sum(axis=1), 64)[:x.size])
+    # wave minimum of 64-bit patterns (non-negative doubles and +inf, the
+    # extinction-time minimum), with a partial last wave
+    for x in (rng.uniform(0, 5, 64 * 50), np.full(128, np.inf), np.concatenate([rng.uniform(0, 1, 64 * 3 + 5)])):
+        x = x.copy()
+        x[rng.random(x.size) < 0.3] = np.inf
+        bits = x.view(np.uint64)
+        pad = np.concatenate([bits, np.full(-x.size % 64, np.iinfo(np.uint64).max, dtype=np.uint64)]).reshape(-1, 64)
+        got = sevc.selftest_math(14, x, np.zeros_like(x)).view(np.uint64)
+        assert np.array_equal(got, np.repeat(pad.min(axis=1), 64)[:x.size])
 
 
 def test_step0_velocities_bitwise():
