@@ -44,6 +44,8 @@ __global__ void k_scan3(Dev d);
 __global__ void k_t2_par(Dev d);
 __global__ void k_t2_serial(Dev d);
 __global__ void k_st2(Dev d);
+__global__ void k_st2_fix_clear(Dev d, int nclear);
+__global__ void k_translate_clear(Dev d);
 __global__ void k_flip_par(Dev d);
 __global__ void k_flip_serial(Dev d);
 __global__ void k_nucl_area_l1(Dev d);
@@ -231,6 +233,7 @@ int count_grid(const Dev& d) { return (d.ngb + d.neb + d.nvb + kCountWaves - 1) 
 constexpr int kFixGrid = 2048;
 // one wave per nucleation event
 constexpr int kNuclInsGrid = 256;
+constexpr int kClearGrid = 64;           // workgroups of k_translate_clear that clear (4 waves each)
 constexpr int kTileMin = 16;             // smallest tile the autotuner tries
 
 // Tile table of velocity pass 1: T consecutive grains of one member per
@@ -397,16 +400,26 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "t2_par", k_t2_par, kListGrid, 256, d);
         launch(s, "t2_serial", k_t2_serial, 1, 64, d);
         launch(s, "fix", k_fix, kFixGrid, 64, d, 1);
-        launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 1, 1);
-        if (s->any_debug) launch_check(s, CHK_REMOVAL);
-        launch(s, "st2", k_st2, kListGrid, 256, d);
+        if (s->any_debug) {
+            // the checks read the cleared fix-up state
+            launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 1, 1);
+            launch_check(s, CHK_REMOVAL);
+            launch(s, "st2", k_st2, kListGrid, 256, d);
+        } else {
+            launch(s, "st2_fix_clear", k_st2_fix_clear, kFixGrid + 4 * kListGrid, 64, d, kFixGrid);
+        }
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
         launch(s, "fix", k_fix, kFixGrid, 64, d, 2);
-        launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 2, 0);
-        if (s->any_debug) launch_check(s, CHK_FLIP);
+        if (s->any_debug) {
+            launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 2, 0);
+            launch_check(s, CHK_FLIP);
+        }
     }
-    launch(s, "translate", k_translate, d.nvb, kBlock, d);
+    if (s->any_events && !s->any_debug)
+        launch(s, "translate", k_translate_clear, d.nvb + kClearGrid, kBlock, d);
+    else
+        launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, count_grid(d), 64 * kCountWaves, d);
     launch(s, "member_reduce", k_member_reduce, s->B, 1024, d);
 }

@@ -1200,8 +1200,11 @@ __global__ void k_grain_dadt(Dev d) {
 }
 
 // update_vertices_positions, calculus.h:391-404
-__global__ void k_translate(Dev d) {
-    TABLE_BLOCK(d.vblk, d.nvb)
+__device__ __forceinline__ void translate_block(const Dev& d, int lb) {
+    const Blk bd_ = table_blk(d, d.vblk, lb);
+    const Params& P = d.prm[bd_.m];
+    const MState& M = d.ms[bd_.m];
+    const int i = bd_.b + (int)threadIdx.x;
     // the vertex's data is loaded together with the member's state
     const bool in = i < bd_.e;
     const double2 p = in ? reinterpret_cast<const double2*>(d.pos)[i] : make_double2(0.0, 0.0);
@@ -1213,6 +1216,34 @@ __global__ void k_translate(Dev d) {
     x += v.x * dt;
     y += v.y * dt;
     reinterpret_cast<double2*>(d.pos)[i] = make_double2(dom_mod(x, P.D), dom_mod(y, P.D));
+}
+
+__global__ void k_translate(Dev d) {
+    const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb < d.nvb) translate_block(d, lb);
+}
+
+// k_translate and the clearing after the flip stage's fix-up (k_fix_clear,
+// stage 2, no areas) in one launch: table blocks [0, nvb) translate, the
+// other workgroups clear, one listed grain per wave.  Disjoint data:
+// positions, and fix flags and vertex edit records.
+__global__ void k_translate_clear(Dev d) {
+    const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (lb < d.nvb) {
+        translate_block(d, lb);
+        return;
+    }
+    const int n = d.cnt[C_FIX0 + 2];
+    const int lane = (int)threadIdx.x & 63;
+    const int nw = ((int)gridDim.x - d.nvb) * (int)(blockDim.x >> 6);
+    for (int item = (lb - d.nvb) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6); item < n; item += nw) {
+        const int g = d.l_fix[2][item];
+        if (lane == 0) d.gfix[g] = 0;
+        if (!d.genab[g]) continue;
+        const int* L = glist(d, g);
+        const int m = d.glen[g];
+        for (int i = lane; i < m; i += 64) reset_mods(d, L[i]);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -1875,12 +1906,14 @@ __global__ void __launch_bounds__(64) k_fix(Dev d, int stage) {
 // and load the list's points in parallel, each forming the wrapped delta to
 // its predecessor; lane 0 then accumulates the shoelace sum over the deltas
 // in list order, the same arithmetic as grain_area.
-__global__ void __launch_bounds__(64) k_fix_clear(Dev d, int stage, int with_area) {
+// Items of k_fix_clear's work for one 64-lane workgroup: workgroup `blk` of
+// `nblk` takes items blk, blk + nblk, ...
+__device__ __forceinline__ void fix_clear_items(const Dev& d, int stage, int with_area, int blk, int nblk) {
     __shared__ double sdx[kMaxStride], sdy[kMaxStride];
     const int n = d.cnt[C_FIX0 + stage];
     const int nd = with_area ? min(d.cnt[C_DIRTY], d.G) : 0;
     const int lane = threadIdx.x;
-    for (int item = blockIdx.x; item < n + nd; item += gridDim.x) {
+    for (int item = blk; item < n + nd; item += nblk) {
         int g;
         if (item < n) {
             g = d.l_fix[stage][item];
@@ -1915,6 +1948,10 @@ __global__ void __launch_bounds__(64) k_fix_clear(Dev d, int stage, int with_are
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(64) k_fix_clear(Dev d, int stage, int with_area) {
+    fix_clear_items(d, stage, with_area, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------
@@ -2394,9 +2431,9 @@ __global__ void k_t2_serial(Dev d) {
 }
 
 // unlabel_boundaries_st2, topological.h:588-606
-__global__ void k_st2(Dev d) {
+__device__ __forceinline__ void st2_items(const Dev& d, int k0, int stride) {
     const int n = d.cnt[C_CAND];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (int k = k0; k < n; k += stride) {
         int b = d.l_cand[k];
         if (!(d.benab[b] && d.bcand[b])) continue;
         if (!d.ms[member_of_edge(d, b)].active) continue;
@@ -2404,6 +2441,25 @@ __global__ void k_st2(Dev d) {
         if (grains_intersect(d, d.bvx[2 * b], d.bvx[2 * b + 1], gic) < 2) { d.bcand[b] = 0; continue; }
         if (d.glen[gic[0]] == 3 || d.glen[gic[1]] == 3) d.bcand[b] = 0;
     }
+}
+
+__global__ void k_st2(Dev d) {
+    st2_items(d, (int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x));
+}
+
+// k_st2 and the clearing after the T2 stage's fix-up (k_fix_clear, stage 1
+// with areas) in one launch of 64-lane workgroups: the first `nclear` take
+// the clearing, the others the candidate list.  They touch disjoint data:
+// st2 reads the edges' flags and ends, the ends' grains and the grains'
+// lengths and writes edge labels; the clearing reads lists and positions
+// and writes fix flags, vertex edit records and areas.
+__global__ void __launch_bounds__(64) k_st2_fix_clear(Dev d, int nclear) {
+    const int b = (int)blockIdx.x;
+    if (b < nclear) {
+        fix_clear_items(d, 1, 1, b, nclear);
+        return;
+    }
+    st2_items(d, (b - nclear) * 64 + (int)threadIdx.x, ((int)gridDim.x - nclear) * 64);
 }
 
 // ---------------------------------------------------------------------
