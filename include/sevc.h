@@ -411,6 +411,10 @@ int sevc_kernel_times(sevc_state* s, int32_t max_entries, char (*names)[48], dou
  * one-pass path, "vel_grain" and "velocity" for the two passes),
  * "translate" and "text". */
 int sevc_enable_kernel_timing(sevc_state* s, int32_t on);
+/* With timing on, sevc_step times the kernels of one step in `every`
+ * (default 1; counted from the last sevc_enable_kernel_timing or this
+ * call), so that the events disturb the timed run less. */
+int sevc_set_timing_sample(sevc_state* s, int32_t every);
 
 /* Device self-test of the shared arithmetic: evaluates op on n inputs on
  * the GPU (0 atan2(y,x), 1 sin(x), 2 cos(x), 3 exp(x), 4 x/y, 5 sqrt(x),

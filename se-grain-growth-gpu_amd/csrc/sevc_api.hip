@@ -106,6 +106,9 @@ struct sevc_state {
     // kernel timing
     bool timing = false;
     bool timing_hot_only = false;
+    int timing_every = 1;                // sevc_step times one step in timing_every
+    long long timing_steps = 0;          // steps stepped since timing was enabled
+    bool timing_skip = false;            // the step being launched is not sampled
     std::vector<KTimer> pending;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -196,7 +199,7 @@ hipEvent_t take_event(sevc_state* s) {
 template <typename K, typename... Args>
 void launch(sevc_state* s, const char* name, K kernel, int grid, int block, Args... args) {
     if (grid <= 0) return;
-    bool timed = s->timing;
+    bool timed = s->timing && !s->timing_skip;
     if (timed && s->timing_hot_only)
         timed = std::strcmp(name, "vel_grain") == 0 || std::strcmp(name, "velocity") == 0 ||
                 std::strcmp(name, "vel_fused") == 0 || std::strcmp(name, "vel_glane") == 0 ||
@@ -233,7 +236,7 @@ int count_grid(const Dev& d) { return (d.ngb + d.neb + d.nvb + kCountWaves - 1) 
 constexpr int kFixGrid = 2048;
 // one wave per nucleation event
 constexpr int kNuclInsGrid = 256;
-constexpr int kClearGrid = 64;           // workgroups of k_translate_clear that clear (4 waves each)
+constexpr int kClearGrid = kFixGrid / 4; // workgroups of k_translate_clear that clear (4 waves each)
 constexpr int kTileMin = 16;             // smallest tile the autotuner tries
 
 // Tile table of velocity pass 1: T consecutive grains of one member per
@@ -771,7 +774,15 @@ int sevc_enable_kernel_timing(sevc_state* s, int32_t on) {
     if (!s) return SEVC_E_INVALID_ARG;
     s->timing = on != 0;
     s->timing_hot_only = on == 2;
+    s->timing_steps = 0;
     s->ktimes.clear();
+    return SEVC_OK;
+}
+
+int sevc_set_timing_sample(sevc_state* s, int32_t every) {
+    if (!s || every < 1) return SEVC_E_INVALID_ARG;
+    s->timing_every = every;
+    s->timing_steps = 0;
     return SEVC_OK;
 }
 
@@ -845,7 +856,9 @@ int sevc_step(sevc_state* s, int32_t n_steps, sevc_step_stats* st) {
     for (int k = 0; k < n_steps; ++k) {
         bool diag = s->diag_next;
         s->diag_next = false;
+        s->timing_skip = s->timing && (s->timing_steps++ % s->timing_every) != 0;
         launch_step(s, diag);
+        s->timing_skip = false;
         if (s->timing && s->pending.size() > 4000) {
             (void)hipStreamSynchronize(s->stream);
             collect_timers(s);

@@ -189,12 +189,43 @@ __device__ __forceinline__ void set_mod(const Dev& d, int v, int slot, int fix, 
     d.mods[v].vrt[slot] = vrt;
 }
 
+// A slot of a list whose length is *counter, for each active lane with
+// `want` (-1 for the others), with one atomic per wave: lanes that append
+// at the same point share it (the slots are consecutive in lane order).  The
+// lists' order is unspecified, as with an atomic per lane; one atomic per
+// wave keeps a step with thousands of events from queueing them on one
+// address.
+__device__ __forceinline__ int wave_slot(int* counter, bool want) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0) return -1;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, leader);
+    return want ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Adds, for each active lane with `want`, one to the member counter
+// `field` of member `mem`: one atomic per distinct member among the wave's
+// lanes.
+__device__ __forceinline__ void wave_count(const Dev& d, bool want, int mem, long long MState::*field) {
+    unsigned long long m = __ballot(want);
+    const int lane = (int)(threadIdx.x & 63);
+    while (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        const int k = __builtin_amdgcn_readlane(mem, leader);
+        const unsigned long long same = __ballot(want && mem == k);
+        if (lane == leader) atomicAdd(reinterpret_cast<unsigned long long*>(&(d.ms[k].*field)), (unsigned long long)__popcll(same));
+        m &= ~same;
+    }
+}
+
 // Flag grain g for the fix stage `stage` (grain.fix = true) and list it once.
 __device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
-    if (atomicExch(&d.gfix[g], 1) == 0) {
-        int i = atomicAdd(&d.cnt[C_FIX0 + stage], 1);
-        d.l_fix[stage][i] = g;
-    }
+    const bool first = atomicExch(&d.gfix[g], 1) == 0;
+    const int i = wave_slot(&d.cnt[C_FIX0 + stage], first);
+    if (first) d.l_fix[stage][i] = g;
 }
 
 // Inclusive prefix sum of v over the 64 lanes of a wave (every lane active),

@@ -1315,29 +1315,37 @@ __global__ void k_text(Dev d) {
 // label_boundaries, topological.h:343-353 (every slot, quirk A-6), and the
 // compaction of enabled candidates.
 __global__ void k_label(Dev d) {
+    __shared__ int bcnt, bbase;
     TABLE_BLOCK(d.eblk, d.neb)
     // the edge's data is loaded together with the member's state
     const bool in = i < bd_.e;
     const double te = in ? d.bt[i] : 0.0, len = in ? d.blen[i] : 0.0;
     const uint8_t en = in ? d.benab[i] : 0;
-    if (!M.active || !P.events || !in) return;
+    if (!M.active || !P.events) return;             // the same for the whole workgroup
+    if (threadIdx.x == 0) bcnt = 0;
     const double dt = member_dt(P, M);
-    bool c = is_candidate(te, len, dt, P.flip_thr);
-    d.bcand[i] = c ? 1 : 0;
-    d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));     // for k_scan3: one byte per edge (bit 2: k_poll)
-    // the wave's candidates take consecutive list positions with one atomic
+    const bool c = in && is_candidate(te, len, dt, P.flip_thr);
+    if (in) {
+        d.bcand[i] = c ? 1 : 0;
+        d.blab[i] = (uint8_t)((c && en ? 1 : 0) | (en ? 2 : 0));  // for k_scan3: one byte per edge (bit 2: k_poll)
+    }
+    // the workgroup's candidates take consecutive list positions with one
+    // atomic (a step with many candidates would otherwise queue one per wave
+    // on the counter)
     const bool cand = c && en;
     const unsigned long long bal = __ballot(cand);
-    if (bal) {
-        const int lane = threadIdx.x & 63, leader = __ffsll((long long)bal) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&d.cnt[C_CAND], (int)__popcll(bal));
-        base = __shfl(base, leader, 64);
-        if (cand) {
-            const int k = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
-            d.l_cand[k] = i;
-            d.conf[i] = k;          // list position, for k_poll_gather
-        }
+    const int lane = (int)(threadIdx.x & 63), leader = bal ? __ffsll((long long)bal) - 1 : 0;
+    __syncthreads();
+    int woff = 0;
+    if (bal && lane == leader) woff = atomicAdd(&bcnt, (int)__popcll(bal));
+    __syncthreads();
+    if (threadIdx.x == 0) bbase = bcnt > 0 ? atomicAdd(&d.cnt[C_CAND], bcnt) : 0;
+    __syncthreads();
+    woff = __builtin_amdgcn_readlane(woff, leader);
+    if (cand) {
+        const int k = bbase + woff + (int)__popcll(bal & ((1ull << lane) - 1ull));
+        d.l_cand[k] = i;
+        d.conf[i] = k;          // list position, for k_poll_gather
     }
 }
 
@@ -2173,7 +2181,7 @@ __global__ void k_scan3(Dev d) {
                 if (d.glen[gic[0]] == 3 && d.glen[gic[1]] == 3) {
                     const int lo = gic[0] < gic[1] ? gic[0] : gic[1];
                     if (lo == g) {
-                        const int s = atomicAdd(&d.cnt[C_TWO], 1);
+                        const int s = atomicAdd(&d.cnt[C_TWO], 1);     // rare: two adjacent 3-sided grains
                         d.l_two[s] = bb[e];
                     }
                 }
@@ -2181,10 +2189,8 @@ __global__ void k_scan3(Dev d) {
         }
     }
     d.gcand[g] = cand;
-    if (cand) {
-        const int s = atomicAdd(&d.cnt[C_T2], 1);
-        d.l_t2[s] = g;
-    }
+    const int s = wave_slot(&d.cnt[C_T2], cand != 0);
+    if (cand) d.l_t2[s] = g;
 }
 
 // ---------------------------------------------------------------------
@@ -2246,10 +2252,8 @@ __global__ void k_label3(Dev d) {
         if (hit || ans == 3) cand = 1;
     }
     d.gcand[g] = cand;
-    if (cand) {
-        int s = atomicAdd(&d.cnt[C_T2], 1);
-        d.l_t2[s] = g;
-    }
+    const int s = wave_slot(&d.cnt[C_T2], cand != 0);
+    if (cand) d.l_t2[s] = g;
 }
 
 // remove_grain, topological.h:687-822
@@ -2404,14 +2408,14 @@ __global__ void k_t2_par(Dev d) {
         bool clash = false;
 #pragma unroll
         for (int e = 0; e < 9; e++) clash = clash || (he[e] && hc[e]);
+        const int s = wave_slot(&d.cnt[C_T2C], clash);
         if (clash) {
-            int s = atomicAdd(&d.cnt[C_T2C], 1);
             d.l_t2c[s] = g;
         } else {
             int m = member_of_grain(d, g);
             remove_grain(d, g, d.prm[m].D);
-            atomicAdd((unsigned long long*)&d.ms[m].removals, 1ull);
         }
+        wave_count(d, !clash, member_of_grain(d, g), &MState::removals);
     }
 }
 
@@ -2636,14 +2640,14 @@ __global__ void k_flip_par(Dev d) {
         bool clash = false;
 #pragma unroll
         for (int q = 0; q < 6; q++) clash = clash || (xe[q] && xc[q]);
+        const int s = wave_slot(&d.cnt[C_FLIPC], clash);
         if (clash) {
-            int s = atomicAdd(&d.cnt[C_FLIPC], 1);
             d.l_flipc[s] = c;
         } else {
             int m = member_of_edge(d, c);
             flip(d, c, d.prm[m], fi);
-            atomicAdd((unsigned long long*)&d.ms[m].flips, 1ull);
         }
+        wave_count(d, !clash, member_of_edge(d, c), &MState::flips);
     }
 }
 
@@ -2876,7 +2880,7 @@ __global__ void k_rate_cand(Dev d) {
     for (int q = 0; q < 3; q++) al[q] = edge_len_from(d, vb[q], A, P.D);
     double minAL = al[0];
     for (int q = 0; q < 3; q++) if (minAL > al[q]) minAL = al[q];
-    int s = atomicAdd(&d.cnt[C_NUCL], 1);
+    const int s = wave_slot(&d.cnt[C_NUCL], true);
     d.l_nucl[s] = i;
     d.nrec[4 * s] = (minAL > 0) ? A : -1;
 #pragma unroll

@@ -71,6 +71,7 @@ def lib():
         "sevc_set_velocity_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "sevc_set_poll_limit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "sevc_set_search_round": (C.c_int, [C.c_void_p, C.c_int32]),
+        "sevc_set_timing_sample": (C.c_int, [C.c_void_p, C.c_int32]),
         "sevc_area_histogram": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(P(C.c_double)), C.c_int32]),
         "sevc_window_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -333,6 +334,10 @@ class State:
         """0/False off, 1/True every kernel, 2 only the velocity passes,
         translation and t_ext (HIP events on the state's stream)."""
         _check(lib().sevc_enable_kernel_timing(self.h, int(on)), self.h)
+
+    def set_timing_sample(self, every):
+        """Time the kernels of one step in `every` (sevc_step)."""
+        _check(lib().sevc_set_timing_sample(self.h, int(every)), self.h)
 
     def kernel_times(self):
         n = 64

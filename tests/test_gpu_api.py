@@ -52,6 +52,28 @@ def test_calls_reject_bad_arguments_and_state_keeps_working():
     st.close()
 
 
+def test_sampled_kernel_timing_counts_launches():
+    """Kernel timing of the hot kernels on one step in `every`: the launch
+    counts follow the sample; bad arguments are rejected."""
+    o = ref2k_options(dt=2e-4)
+    st = sevc.State(o, sevc.load_files(o))
+    st.set_velocity_config(0, 64)
+    assert _code(lambda: st.set_timing_sample(0)) == sevc.E_INVALID_ARG
+    assert _code(lambda: st.set_search_round(-1)) == sevc.E_INVALID_ARG
+    st.enable_timing(2)
+    st.set_timing_sample(4)
+    st.step(10)                         # steps 0, 4 and 8 are timed
+    kt = st.kernel_times()
+    assert kt["vel_grain"][1] == 3 and kt["velocity"][1] == 3 and kt["text"][1] == 3
+    assert kt["vel_grain"][0] > 0 and "flip_par" not in kt
+    st.enable_timing(2)                 # counts restart; the sample is kept
+    st.step(5)
+    assert st.kernel_times()["text"][1] == 2
+    st.set_timing_sample(1)
+    st.enable_timing(0)
+    st.close()
+
+
 def test_member_mesh_restore_replays_the_same_steps():
     """sevc_set_member_mesh restores a checkpoint taken with sevc_get_mesh:
     stepping again from it gives the same state as the first time (no
