@@ -45,7 +45,7 @@ __global__ void k_t2_par(Dev d);
 __global__ void k_t2_serial(Dev d);
 __global__ void k_st2(Dev d);
 __global__ void k_st2_fix_clear(Dev d, int nclear);
-__global__ void k_translate_clear(Dev d);
+__global__ void k_translate_clear(Dev d, int nclear);
 __global__ void k_flip_par(Dev d);
 __global__ void k_flip_serial(Dev d);
 __global__ void k_nucl_area_l1(Dev d);
@@ -420,7 +420,7 @@ void launch_step(sevc_state* s, bool diag) {
         }
     }
     if (s->any_events && !s->any_debug)
-        launch(s, "translate", k_translate_clear, d.nvb + kClearGrid, kBlock, d);
+        launch(s, "translate", k_translate_clear, kClearGrid + d.nvb, kBlock, d, kClearGrid);
     else
         launch(s, "translate", k_translate, d.nvb, kBlock, d);
     launch(s, "count_all", k_count_all, count_grid(d), 64 * kCountWaves, d);

@@ -1224,19 +1224,19 @@ __global__ void k_translate(Dev d) {
 }
 
 // k_translate and the clearing after the flip stage's fix-up (k_fix_clear,
-// stage 2, no areas) in one launch: table blocks [0, nvb) translate, the
-// other workgroups clear, one listed grain per wave.  Disjoint data:
-// positions, and fix flags and vertex edit records.
-__global__ void k_translate_clear(Dev d) {
+// stage 2, no areas) in one launch: the first `nclear` workgroups (in
+// dispatch order, so that their chains of loads start first) clear, one
+// listed grain per wave; the others translate table blocks [0, nvb).
+// Disjoint data: positions, and fix flags and vertex edit records.
+__global__ void k_translate_clear(Dev d, int nclear) {
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    if (lb < d.nvb) {
-        translate_block(d, lb);
+    if (lb >= nclear) {
+        translate_block(d, lb - nclear);
         return;
     }
     const int n = d.cnt[C_FIX0 + 2];
-    const int lane = (int)threadIdx.x & 63;
-    const int nw = ((int)gridDim.x - d.nvb) * (int)(blockDim.x >> 6);
-    for (int item = (lb - d.nvb) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6); item < n; item += nw) {
+    const int lane = (int)threadIdx.x & 63, wpb = (int)(blockDim.x >> 6);
+    for (int item = lb * wpb + (int)(threadIdx.x >> 6); item < n; item += nclear * wpb) {
         const int g = d.l_fix[2][item];
         if (lane == 0) d.gfix[g] = 0;
         if (!d.genab[g]) continue;
