@@ -1315,14 +1315,13 @@ __global__ void k_text(Dev d) {
 // label_boundaries, topological.h:343-353 (every slot, quirk A-6), and the
 // compaction of enabled candidates.
 __global__ void k_label(Dev d) {
-    __shared__ int bcnt, bbase;
+    __shared__ int wcnt[kBlock / 64], bbase;
     TABLE_BLOCK(d.eblk, d.neb)
     // the edge's data is loaded together with the member's state
     const bool in = i < bd_.e;
     const double te = in ? d.bt[i] : 0.0, len = in ? d.blen[i] : 0.0;
     const uint8_t en = in ? d.benab[i] : 0;
     if (!M.active || !P.events) return;             // the same for the whole workgroup
-    if (threadIdx.x == 0) bcnt = 0;
     const double dt = member_dt(P, M);
     const bool c = in && is_candidate(te, len, dt, P.flip_thr);
     if (in) {
@@ -1334,16 +1333,20 @@ __global__ void k_label(Dev d) {
     // on the counter)
     const bool cand = c && en;
     const unsigned long long bal = __ballot(cand);
-    const int lane = (int)(threadIdx.x & 63), leader = bal ? __ffsll((long long)bal) - 1 : 0;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) wcnt[wave] = (int)__popcll(bal);
     __syncthreads();
-    int woff = 0;
-    if (bal && lane == leader) woff = atomicAdd(&bcnt, (int)__popcll(bal));
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) {
+        const int x = wcnt[w];
+        before += w < wave ? x : 0;
+        total += x;
+    }
+    if (threadIdx.x == 0 && total > 0) bbase = atomicAdd(&d.cnt[C_CAND], total);
     __syncthreads();
-    if (threadIdx.x == 0) bbase = bcnt > 0 ? atomicAdd(&d.cnt[C_CAND], bcnt) : 0;
-    __syncthreads();
-    woff = __builtin_amdgcn_readlane(woff, leader);
     if (cand) {
-        const int k = bbase + woff + (int)__popcll(bal & ((1ull << lane) - 1ull));
+        const int k = bbase + before + (int)__popcll(bal & ((1ull << lane) - 1ull));
         d.l_cand[k] = i;
         d.conf[i] = k;          // list position, for k_poll_gather
     }
