@@ -236,7 +236,10 @@ int count_grid(const Dev& d) { return (d.ngb + d.neb + d.nvb + kCountWaves - 1) 
 constexpr int kFixGrid = 2048;
 // one wave per nucleation event
 constexpr int kNuclInsGrid = 256;
-constexpr int kClearGrid = kFixGrid / 4; // workgroups of k_translate_clear that clear (4 waves each)
+// the flip stage flags four grains per flip: a step with thousands of
+// flips (the Monte Carlo workload) flags tens of thousands
+constexpr int kFlipFixGrid = 8192;
+constexpr int kClearGrid = kFixGrid / 4;  // workgroups of k_translate_clear that clear (4 waves each; more cost the plain step)
 constexpr int kTileMin = 16;             // smallest tile the autotuner tries
 
 // Tile table of velocity pass 1: T consecutive grains of one member per
@@ -413,7 +416,7 @@ void launch_step(sevc_state* s, bool diag) {
         }
         launch(s, "flip_par", k_flip_par, kListGrid, 256, d);
         launch(s, "flip_serial", k_flip_serial, 1, 64, d);
-        launch(s, "fix", k_fix, kFixGrid, 64, d, 2);
+        launch(s, "fix", k_fix, kFlipFixGrid, 64, d, 2);
         if (s->any_debug) {
             launch(s, "fix_clear", k_fix_clear, kFixGrid, 64, d, 2, 0);
             launch_check(s, CHK_FLIP);
