@@ -173,14 +173,17 @@ __device__ __forceinline__ double grain_area_at(const Dev& d, const double* X, i
 
 __device__ __forceinline__ double grain_area(const Dev& d, int g, double D) { return grain_area_at(d, d.pos, g, D); }
 
-// geometry.h:369-376
+// geometry.h:369-376, as three 16-byte stores (grn, vrt, prev) and one
+// 4-byte store (the four fix bytes) into the vertex's 64-byte record
 __device__ __forceinline__ void reset_mods(const Dev& d, int v) {
-    for (int j = 0; j < 4; j++) {
-        d.mods[v].fix[j] = FIX_NONE;
-        d.mods[v].grn[j] = -1;
-        d.mods[v].vrt[j] = -1;
-        d.mods[v].prev[j] = -1;
-    }
+    static_assert(offsetof(ModRec, grn) == 0 && offsetof(ModRec, vrt) == 16 && offsetof(ModRec, prev) == 32 &&
+                      offsetof(ModRec, fix) == 48,
+                  "record layout");
+    int4* r = reinterpret_cast<int4*>(&d.mods[v]);
+    r[0] = make_int4(-1, -1, -1, -1);
+    r[1] = make_int4(-1, -1, -1, -1);
+    r[2] = make_int4(-1, -1, -1, -1);
+    *reinterpret_cast<unsigned*>(d.mods[v].fix) = 0x01010101u * (unsigned)FIX_NONE;
 }
 
 __device__ __forceinline__ void set_mod(const Dev& d, int v, int slot, int fix, int grn, int vrt) {

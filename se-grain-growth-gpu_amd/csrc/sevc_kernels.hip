@@ -1225,8 +1225,8 @@ __global__ void k_translate(Dev d) {
 
 // k_translate and the clearing after the flip stage's fix-up (k_fix_clear,
 // stage 2, no areas) in one launch: the first `nclear` workgroups (in
-// dispatch order, so that their chains of loads start first) clear, one
-// listed grain per wave; the others translate table blocks [0, nvb).
+// dispatch order, so that their chains of loads start first) clear, eight
+// listed grains per wave; the others translate table blocks [0, nvb).
 // Disjoint data: positions, and fix flags and vertex edit records.
 __global__ void k_translate_clear(Dev d, int nclear) {
     const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1234,15 +1234,19 @@ __global__ void k_translate_clear(Dev d, int nclear) {
         translate_block(d, lb - nclear);
         return;
     }
+    // eight listed grains per wave, eight lanes each
     const int n = d.cnt[C_FIX0 + 2];
     const int lane = (int)threadIdx.x & 63, wpb = (int)(blockDim.x >> 6);
-    for (int item = lb * wpb + (int)(threadIdx.x >> 6); item < n; item += nclear * wpb) {
+    const int sub = lane >> 3, k = lane & 7;
+    for (int base = 8 * (lb * wpb + (int)(threadIdx.x >> 6)); base < n; base += 8 * nclear * wpb) {
+        const int item = base + sub;
+        if (item >= n) continue;
         const int g = d.l_fix[2][item];
-        if (lane == 0) d.gfix[g] = 0;
+        if (k == 0) d.gfix[g] = 0;
         if (!d.genab[g]) continue;
         const int* L = glist(d, g);
         const int m = d.glen[g];
-        for (int i = lane; i < m; i += 64) reset_mods(d, L[i]);
+        for (int i = k; i < m; i += 8) reset_mods(d, L[i]);
     }
 }
 
