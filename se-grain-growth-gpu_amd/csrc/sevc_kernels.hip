@@ -1573,9 +1573,38 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         for (;;) {
             if (tid == 0) changed = 0;
             unsigned two = 0;
-            for (int u = 0, k = tid; k < nc; u++, k += nt) {
-                const int bk = G[24 * kPollBig + k];
-                if (cf[k] && votes_self(0, k, bk) && votes_self(1, k, bk)) two |= 1u << u;
+            // four of the lane's candidates at a time: their mapped
+            // neighbours (columns 0-5) and bits read together, then the
+            // votes from LDS
+            constexpr int kChunk = 4;
+            for (int u0 = 0; tid + u0 * nt < nc; u0 += kChunk) {
+                int bk[kChunk], pp[kChunk][6];
+#pragma unroll
+                for (int q = 0; q < kChunk; q++) {
+                    const int k = tid + (u0 + q) * nt;
+                    const bool on = k < nc;
+                    bk[q] = on ? G[24 * kPollBig + k] : 0;
+#pragma unroll
+                    for (int c = 0; c < 6; c++) pp[q][c] = on ? G[c * kPollBig + k] : -1;
+                }
+#pragma unroll
+                for (int q = 0; q < kChunk; q++) {
+                    const int k = tid + (u0 + q) * nt;
+                    if (k >= nc || !cf[k]) continue;
+                    bool both = true;
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        double min_t = 99999999;
+                        int arg = -1;
+#pragma unroll
+                        for (int r = 0; r < 3; r++) {
+                            const int p = pp[q][3 * e + r];
+                            if (p >= 0 && cf[p] && ct[p] <= min_t) { min_t = ct[p]; arg = p; }
+                        }
+                        both = both && (arg >= 0 ? arg == k : ((bk[q] >> e) & 1) != 0);
+                    }
+                    if (both) two |= 1u << (u0 + q);
+                }
             }
             __syncthreads();
             for (int u = 0, k = tid; k < nc; u++, k += nt)
@@ -1595,18 +1624,22 @@ __global__ void __launch_bounds__(1024) k_poll(Dev d) {
         for (int u = 0, k = tid; k < nc; u++, k += nt) {
             bool kp = cf[k] != 0;
             if (kp) {
+                // every column of the candidate in one round of loads
                 const int bk = G[24 * kPollBig + k];
-                int ep[6];
+                int ep[6], fw[18];
 #pragma unroll
                 for (int j = 0; j < 6; j++) ep[j] = G[j * kPollBig + k];
-                for (int j = 0; j < 6 && kp; j++) {
-                    if (!((bk >> (2 + j)) & 1) || flagp(ep[j])) continue;
+#pragma unroll
+                for (int x = 0; x < 18; x++) fw[x] = G[(6 + x) * kPollBig + k];
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    if (!kp || !((bk >> (2 + j)) & 1) || flagp(ep[j])) continue;
                     int pbi = -1, pbe = -1;
                     const int e3 = 3 * (j / 3);
 #pragma unroll
                     for (int r = 2; r >= 0; r--) {           // first in slot order
                         if (flagp(ep[e3 + r])) pbi = ep[e3 + r];
-                        const int pw = G[(6 + 3 * j + r) * kPollBig + k];
+                        const int pw = fw[3 * j + r];
                         if (flagp(pw)) pbe = pw;
                     }
                     if (pbi < 0 || pbe < 0) continue;
