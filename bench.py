@@ -553,7 +553,7 @@ def cpu_baseline(opts, mesh, steps, omp=False):
                       f"(gcc -O2 -ffp-contract=off, {how}), {dt:.1f} s", **host_cpu()}
 
 
-def bench_config1(sevc, seed=0):
+def bench_half_removal(sevc, seed=0):
     """The 2,000-grain workload run to its stop (50 % of the grains
     removed): the reference's 2K tessellation, SE = 0.5 U[0,1), rate
     nucleation 1e-3 SE per grain and step, adaptive dt, flip threshold
@@ -588,7 +588,7 @@ def bench_config1(sevc, seed=0):
             and s.n_grains == s2.n_grains)
     orc.close()
     st.close()
-    log(f"config 1 (2K grains to 50 %): {s.n} steps, GPU {t_gpu:.3f} s ({t_graph:.3f} s as graph replays), "
+    log(f"2K grains to 50 % removal: {s.n} steps, GPU {t_gpu:.3f} s ({t_graph:.3f} s as graph replays), "
         f"oracle {t_cpu:.2f} s ({t_cpu / t_gpu:.1f}x), same end state counts: {same and same_graph}")
     return {"steps": s.n, "vertex_steps": s.n_vertex_steps, "flips": s.n_flips, "removals": s.n_removals,
             "nucleations": s.n_nucleations, "grains_left": s.n_grains, "gpu_s": t_gpu,
@@ -626,7 +626,7 @@ def main():
     ap.add_argument("--grains", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for cpu_baseline (0: skip)")
-    ap.add_argument("--config1", type=int, default=1, help="time the 2K workload to its stop, GPU and oracle (0: skip)")
+    ap.add_argument("--half-removal", type=int, default=1, help="time the 2K workload to its stop, GPU and oracle (0: skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the step as a HIP graph (no per-kernel timing)")
     ap.add_argument("--graph-steps", type=int, default=200, help="extra steps timed as HIP graph replays (0: skip)")
     ap.add_argument("--scale-steps", type=int, default=200,
@@ -757,8 +757,8 @@ def main():
     if args.calib_members > 0:
         line["calibration"] = bench_calibration(sevc, args.calib_members, args.calib_iters, args.calib_steps,
                                                 args.calib_window, 50, 5e-7)
-    if args.config1:
-        line["config1"] = bench_config1(sevc)
+    if args.half_removal:
+        line["half_removal_2k"] = bench_half_removal(sevc)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(opts, mesh, args.cpu_steps)
         line["cpu_baseline_openmp"] = cpu_baseline(opts, mesh, 2 * args.cpu_steps, omp=True)

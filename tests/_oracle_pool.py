@@ -46,7 +46,7 @@ def _init():
             sys.path.insert(0, p)
 
 
-def config1_options(seed):
+def half_removal_options(seed):
     """The 2,000-grain workload of the small-scale runs: the reference's 2K
     tessellation, gamma = mu = 1, SE = U[0,1) * 0.5 (uniform.txt x SE_eps),
     rate nucleation 1e-3 * SE per grain and step with Philox seed `seed`,
@@ -60,11 +60,11 @@ def config1_options(seed):
                          seed=seed, extra_capacity=0.05, leave_percent=0.5)
 
 
-def _config1_member(seed, chunk, max_chunks):
+def _half_removal_member(seed, chunk, max_chunks):
     _init()
     import sevc
     from _util import Oracle
-    o = config1_options(seed)
+    o = half_removal_options(seed)
     orc = Oracle(o, sevc.load_files(o))
     out = []
     for _ in range(max_chunks):
@@ -94,9 +94,9 @@ def pool(n=None):
     return mp.get_context("spawn").Pool(n, initializer=_init)
 
 
-def config1_runs(seeds, chunk, max_chunks):
+def half_removal_runs(seeds, chunk, max_chunks):
     with pool() as p:
-        return p.starmap(_config1_member, [(s, chunk, max_chunks) for s in seeds])
+        return p.starmap(_half_removal_member, [(s, chunk, max_chunks) for s in seeds])
 
 
 def ensemble_runs(members, steps):

@@ -325,7 +325,7 @@ def test_list_strides_reference_and_largest():
     st.close()
 
 
-def _config1(seed, dt=2e-4):
+def _half_removal_2k(seed, dt=2e-4):
     """2,000-grain fixture with the model options of the small-scale runs:
     gamma = mu = 1, SE = U[0,1) * 0.5, rate nucleation 1e-3 * SE per grain
     and step with Philox seed `seed`, adaptive dt (upper bound dt), flip
@@ -340,7 +340,7 @@ def test_seed_ensemble_curves_match_oracle():
     member's state, grain count and mean grain area equal its own oracle
     run after each chunk (so the ensemble curves agree exactly)."""
     seeds = list(range(32))
-    opts = [_config1(s) for s in seeds]
+    opts = [_half_removal_2k(s) for s in seeds]
     meshes = [sevc.load_files(o) for o in opts]
     st = sevc.State(opts, meshes)
     orcs = [Oracle(o, m) for o, m in zip(opts, meshes)]

@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 import sevc
-from _oracle_pool import STAT_KEYS, config1_options, config1_runs, ensemble_runs, record
+from _oracle_pool import STAT_KEYS, half_removal_options, half_removal_runs, ensemble_runs, record
 from _util import Oracle, REPO, diff_fields
 
 pytestmark = pytest.mark.gpu
@@ -31,13 +31,13 @@ pytestmark = pytest.mark.gpu
 sys.path.insert(0, REPO)
 
 
-def test_config1_32_seeds_to_half_the_grains():
+def test_32_seeds_of_2k_grains_to_half_the_grains():
     seeds = list(range(32))
     chunk, max_chunks = 100, 30
     t0 = time.time()
-    ref = config1_runs(seeds, chunk, max_chunks)
+    ref = half_removal_runs(seeds, chunk, max_chunks)
     print(f"  oracle runs: {time.time() - t0:.1f} s, chunks per member {[len(r) for r in ref]}", flush=True)
-    opts = [config1_options(s) for s in seeds]
+    opts = [half_removal_options(s) for s in seeds]
     st = sevc.State(opts, [sevc.load_files(o) for o in opts])
     done = [False] * len(seeds)
     for c in range(max_chunks):
@@ -63,7 +63,7 @@ def test_config1_32_seeds_to_half_the_grains():
     st.close()
 
 
-def test_config3_ensemble_256_members_three_steps():
+def test_ensemble_256_members_three_steps():
     import bench
     t0 = time.time()
     ref = ensemble_runs(256, 3)
@@ -83,7 +83,7 @@ def test_config3_ensemble_256_members_three_steps():
     st.close()
 
 
-def test_config4_adjoint_window_10k_500_steps():
+def test_adjoint_window_10k_grains_500_steps():
     n, steps, ckpt = 10000, 500, 50
     pos, rid = sevc.voronoi(n, 3)
     rng = np.random.default_rng(3)
