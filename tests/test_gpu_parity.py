@@ -163,6 +163,31 @@ def _generated(n, gen_seed, **kw):
     return o, sevc.build_mesh(o, pos, rid, rng.random(n), se, n)
 
 
+@pytest.mark.parametrize("path,tile", [(0, 64), (1, 64), (2, 0)])
+def test_vertex_on_the_domain_edge(path, tile):
+    """Vertices whose coordinate dom_mod moves (a coordinate stored as D
+    itself, which an addition of D to a tiny negative value rounds to) take
+    the reference's two energy_diff passes (calculus.h:19-73) in
+    k_velocity_fallback, listed by pass 2 or by the one-pass tiles.  Every
+    path equals the oracle."""
+    o = ref2k_options(dt=1e-4)
+    mesh = sevc.load_files(o)
+    D = o.domain_bound
+    pos = mesh["pos"]
+    pos[2 * 5] = D                        # x of vertex 5
+    pos[2 * 40 + 1] = D                   # y of vertex 40
+    pos[2 * 77] = D
+    pos[2 * 77 + 1] = D
+    st, orc = sevc.State(o, mesh), Oracle(o, mesh)
+    st.set_velocity_config(path, tile)
+    for chunk in (1, 1, 5):
+        s1 = st.step(chunk)
+        rc, s2 = orc.step(chunk)
+        assert rc == 0 and s1.status == 0
+        assert diff_fields(st.get_mesh(), orc.get_mesh()) == {}
+    st.close()
+
+
 def test_rate_nucleation_adaptive_threshold():
     """The benchmark's model options (rate nucleation, adaptive dt, flip
     threshold) on a 3,000-grain tessellation."""
