@@ -426,7 +426,7 @@ void launch_step(sevc_state* s, bool diag) {
         launch(s, "translate", k_translate_clear, kClearGrid + d.nvb, kBlock, d, kClearGrid);
     else
         launch(s, "translate", k_translate, d.nvb, kBlock, d);
-    launch(s, "count_all", k_count_all, count_grid(d), 64 * kCountWaves, d);
+    // the per-block slot counts are kept by the events themselves
     launch(s, "member_reduce", k_member_reduce, s->B, 1024, d);
 }
 

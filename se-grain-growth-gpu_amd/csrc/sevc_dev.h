@@ -224,6 +224,54 @@ __device__ __forceinline__ void wave_count(const Dev& d, bool want, int mem, lon
     }
 }
 
+// The per-table-block (enabled, free) slot counts follow every event that
+// enables or disables a slot (k_count_all recounts them only after a host
+// upload): one 64-bit atomic adds (+1, -1) or (-1, +1) to the pair.
+__device__ __forceinline__ void slot_count_add(int2* bc, int blk, bool enable) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&bc[blk]),
+              enable ? 0xFFFFFFFF00000001ull : 0x00000000FFFFFFFFull);
+}
+
+// Table block of a slot: members own consecutive runs of kBlock slots from
+// their offsets.
+__device__ __forceinline__ int vertex_block(const Dev& d, int v) {
+    if (d.B == 1) return v / kBlock;
+    const int m = d.vmem[v];
+    return d.vblk_first[m] + (v - d.prm[m].voff) / kBlock;
+}
+__device__ __forceinline__ int edge_block(const Dev& d, int e) {
+    if (d.B == 1) return e / kBlock;
+    const int m = d.emem[e];
+    return d.eblk_first[m] + (e - d.prm[m].eoff) / kBlock;
+}
+__device__ __forceinline__ int grain_block(const Dev& d, int g) {
+    if (d.B == 1) return g / kBlock;
+    const int m = d.gmem[g];
+    return d.gblk_first[m] + (g - d.prm[m].goff) / kBlock;
+}
+
+// Set an enable flag, counting a change of state.  No other thread writes
+// the same flag in the same launch (events that touch a slot are serial or
+// disjoint), so the read needs no atomic.
+__device__ __forceinline__ void set_vertex_enabled(const Dev& d, int v, bool on) {
+    if ((d.ven[v] != 0) != on) {
+        d.ven[v] = on ? 1 : 0;
+        slot_count_add(d.bc_v, vertex_block(d, v), on);
+    }
+}
+__device__ __forceinline__ void set_edge_enabled(const Dev& d, int e, bool on) {
+    if ((d.benab[e] != 0) != on) {
+        d.benab[e] = on ? 1 : 0;
+        slot_count_add(d.bc_e, edge_block(d, e), on);
+    }
+}
+__device__ __forceinline__ void set_grain_enabled(const Dev& d, int g, bool on) {
+    if ((d.genab[g] != 0) != on) {
+        d.genab[g] = on ? 1 : 0;
+        slot_count_add(d.bc_g, grain_block(d, g), on);
+    }
+}
+
 // Flag grain g for the fix stage `stage` (grain.fix = true) and list it once.
 __device__ __forceinline__ void flag_grain(const Dev& d, int g, int stage) {
     const bool first = atomicExch(&d.gfix[g], 1) == 0;

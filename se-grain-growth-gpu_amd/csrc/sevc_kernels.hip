@@ -76,11 +76,13 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
 }
 
 // Per table block, for grains, edges and vertices in one launch: the
-// numbers of enabled and of free slots.  The enabled counts feed the
-// grain / vertex totals of main.cu:317 (one partial per table block, summed
-// per member by k_member_reduce); the pairs also serve the next step's
-// nucleation (free-slot search, alive-vertex draw), since nothing enables
-// or disables a slot between the end of a step and nucleation.  One wave
+// numbers of enabled and of free slots, after the host uploads a state.
+// The steps keep them from then on (each event that enables or disables a
+// slot adds to its block's pair: set_*_enabled).  The enabled counts feed
+// the grain / vertex totals of main.cu:317 (one partial per table block,
+// summed per member by k_member_reduce); the pairs also serve the
+// nucleation (free-slot search, alive-vertex draw), which reads them before
+// the step's insertions.  One wave
 // per table block: each lane reads four flags 64 bytes apart (four
 // coalesced 64-byte loads per wave) and the wave counts them by ballots.
 __global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
@@ -2040,12 +2042,13 @@ __device__ int remove_two_grains(const Dev& d, int bnd, const int gnic[2]) {
     set_mod(d, B, 1, FIX_REMOVE, G4, C);
     set_mod(d, B, 2, FIX_REMOVE, G4, Dv);
     for (int i = 0; i < 3; i++) {
-        d.benab[d.vb[3 * A + i]] = 0;
-        d.benab[d.vb[3 * B + i]] = 0;
+        set_edge_enabled(d, d.vb[3 * A + i], false);
+        set_edge_enabled(d, d.vb[3 * B + i], false);
     }
-    d.benab[DF] = 0;
-    d.benab[bnd] = 0;
-    d.ven[A] = 0; d.ven[Dv] = 0; d.ven[C] = 0; d.ven[B] = 0;
+    set_edge_enabled(d, DF, false);
+    set_edge_enabled(d, bnd, false);
+    set_vertex_enabled(d, A, false); set_vertex_enabled(d, Dv, false);
+    set_vertex_enabled(d, C, false); set_vertex_enabled(d, B, false);
     return CE;
 }
 
@@ -2079,8 +2082,8 @@ __global__ void k_two_grain(Dev d) {
             grains_symdiff(d, d.bvx[2 * e], d.bvx[2 * e + 1], gnic);
             if (gnic[0] < 0 || gnic[1] < 0) { set_error(d, member_of_edge(d, e), SEVC_E_STATE); continue; }
             int CE = remove_two_grains(d, e, gnic);
-            d.genab[gic[0]] = 0;
-            d.genab[gic[1]] = 0;
+            set_grain_enabled(d, gic[0], false);
+            set_grain_enabled(d, gic[1], false);
             d.ms[member_of_edge(d, e)].two += 1;
             // the T2 labels k_scan3 took before this stage are redone
             d.cnt[C_RELABEL] = 1;
@@ -2358,7 +2361,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const int ei = xC[i][0], ee = xC[i][1];
-        if (!found && ((ei == B && ee == C) || (ee == B && ei == C))) { d.benab[vC[i]] = 0; found = true; }
+        if (!found && ((ei == B && ee == C) || (ee == B && ei == C))) { set_edge_enabled(d, vC[i], false); found = true; }
     }
     int BEb = -1, CF = -1, be0 = -1, be1 = -1, cf0 = -1, cf1 = -1;
 #pragma unroll
@@ -2395,7 +2398,7 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
         int e0, e1;
         ends(curA[i], i, e0, e1);
         if (!found && (e0 == B || e1 == B)) {
-            d.benab[curA[i]] = 0; curA[i] = BEb; d.vb[3 * A + i] = BEb; found = true;
+            set_edge_enabled(d, curA[i], false); curA[i] = BEb; d.vb[3 * A + i] = BEb; found = true;
         }
     }
     found = false;
@@ -2404,11 +2407,11 @@ __device__ void remove_grain(const Dev& d, int g, double D) {
         int e0, e1;
         if (curA[i] == BEb) { e0 = be0; e1 = be1; }
         else ends(curA[i], i, e0, e1);
-        if (!found && (e0 == C || e1 == C)) { d.benab[curA[i]] = 0; d.vb[3 * A + i] = CF; found = true; }
+        if (!found && (e0 == C || e1 == C)) { set_edge_enabled(d, curA[i], false); d.vb[3 * A + i] = CF; found = true; }
     }
-    d.ven[B] = 0;
-    d.ven[C] = 0;
-    d.genab[g] = 0;
+    set_vertex_enabled(d, B, false);
+    set_vertex_enabled(d, C, false);
+    set_grain_enabled(d, g, false);
     set_mod(d, B, 0, FIX_REMOVE, G2, B); flag_grain(d, G2, 1);
     set_mod(d, C, 0, FIX_REMOVE, G1, C); flag_grain(d, G1, 1);
     set_mod(d, B, 1, FIX_REMOVE, G3, B); flag_grain(d, G3, 1);
@@ -3559,7 +3562,7 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
                 d.pos[2 * v + 1] = q == 0 ? r.Bpos[1] : r.Cpos[1];
                 d.vel[2 * v] = 0.0; d.vel[2 * v + 1] = 0.0;
                 d.venergy[v] = 0.0;
-                d.ven[v] = 1;
+                set_vertex_enabled(d, v, true);
                 d.vnuc[v] = 0;
                 reset_mods(d, v);
             }
@@ -3577,7 +3580,7 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
                 d.bvx[2 * b + 1] = q == 0 ? B : (q == 1 ? C : A);
                 d.blen[b] = 0;
                 d.bt[b] = SEVC_MAX_T_EXT;
-                d.benab[b] = 1;
+                set_edge_enabled(d, b, true);
                 d.bcand[b] = 0;
                 d.conf[b] = -1;
             }
@@ -3588,7 +3591,7 @@ __global__ void __launch_bounds__(64) k_nucl_insert(Dev d) {
             d.glen[g] = 3;
             d.garea[g] = 0;
             d.gse[g] = 0;
-            d.genab[g] = 1;
+            set_grain_enabled(d, g, true);
             d.gcand[g] = 0;
             d.gfix[g] = 0;
             d.gtype[g] = T_NUCLEATED;
