@@ -3046,7 +3046,7 @@ __device__ __forceinline__ void read_free_rounds(const SlotKind (&K)[3], int r, 
 // (one while a kind has at most kFreeRunOf nt blocks) every lane holds one
 // free count per row of each kind (read_free_rows; y holds round 0 on
 // entry, read by the caller ahead of time).  The waves scan each row
-// (shuffles), waves 0-2 scan the (row, wave) totals of one kind each in
+// (wave_inclusive_scan), waves 0-2 scan the (row, wave) totals of one kind each in
 // block order, and every lane then knows the first rank of each of its
 // blocks: it lists (kind, block, first rank, free count) for those that
 // hold a wanted rank in `bl` (LDS when the caller's bound on the entries
