@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.join(REPO, "se-grain-growth-gpu_amd"))
 
 METRIC = "FP64 vertex-steps/s at 1M grains & 256-run batch vs plain; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
-TIMING_SAMPLE = 8                # the timed run records kernel events on one step in 8
+TIMING_SAMPLE = 16               # the timed run records kernel events on one step in 16
 # Algorithmic bytes per enabled vertex and step (FP64 values, int32 ids,
 # each datum once): velocity 98 B, translation 48 B.
 B_VELOCITY = 98.0
