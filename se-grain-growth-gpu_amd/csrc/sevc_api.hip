@@ -50,7 +50,7 @@ __global__ void k_flip_par(Dev d);
 __global__ void k_flip_serial(Dev d);
 __global__ void k_nucl_area_l1(Dev d);
 __global__ void k_nucl_area_up(Dev d);
-__global__ void k_rate_cand(Dev d);
+__global__ void k_begin_rate(Dev d, int rescale_flipped);
 __global__ void k_nucl_manager(Dev d);
 __global__ void k_nucl_insert(Dev d);
 // kernels of sevc_adjoint.hip
@@ -375,13 +375,16 @@ void launch_step(sevc_state* s, bool diag) {
     // step after creation or after a new scale (for the members active then,
     // as the reference's per-step pass would).
     const bool all_energies = diag || s->any_aniso || s->ben_dirty;
-    launch(s, "begin", k_begin, 1, 1024, d, all_energies ? 0 : 1);
+    // the rate mode's candidate draw runs in the begin's launch
+    if (s->any_nucl && s->any_rate)
+        launch(s, "begin", k_begin_rate, 1 + d.ngb, kBlock, d, all_energies ? 0 : 1);
+    else
+        launch(s, "begin", k_begin, 1, 1024, d, all_energies ? 0 : 1);
     if (all_energies) launch(s, "edge_len_energy", k_edge_len_energy, d.neb, kBlock, d, 0);
     s->ben_dirty = false;
     if (diag || s->any_analytic) launch(s, "grain_area", k_grain_area, d.ngb, kBlock, d, 0);
     if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
     if (s->any_nucl) {
-        if (s->any_rate) launch(s, "rate_cand", k_rate_cand, d.ngb, kBlock, d);
         launch(s, "nucl_manager", k_nucl_manager, s->B, kNuclThreads, d);
         launch(s, "nucl_insert", k_nucl_insert, kNuclInsGrid, 64, d);
         if (diag) launch(s, "vertex_energy", k_vertex_energy, d.nvb, kBlock, d);
@@ -1051,7 +1054,10 @@ int sevc_autotune(sevc_state* s, int32_t* tile_grains) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const bool back = hipMemcpyAsync(s->d.vel, keep_vel, nvel * sizeof(double), hipMemcpyDeviceToDevice,
+    // the timing launches of the lane-per-grain pass listed long grains: the
+    // next step starts from an empty list
+    const bool back = hipMemsetAsync(s->d.cnt + C_BIGR, 0, sizeof(int), s->stream) == hipSuccess &&
+                      hipMemcpyAsync(s->d.vel, keep_vel, nvel * sizeof(double), hipMemcpyDeviceToDevice,
                                      s->stream) == hipSuccess &&
                       hipMemcpyAsync(s->d.garea, keep_area, narea * sizeof(double), hipMemcpyDeviceToDevice,
                                      s->stream) == hipSuccess &&

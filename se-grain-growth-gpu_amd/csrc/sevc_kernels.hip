@@ -28,21 +28,35 @@ namespace sevc {
 // step begin / end
 // ---------------------------------------------------------------------
 
+// Whether a member runs the coming step: the loop condition of main.cu:120
+// (time rule) and the grain rule, from the loop state before the step.
+__device__ __forceinline__ bool member_runs(const Params& P, const MState& M) {
+    if (M.done) return false;
+    const bool tok = (P.tmax <= 0 && P.nmax <= 0) || (M.t <= P.tmax && P.tmax > 0) || (M.n <= P.nmax && P.nmax > 0);
+    const bool gok = (P.min_grains <= 0) || (M.curr_grains >= P.min_grains);
+    return tok && gok;
+}
+
+// Whether a rate-mode member nucleates in the coming step (every gap-th).
+__device__ __forceinline__ bool rate_nucleates(const Params& P, const MState& M) {
+    if (!(P.events && P.do_nucl) || P.nucl_mode != SEVC_NUCL_RATE) return false;
+    const int gap = P.gap > 0 ? P.gap : 1;
+    return (M.n % gap) == 0;
+}
+
 // Loop condition of main.cu:120 (time rule), grain rule, nucleation gate
-// main.cu:137-141; clears the per-step counters.
-__global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
+// main.cu:137-141, for every member, by one workgroup.  The per-step
+// counters were cleared at the end of the previous step (k_member_reduce)
+// or at set-up.
+__device__ __forceinline__ void begin_members(const Dev& d, int rescale_flipped) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0)
-        for (int c = 0; c < C_COUNT; c++) d.cnt[c] = 0;
     for (int m = tid; m < d.B; m += nt) {
         const Params& P = d.prm[m];
         MState& M = d.ms[m];
         M.tau_bits = 0x7FF0000000000000ull;
         M.nucl_now = 0;
         if (M.done) { M.active = 0; continue; }
-        bool tok = (P.tmax <= 0 && P.nmax <= 0) || (M.t <= P.tmax && P.tmax > 0) || (M.n <= P.nmax && P.nmax > 0);
-        bool gok = (P.min_grains <= 0) || (M.curr_grains >= P.min_grains);
-        if (!tok || !gok) { M.active = 0; M.done = 1; continue; }
+        if (!member_runs(P, M)) { M.active = 0; M.done = 1; continue; }
         M.active = 1;
         M.vsteps += M.n_venabled;
         if (P.events && P.do_nucl) {
@@ -51,7 +65,7 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
                 if (M.curr_grains <= P.gate_grains) M.allow_nucl = 1;   // main.cu:137-139
                 M.nucl_now = ((M.n % gap) == 0 && M.allow_nucl) ? 1 : 0;
             } else {
-                M.nucl_now = ((M.n % gap) == 0) ? 1 : 0;
+                M.nucl_now = rate_nucleates(P, M) ? 1 : 0;
             }
         }
     }
@@ -73,6 +87,10 @@ __global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
     }
     __syncthreads();
     if (tid == 0) *d.n_flipped = 0;
+}
+
+__global__ void __launch_bounds__(1024) k_begin(Dev d, int rescale_flipped) {
+    begin_members(d, rescale_flipped);
 }
 
 // Per table block, for grains, edges and vertices in one launch: the
@@ -117,6 +135,8 @@ __global__ void __launch_bounds__(64 * kCountWaves) k_count_all(Dev d) {
 __global__ void __launch_bounds__(1024) k_member_reduce(Dev d) {
     __shared__ int sa[16], sb[16];
     const int m = blockIdx.x;
+    // the step's work lists are done with: clear their counters for the next step
+    if (m == 0 && (int)threadIdx.x < C_COUNT) d.cnt[threadIdx.x] = 0;
     if (m >= d.B) return;
     MState& M = d.ms[m];
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
@@ -2899,14 +2919,21 @@ __device__ void nucl_factor_at(const Dev& d, const Params& P, const MState& M, i
 // list entry floor(u1 * n) of the grain; the candidate's record holds the
 // site (-1 when its shortest boundary has zero length: no candidate), the
 // site's three grains and that shortest length.
-__global__ void k_rate_cand(Dev d) {
-    TABLE_BLOCK(d.gblk, d.ngb)
+// One grain table block of the rate mode's candidate draw.  It runs in the
+// launch of the step's begin (k_begin_rate), so it decides whether its
+// member runs and nucleates from the loop state itself, by the same rules
+// (member_runs, rate_nucleates) that the begin workgroup writes into it.
+__device__ __forceinline__ void rate_cand_block(const Dev& d, int lb) {
+    const Blk bd_ = table_blk(d, d.gblk, lb);
+    const Params& P = d.prm[bd_.m];
+    const MState& M = d.ms[bd_.m];
+    const int i = bd_.b + (int)threadIdx.x;
     // the grain's data is loaded together with the member's state
     const bool in = i < bd_.e;
     const uint8_t en = in ? d.genab[i] : 0;
     const int len = in ? d.glen[i] : 0;
     const double se = in ? d.gse[i] : 0.0;
-    if (!M.active || !M.nucl_now || P.nucl_mode != SEVC_NUCL_RATE) return;
+    if (!member_runs(P, M) || !rate_nucleates(P, M)) return;
     if (!en || len < 1) return;
     double p = P.nucl_rate * se;
     double u0, u1;
@@ -2929,6 +2956,17 @@ __global__ void k_rate_cand(Dev d) {
 #pragma unroll
     for (int q = 0; q < 3; q++) d.nrec[4 * s + 1 + q] = vg[q];
     d.nal[s] = minAL;
+}
+
+// The step's begin (workgroup 0, dispatched first) and the rate mode's
+// candidate draw (one workgroup per grain table block) in one launch.
+__global__ void __launch_bounds__(kBlock) k_begin_rate(Dev d, int rescale_flipped) {
+    if (blockIdx.x == 0) {
+        begin_members(d, rescale_flipped);
+        return;
+    }
+    const int lb = xcd_remap((int)blockIdx.x - 1, (int)gridDim.x - 1);
+    if (lb < d.ngb) rate_cand_block(d, lb);
 }
 
 // Exclusive prefix sum of one int per thread over the workgroup: an
